@@ -1,0 +1,774 @@
+// C ABI of the MI355X DG residual path: context construction (space tables, FP64 geometry,
+// connectivity codes), state transfer, the fused RK3 stepping loop and FP64 diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dgx.h"
+#include "dgx_internal.hpp"
+#include "dgx_space.hpp"
+
+using namespace dgx;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Fail : std::runtime_error {
+    dgx_status code;
+    Fail(dgx_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Fail(DGX_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+dgx_status guarded(F&& f) {
+    try {
+        f();
+        return DGX_OK;
+    } catch (const Fail& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return DGX_E_INVALID;
+    } catch (const std::bad_alloc&) {
+        g_error = "out of host memory";
+        return DGX_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return DGX_E_INVALID;
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t bytes) {
+        release();
+        if (bytes == 0) return;
+        hip_check(hipMalloc(&p, bytes), "hipMalloc");
+        n = bytes;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Inverse of a small dense SPD/general matrix by Gauss-Jordan with partial pivoting.
+std::vector<double> invert(std::vector<double> M, int n) {
+    std::vector<double> I(n * n, 0.0);
+    for (int i = 0; i < n; ++i) I[i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(M[r * n + c]) > std::fabs(M[piv * n + c])) piv = r;
+        if (std::fabs(M[piv * n + c]) < 1e-300) throw std::runtime_error("singular reference mass matrix");
+        if (piv != c)
+            for (int k = 0; k < n; ++k) { std::swap(M[c * n + k], M[piv * n + k]); std::swap(I[c * n + k], I[piv * n + k]); }
+        double d = 1.0 / M[c * n + c];
+        for (int k = 0; k < n; ++k) { M[c * n + k] *= d; I[c * n + k] *= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c || M[r * n + c] == 0.0) continue;
+            double f = M[r * n + c];
+            for (int k = 0; k < n; ++k) { M[r * n + k] -= f * M[c * n + k]; I[r * n + k] -= f * I[c * n + k]; }
+        }
+    }
+    return I;
+}
+
+}  // namespace
+
+struct dgx_ctx {
+    int K = 0, Kp = 0, p = 1, rule = 0, np = 3, nq = 3, nfq = 2, basis = 1, precision = 0, physics = 0;
+    double gamma = 1.4, mu = 0, prandtl = 0.72, sigma0 = 5;
+    uint32_t quirks = 0;
+    int device = 0;
+    int n_bfaces = 0;
+    size_t rsz = 8;
+    hipStream_t stream = nullptr;
+    DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
+    std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][3][2]
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~dgx_ctx() {
+        for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage, &partial})
+            b->release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t state_bytes() const { return (size_t)np * 4 * Kp * rsz; }
+    int red_blocks() const { return (K + 255) / 256; }
+
+    void upload(DevBuf& b, const void* src, size_t bytes) {
+        b.alloc(bytes);
+        hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    template <typename T>
+    void upload_real(DevBuf& b, const std::vector<double>& v) {
+        std::vector<T> t(v.begin(), v.end());
+        upload(b, t.data(), t.size() * sizeof(T));
+    }
+
+    void stage_launch(const void* uk, const void* un, void* out, int mode, double dt, double ca, double cb) {
+        StageParams sp{};
+        sp.uk = uk;
+        sp.un = un;
+        sp.out = out;
+        sp.geo = geo.as<double>();
+        sp.nbr = nbr.as<int32_t>();
+        sp.code = code.as<uint32_t>();
+        sp.tab = (mode == MODE_RESIDUAL) ? tabR.p : tabL.p;
+        sp.bc = bc.as<double>();
+        sp.K = K;
+        sp.Kp = Kp;
+        sp.dt = dt;
+        sp.ca = ca;
+        sp.cb = cb;
+        sp.gamma = gamma;
+        sp.mu = mu;
+        sp.prandtl = prandtl;
+        sp.sigma0 = sigma0;
+        sp.mode = mode;
+        sp.ns_face_quirk = (quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) ? 1 : 0;
+        hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
+    }
+
+    // One SSP-RK3 step (SSP_RK::step_rk3, I/solver/time_stepping.hpp:39-55):
+    //   u1 = u + dt L(u); u2 = 3/4 u + 1/4 (u1 + dt L(u1)); u = 1/3 u + 2/3 (u2 + dt L(u2)).
+    void rk3_step(double dt) {
+        stage_launch(u.p, u.p, u1.p, MODE_RK, dt, 0.0, 1.0);
+        stage_launch(u1.p, u.p, u2.p, MODE_RK, dt, 0.75, 0.25);
+        stage_launch(u2.p, u.p, u1.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
+        std::swap(u, u1);
+    }
+
+    std::vector<double> reduce(int kind) {
+        int nb = red_blocks();
+        hip_check(launch_reduce(kind, precision, p, rule, u.p, tab64.p, geo.as<double>(), hmin.as<double>(),
+                                qw.as<double>(), K, Kp, gamma, partial.as<double>(), nb, stream),
+                  "reduce kernel launch");
+        std::vector<double> h((size_t)nb * 4);
+        hip_check(hipMemcpyAsync(h.data(), partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "copy");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        return h;
+    }
+};
+
+namespace {
+
+void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
+    if (!m.vertices || !m.elements || !m.face_neighbors || m.n_elements <= 0 || m.n_vertices <= 0)
+        throw std::invalid_argument("mesh descriptor is incomplete");
+    if (s.order < 1 || s.order > 4) throw std::invalid_argument("Element order must be between 1 and 4");
+    if (s.basis != DGX_BASIS_MONOMIAL && s.basis != DGX_BASIS_DUBINER) throw std::invalid_argument("unknown basis");
+    if (s.rule != DGX_RULE_2P && s.rule != DGX_RULE_2P1) throw std::invalid_argument("unknown volume rule");
+    if (precision != DGX_FP64 && precision != DGX_FP32) {
+        if (precision == DGX_FP32_BF16VOL) throw Fail(DGX_E_UNSUPPORTED, "the BF16-input volume variant is not built");
+        throw std::invalid_argument("unknown precision");
+    }
+    if (ph.kind != DGX_EULER && ph.kind != DGX_NAVIER_STOKES) throw std::invalid_argument("unknown physics kind");
+    if (!(ph.gamma > 1.0)) throw std::invalid_argument("gamma must exceed 1");
+    if (ph.kind == DGX_NAVIER_STOKES) {  // NavierStokesWeakFormulation ctor (:18-34)
+        if (!(ph.mu > 0)) throw std::invalid_argument("Dynamic viscosity must be positive for Navier-Stokes weak formulation");
+        if (!(ph.prandtl > 0)) throw std::invalid_argument("Prandtl number must be positive for Navier-Stokes weak formulation");
+        if (!(ph.sigma0 > 0)) throw std::invalid_argument("Penalty prefactor must be positive for Navier-Stokes weak formulation");
+    }
+
+    c.K = m.n_elements;
+    c.Kp = (c.K + 255) / 256 * 256;
+    c.p = s.order;
+    c.rule = s.rule;
+    c.basis = s.basis;
+    c.precision = precision;
+    c.physics = ph.kind;
+    c.gamma = ph.gamma;
+    c.mu = ph.mu;
+    c.prandtl = ph.prandtl;
+    c.sigma0 = ph.sigma0;
+    c.quirks = ph.quirks;
+    c.rsz = precision == DGX_FP64 ? 8 : 4;
+
+    // ---------------- reference-element tables (FP64) ----------------
+    RuleTri vr = dunavant(2 * c.p + c.rule);
+    Rule1D fr = gauss_legendre(c.p + 1);
+    TriBasis B(c.basis, c.p);
+    const int np = B.np, nq = (int)vr.w.size(), nfq = (int)fr.w.size();
+    if (nq != nq_of(c.p, c.rule)) throw std::runtime_error("volume rule size mismatch");
+    c.np = np;
+    c.nq = nq;
+    c.nfq = nfq;
+    TabLayout L(c.p, c.rule);
+
+    std::vector<double> phiV(nq * np), gV(nq * np * 2);
+    for (int q = 0; q < nq; ++q) {
+        B.values(vr.xi[q], vr.eta[q], &phiV[q * np]);
+        B.gradients(vr.xi[q], vr.eta[q], &gV[q * np * 2]);
+    }
+    std::vector<double> Mref(np * np, 0.0);
+    for (int q = 0; q < nq; ++q)
+        for (int i = 0; i < np; ++i)
+            for (int j = 0; j < np; ++j) Mref[i * np + j] += vr.w[q] * phiV[q * np + i] * phiV[q * np + j];
+    std::vector<double> Mi = invert(Mref, np);
+    c.h_minv = Mi;
+    c.h_qw = vr.w;
+
+    std::vector<double> phiF(3 * nfq * np), gF(3 * nfq * np * 2);
+    for (int f = 0; f < 3; ++f)
+        for (int q = 0; q < nfq; ++q) {
+            double xi, eta;
+            face_point(f, fr.x[q], xi, eta);
+            B.values(xi, eta, &phiF[(f * nfq + q) * np]);
+            B.gradients(xi, eta, &gF[(f * nfq + q) * np * 2]);
+        }
+
+    auto fold = [&](const std::vector<double>& raw, int rows, std::vector<double>& outv, int off) {
+        // out[r][i] = sum_j Mi[i][j] raw[r][j]
+        for (int r = 0; r < rows; ++r)
+            for (int i = 0; i < np; ++i) {
+                double sacc = 0.0;
+                for (int j = 0; j < np; ++j) sacc += Mi[i * np + j] * raw[r * np + j];
+                outv[off + r * np + i] = sacc;
+            }
+    };
+    std::vector<double> tR(L.total, 0.0), tL(L.total, 0.0);
+    std::vector<double> px(nq * np), py(nq * np), lf(3 * nfq * np);
+    for (int q = 0; q < nq; ++q)
+        for (int i = 0; i < np; ++i) {
+            px[q * np + i] = vr.w[q] * gV[(q * np + i) * 2];
+            py[q * np + i] = vr.w[q] * gV[(q * np + i) * 2 + 1];
+        }
+    for (int f = 0; f < 3; ++f)
+        for (int q = 0; q < nfq; ++q)
+            for (int i = 0; i < np; ++i) lf[(f * nfq + q) * np + i] = -0.5 * fr.w[q] * phiF[(f * nfq + q) * np + i];
+    for (auto* t : {&tR, &tL}) {
+        std::copy(phiV.begin(), phiV.end(), t->begin() + L.phiV);
+        std::copy(phiF.begin(), phiF.end(), t->begin() + L.phiF);
+        for (int q = 0; q < nq; ++q)
+            for (int i = 0; i < np; ++i) {
+                (*t)[L.DXV + q * np + i] = gV[(q * np + i) * 2];
+                (*t)[L.DYV + q * np + i] = gV[(q * np + i) * 2 + 1];
+            }
+        // face-gradient slots (c, d, k): reference read = column-major Map over the interleaved
+        // storage, so component c of row r = q*np+i is element r + c*nfq*np of the vector whose
+        // entry ((q'*np+i')*2 + d) is d-th physical derivative of phi_i' at point q'.
+        const bool quirk = (c.quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) != 0;
+        for (int f = 0; f < 3; ++f)
+            for (int q = 0; q < nfq; ++q)
+                for (int i = 0; i < np; ++i)
+                    for (int cc = 0; cc < 2; ++cc) {
+                        int qs = q, is = i, ds = cc;
+                        if (quirk) {
+                            int k = q * np + i + cc * nfq * np;
+                            ds = k % 2;
+                            qs = (k / 2) / np;
+                            is = (k / 2) % np;
+                        }
+                        for (int kk = 0; kk < 2; ++kk) {
+                            int slot = (cc * 2 + ds) * 2 + kk;
+                            (*t)[L.FG + ((f * nfq + q) * 8 + slot) * np + i] = gF[((f * nfq + qs) * np + is) * 2 + kk];
+                        }
+                    }
+        for (int i = 0; i < np; ++i) {
+            double sm = 0.0;
+            for (int q = 0; q < nq; ++q) sm += vr.w[q] * phiV[q * np + i];
+            (*t)[L.mass + i] = sm;
+        }
+    }
+    std::copy(px.begin(), px.end(), tR.begin() + L.PX);
+    std::copy(py.begin(), py.end(), tR.begin() + L.PY);
+    std::copy(lf.begin(), lf.end(), tR.begin() + L.LF);
+    fold(px, nq, tL, L.PX);
+    fold(py, nq, tL, L.PY);
+    fold(lf, 3 * nfq, tL, L.LF);
+    std::vector<double> Pm(nq * np);  // projection: Mi phi w
+    for (int q = 0; q < nq; ++q)
+        for (int i = 0; i < np; ++i) {
+            double sacc = 0.0;
+            for (int j = 0; j < np; ++j) sacc += Mi[i * np + j] * vr.w[q] * phiV[q * np + j];
+            Pm[q * np + i] = sacc;
+        }
+
+    // ---------------- geometry (FP64) and connectivity codes ----------------
+    const int K = c.K, Kp = c.Kp;
+    const double* V = m.vertices;
+    const int32_t* T = m.elements;
+    const int32_t* N = m.face_neighbors;
+    std::vector<double> geo((size_t)kGeoRows * Kp, 0.0), hmin(Kp, 1.0);
+    std::vector<int32_t> nbr((size_t)3 * Kp, 0);
+    std::vector<uint32_t> code(Kp, 0);
+    for (int e = 0; e < K; ++e) geo[G_DET * Kp + e] = 1.0;
+    c.h_verts.assign((size_t)K * 6, 0.0);
+    auto vert = [&](int e, int k, double& x, double& y) {
+        int id = T[e * 3 + k];
+        if (id < 0 || id >= m.n_vertices) throw std::invalid_argument("element references a missing vertex");
+        x = V[2 * id];
+        y = V[2 * id + 1];
+    };
+    // GeometricMapping::compute_mapping (S/reference/mapping.cpp:22-50), constant on a triangle
+    for (int e = 0; e < K; ++e) {
+        double x0, y0, x1, y1, x2, y2;
+        vert(e, 0, x0, y0); vert(e, 1, x1, y1); vert(e, 2, x2, y2);
+        double jt00 = x1 - x0, jt01 = y1 - y0, jt10 = x2 - x0, jt11 = y2 - y0;
+        double det = jt00 * jt11 - jt01 * jt10;
+        if (std::fabs(det) < 1e-12) throw std::invalid_argument("Singular Jacobian matrix");
+        double id = 1.0 / det;
+        geo[G_A00 * Kp + e] = jt11 * id;
+        geo[G_A01 * Kp + e] = -jt01 * id;
+        geo[G_A10 * Kp + e] = -jt10 * id;
+        geo[G_A11 * Kp + e] = jt00 * id;
+        double* hv = &c.h_verts[(size_t)e * 6];
+        hv[0] = x0; hv[1] = y0; hv[2] = x1; hv[3] = y1; hv[4] = x2; hv[5] = y2;
+        geo[G_DET * Kp + e] = std::fabs(det);
+        double l01 = std::hypot(x1 - x0, y1 - y0), l12 = std::hypot(x2 - x1, y2 - y1), l20 = std::hypot(x0 - x2, y0 - y2);
+        hmin[e] = std::min(l01, std::min(l12, l20));
+    }
+    // face normal/length of element e, local face f (DGSpace::compute_face_data, S/core/space.cpp:104-137)
+    auto face_geom = [&](int e, int f, double& nx, double& ny, double& len, double* v1, double* v2) {
+        double xs[3], ys[3];
+        for (int k = 0; k < 3; ++k) vert(e, k, xs[k], ys[k]);
+        int a = f, b = (f + 1) % 3;
+        v1[0] = xs[a]; v1[1] = ys[a]; v2[0] = xs[b]; v2[1] = ys[b];
+        double tx = xs[b] - xs[a], ty = ys[b] - ys[a];
+        len = std::sqrt(tx * tx + ty * ty);
+        if (len < 1e-14) throw std::invalid_argument("Degenerate edge detected");
+        nx = ty / len;
+        ny = -tx / len;
+        double cx = (xs[0] + xs[1] + xs[2]) / 3.0, cy = (ys[0] + ys[1] + ys[2]) / 3.0;
+        if (nx * (0.5 * (xs[a] + xs[b]) - cx) + ny * (0.5 * (ys[a] + ys[b]) - cy) < 0) { nx = -nx; ny = -ny; }
+    };
+    std::map<std::pair<int, int>, int> pair_count;
+    int nbf = 0;
+    for (int e = 0; e < K; ++e)
+        for (int f = 0; f < 3; ++f) {
+            int en = N[(e * 3 + f) * 2], fn = N[(e * 3 + f) * 2 + 1];
+            double nx, ny, len, a1[2], a2[2];
+            face_geom(e, f, nx, ny, len, a1, a2);
+            uint32_t cfield = 0;
+            if (en < 0) {
+                ++nbf;
+                int bci = fn;
+                if (bci < 0) bci = m.n_bcs;  // "no boundary condition" row
+                if (bci > m.n_bcs) throw std::invalid_argument("boundary face references a missing boundary condition");
+                nbr[f * Kp + e] = bci;
+                cfield = kFaceBoundary | (1u << 3);
+            } else {
+                if (en >= K || fn < 0 || fn > 2) throw std::invalid_argument("face neighbour out of range");
+                if (N[(en * 3 + fn) * 2] != e || N[(en * 3 + fn) * 2 + 1] != f)
+                    throw std::invalid_argument("face_neighbors is not symmetric");
+                if (++pair_count[{std::min(e, en), std::max(e, en)}] > 2)
+                    throw Fail(DGX_E_UNSUPPORTED, "two elements share more than one face (periodic mesh with n < 2)");
+                bool isL = (e < en) || (e == en && f < fn);
+                int eL = isL ? e : en, fL = isL ? f : fn, eR = isL ? en : e, fR = isL ? fn : f;
+                double nLx, nLy, lenL, L1[2], L2[2], nRx, nRy, lenR, R1[2], R2[2];
+                face_geom(eL, fL, nLx, nLy, lenL, L1, L2);
+                face_geom(eR, fR, nRx, nRy, lenR, R1, R2);
+                // DGSpace::compute_face_permutation (S/core/space.cpp:173-228)
+                const double tol = 1e-10;
+                auto d = [](const double* p, const double* q) { return std::hypot(p[0] - q[0], p[1] - q[1]); };
+                bool rev = d(L1, R2) < tol && d(L2, R1) < tol;
+                bool same = d(L1, R1) < tol && d(L2, R2) < tol;
+                bool reverse;
+                if (rev || same) reverse = rev;
+                else {
+                    double t1x = L2[0] - L1[0], t1y = L2[1] - L1[1], t2x = R2[0] - R1[0], t2y = R2[1] - R1[1];
+                    double n1 = std::hypot(t1x, t1y), n2 = std::hypot(t2x, t2y);
+                    reverse = (t1x / n1 * t2x / n2 + t1y / n1 * t2y / n2) < 0.0;
+                }
+                // the face is evaluated with the L side's normal and length on both elements
+                nx = isL ? nLx : -nLx;
+                ny = isL ? nLy : -nLy;
+                len = lenL;
+                nbr[f * Kp + e] = en;
+                cfield = uint32_t(fn) | (reverse ? 4u : 0u) | (isL ? 8u : 0u);
+            }
+            geo[(G_N0X + 2 * f) * Kp + e] = nx;
+            geo[(G_N0X + 2 * f + 1) * Kp + e] = ny;
+            geo[(G_L0 + f) * Kp + e] = len;
+            code[e] |= cfield << (4 * f);
+        }
+    c.n_bfaces = nbf;
+    std::vector<double> bct((size_t)(m.n_bcs + 1) * 5, 0.0);
+    for (int b = 0; b < m.n_bcs; ++b) {
+        int t = m.bc_types ? m.bc_types[b] : -1;
+        if (t == DGX_BC_PERIODIC)
+            throw std::invalid_argument("PERIODIC BC encountered on a boundary face - periodic boundaries should be paired");
+        if (t < -1 || t > DGX_BC_NO_SLIP_WALL) throw std::invalid_argument("unknown boundary condition type");
+        bct[b * 5] = t;
+        for (int v = 0; v < 4; ++v) bct[b * 5 + 1 + v] = m.bc_values ? m.bc_values[b * 4 + v] : 0.0;
+    }
+    bct[(size_t)m.n_bcs * 5] = -1.0;
+
+    // ---------------- device ----------------
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Fail(DGX_E_DEVICE, "no HIP device available");
+    hip_check(hipGetDevice(&c.device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&c.ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&c.ev1), "hipEventCreate");
+    c.upload(c.geo, geo.data(), geo.size() * sizeof(double));
+    c.upload(c.nbr, nbr.data(), nbr.size() * sizeof(int32_t));
+    c.upload(c.code, code.data(), code.size() * sizeof(uint32_t));
+    c.upload(c.hmin, hmin.data(), hmin.size() * sizeof(double));
+    c.upload(c.bc, bct.data(), bct.size() * sizeof(double));
+    c.upload(c.tab64, tL.data(), tL.size() * sizeof(double));
+    c.upload(c.minv, Mi.data(), Mi.size() * sizeof(double));
+    c.upload(c.pm, Pm.data(), Pm.size() * sizeof(double));
+    c.upload(c.qw, vr.w.data(), vr.w.size() * sizeof(double));
+    if (precision == DGX_FP64) {
+        c.upload_real<double>(c.tabL, tL);
+        c.upload_real<double>(c.tabR, tR);
+    } else {
+        c.upload_real<float>(c.tabL, tL);
+        c.upload_real<float>(c.tabR, tR);
+    }
+    c.u.alloc(c.state_bytes());
+    c.u1.alloc(c.state_bytes());
+    c.u2.alloc(c.state_bytes());
+    hip_check(hipMemset(c.u.p, 0, c.state_bytes()), "hipMemset");
+    hip_check(hipMemset(c.u1.p, 0, c.state_bytes()), "hipMemset");
+    hip_check(hipMemset(c.u2.p, 0, c.state_bytes()), "hipMemset");
+    c.stage.alloc((size_t)K * std::max(np, nq) * 4 * sizeof(double));
+    c.partial.alloc((size_t)c.red_blocks() * 4 * sizeof(double));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dgx_last_error(void) { return g_error.c_str(); }
+
+dgx_status dgx_mesh_split_quads(int32_t n, double length, double x0, double y0, double* vertices, int32_t* elements) {
+    return guarded([&] {
+        if (n < 1 || !(length > 0) || !vertices || !elements) throw std::invalid_argument("bad mesh size");
+        const double h = length / n;
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) {
+                vertices[2 * (j * (n + 1) + i)] = x0 + i * h;
+                vertices[2 * (j * (n + 1) + i) + 1] = y0 + j * h;
+            }
+        auto vid = [n](int i, int j) { return j * (n + 1) + i; };
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                int32_t* t = elements + 6 * (j * n + i);
+                t[0] = vid(i, j); t[1] = vid(i + 1, j); t[2] = vid(i + 1, j + 1);
+                t[3] = vid(i, j); t[4] = vid(i + 1, j + 1); t[5] = vid(i, j + 1);
+            }
+    });
+}
+
+dgx_status dgx_mesh_connect(int32_t nv, const double* V, int32_t K, const int32_t* T, int32_t periodic_mask,
+                            int32_t* N) {
+    return guarded([&] {
+        if (nv <= 0 || K <= 0 || !V || !T || !N) throw std::invalid_argument("bad mesh");
+        for (int k = 0; k < K * 6; ++k) N[k] = -1;
+        // sorted-vertex-pair signature -> face (DGMesh::build_face_connectivity, S/core/mesh.cpp:312-359)
+        std::vector<std::pair<uint64_t, int>> sig;
+        sig.reserve((size_t)K * 3);
+        for (int e = 0; e < K; ++e)
+            for (int f = 0; f < 3; ++f) {
+                uint32_t a = (uint32_t)T[e * 3 + f], b = (uint32_t)T[e * 3 + (f + 1) % 3];
+                if ((int)a >= nv || (int)b >= nv) throw std::invalid_argument("element references a missing vertex");
+                uint64_t s = ((uint64_t)std::min(a, b) << 32) | std::max(a, b);
+                sig.push_back({s, e * 3 + f});
+            }
+        std::sort(sig.begin(), sig.end());
+        for (size_t k = 0; k < sig.size();) {
+            size_t j = k;
+            while (j < sig.size() && sig[j].first == sig[k].first) ++j;
+            if (j - k == 2) {
+                int A = sig[k].second, B = sig[k + 1].second;
+                N[A * 2] = B / 3; N[A * 2 + 1] = B % 3;
+                N[B * 2] = A / 3; N[B * 2 + 1] = A % 3;
+            } else if (j - k > 2) {
+                throw std::invalid_argument("non-manifold mesh: an edge is shared by more than two elements");
+            }
+            k = j;
+        }
+        // boundary tags by position: 0 Bottom, 1 Right, 2 Top, 3 Left (GMSH physical names,
+        // S/utils/mesh_creation.cpp:256-260); bc index = tag
+        double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+        for (int i = 0; i < nv; ++i) {
+            xmin = std::min(xmin, V[2 * i]); xmax = std::max(xmax, V[2 * i]);
+            ymin = std::min(ymin, V[2 * i + 1]); ymax = std::max(ymax, V[2 * i + 1]);
+        }
+        const double tol = 1e-10 * std::max(xmax - xmin, ymax - ymin);
+        std::vector<std::vector<std::pair<double, int>>> side(4);
+        for (int k = 0; k < K * 3; ++k) {
+            if (N[k * 2] >= 0) continue;
+            int e = k / 3, f = k % 3;
+            int a = T[e * 3 + f], b = T[e * 3 + (f + 1) % 3];
+            double xa = V[2 * a], ya = V[2 * a + 1], xb = V[2 * b], yb = V[2 * b + 1];
+            int tag = -1;
+            if (std::fabs(ya - ymin) < tol && std::fabs(yb - ymin) < tol) tag = 0;
+            else if (std::fabs(xa - xmax) < tol && std::fabs(xb - xmax) < tol) tag = 1;
+            else if (std::fabs(ya - ymax) < tol && std::fabs(yb - ymax) < tol) tag = 2;
+            else if (std::fabs(xa - xmin) < tol && std::fabs(xb - xmin) < tol) tag = 3;
+            N[k * 2 + 1] = tag;
+            if (tag >= 0) side[tag].push_back({(tag == 1 || tag == 3) ? 0.5 * (ya + yb) : 0.5 * (xa + xb), k});
+        }
+        // periodic pairing by transverse midpoint (DGMesh::set_periodic_boundaries, :482-588)
+        auto pair_sides = [&](int s1, int s2) {
+            auto& A = side[s1];
+            auto& B = side[s2];
+            if (A.empty() || A.size() != B.size())
+                throw std::invalid_argument("Cannot set periodic boundaries: unmatched boundary faces");
+            std::sort(A.begin(), A.end());
+            std::sort(B.begin(), B.end());
+            for (size_t i = 0; i < A.size(); ++i) {
+                int a = A[i].second, b = B[i].second;
+                N[a * 2] = b / 3; N[a * 2 + 1] = b % 3;
+                N[b * 2] = a / 3; N[b * 2 + 1] = a % 3;
+            }
+        };
+        if (periodic_mask & 1) pair_sides(3, 1);
+        if (periodic_mask & 2) pair_sides(0, 2);
+    });
+}
+
+dgx_status dgx_create(const dgx_mesh_desc* mesh, const dgx_space_desc* space, const dgx_physics* physics,
+                      int32_t precision, dgx_ctx** out) {
+    if (!out) { g_error = "out is null"; return DGX_E_INVALID; }
+    *out = nullptr;
+    auto* c = new (std::nothrow) dgx_ctx();
+    if (!c) { g_error = "out of host memory"; return DGX_E_NOMEM; }
+    dgx_status st = guarded([&] {
+        if (!mesh || !space || !physics) throw std::invalid_argument("null descriptor");
+        build_context(*c, *mesh, *space, *physics, precision);
+    });
+    if (st != DGX_OK) { delete c; return st; }
+    *out = c;
+    return DGX_OK;
+}
+
+void dgx_destroy(dgx_ctx* ctx) { delete ctx; }
+
+dgx_status dgx_info(const dgx_ctx* c, dgx_info_t* o) {
+    if (!c || !o) { g_error = "null argument"; return DGX_E_INVALID; }
+    o->n_elements = c->K;
+    o->n_modes = c->np;
+    o->n_volume_points = c->nq;
+    o->n_face_points = c->nfq;
+    o->order = c->p;
+    o->basis = c->basis;
+    o->precision = c->precision;
+    o->physics = c->physics;
+    o->n_boundary_faces = c->n_bfaces;
+    o->device = c->device;
+    // per stage: read u_k, (stages 2,3) u^n, write u_{k+1}: 8 state touches per 3 stages,
+    // plus 128 B of FP64 geometry + connectivity per element
+    o->bytes_per_stage_alg = (double)c->K * (8.0 / 3.0 * 4.0 * c->np * (double)c->rsz + 128.0);
+    return DGX_OK;
+}
+
+dgx_status dgx_set_state(dgx_ctx* c, const double* u) {
+    return guarded([&] {
+        if (!c || !u) throw std::invalid_argument("null argument");
+        size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
+        hip_check(hipMemcpyAsync(c->stage.p, u, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+        hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u.p, c->K, c->Kp, c->stream), "pack");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+dgx_status dgx_get_state(dgx_ctx* c, double* u) {
+    return guarded([&] {
+        if (!c || !u) throw std::invalid_argument("null argument");
+        size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
+        hip_check(launch_unpack(c->precision, c->np, c->u.p, c->stage.as<double>(), c->K, c->Kp, c->stream), "unpack");
+        hip_check(hipMemcpyAsync(u, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
+    return guarded([&] {
+        if (!c || !xy) throw std::invalid_argument("null argument");
+        // GeometricMapping::map_to_physical (S/reference/mapping.cpp:52-58)
+        RuleTri vr = dunavant(2 * c->p + c->rule);
+        for (int e = 0; e < c->K; ++e) {
+            const double* V = &c->h_verts[(size_t)e * 6];
+            for (int q = 0; q < c->nq; ++q) {
+                double n0 = 1.0 - vr.xi[q] - vr.eta[q], n1 = vr.xi[q], n2 = vr.eta[q];
+                xy[((size_t)e * c->nq + q) * 2] = n0 * V[0] + n1 * V[2] + n2 * V[4];
+                xy[((size_t)e * c->nq + q) * 2 + 1] = n0 * V[1] + n1 * V[3] + n2 * V[5];
+            }
+        }
+    });
+}
+
+dgx_status dgx_project(dgx_ctx* c, const double* Uq) {
+    return guarded([&] {
+        if (!c || !Uq) throw std::invalid_argument("null argument");
+        size_t bytes = (size_t)c->K * c->nq * 4 * sizeof(double);
+        hip_check(hipMemcpyAsync(c->stage.p, Uq, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+        hip_check(launch_project(c->precision, c->p, c->rule, c->stage.as<double>(), c->u.p, nullptr, c->K, c->Kp,
+                                 c->pm.as<double>(), c->stream),
+                  "project");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+dgx_status dgx_eval_volume(dgx_ctx* c, double* Uq) {
+    return guarded([&] {
+        if (!c || !Uq) throw std::invalid_argument("null argument");
+        TabLayout L(c->p, c->rule);
+        size_t bytes = (size_t)c->K * c->nq * 4 * sizeof(double);
+        hip_check(launch_eval(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>() + L.phiV, c->K, c->Kp,
+                              c->stage.as<double>(), c->stream),
+                  "eval");
+        hip_check(hipMemcpyAsync(Uq, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+dgx_status dgx_residual(dgx_ctx* c, const double* u, double* r) {
+    return guarded([&] {
+        if (!c || !r) throw std::invalid_argument("null argument");
+        size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
+        const void* src = c->u.p;
+        if (u) {
+            hip_check(hipMemcpyAsync(c->stage.p, u, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+            hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u2.p, c->K, c->Kp, c->stream), "pack");
+            src = c->u2.p;
+        }
+        c->stage_launch(src, src, c->u1.p, MODE_RESIDUAL, 0.0, 0.0, 0.0);
+        hip_check(launch_unpack(c->precision, c->np, c->u1.p, c->stage.as<double>(), c->K, c->Kp, c->stream), "unpack");
+        hip_check(hipMemcpyAsync(r, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+dgx_status dgx_apply_minv(dgx_ctx* c, double* r) {
+    return guarded([&] {
+        if (!c || !r) throw std::invalid_argument("null argument");
+        size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
+        hip_check(hipMemcpyAsync(c->stage.p, r, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+        hip_check(launch_apply_minv(c->precision, c->np, c->minv.as<double>(), c->geo.as<double>(),
+                                    c->stage.as<double>(), c->K, c->Kp, c->stream),
+                  "minv");
+        hip_check(hipMemcpyAsync(r, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+dgx_status dgx_rk3_steps_async(dgx_ctx* c, double dt, int32_t nsteps) {
+    return guarded([&] {
+        if (!c) throw std::invalid_argument("null context");
+        if (nsteps < 0 || !std::isfinite(dt)) throw std::invalid_argument("bad step count or dt");
+        for (int s = 0; s < nsteps; ++s) c->rk3_step(dt);
+    });
+}
+
+dgx_status dgx_synchronize(dgx_ctx* c) {
+    return guarded([&] {
+        if (!c) throw std::invalid_argument("null context");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+dgx_status dgx_rk3_steps(dgx_ctx* c, double dt, int32_t nsteps) {
+    dgx_status st = dgx_rk3_steps_async(c, dt, nsteps);
+    if (st != DGX_OK) return st;
+    return guarded([&] {
+        std::vector<double> h = c->reduce(3);
+        double bad = 0.0;
+        for (int b = 0; b < c->red_blocks(); ++b) bad += h[b * 4];
+        if (bad > 0) throw Fail(DGX_E_NONFINITE, "Solution contains NaN values");
+    });
+}
+
+dgx_status dgx_rk3_profile(dgx_ctx* c, double dt, int32_t nsteps, double* stage_ms, double* total_ms) {
+    return guarded([&] {
+        if (!c || nsteps <= 0) throw std::invalid_argument("bad arguments");
+        // per-launch events around every stage launch on the context stream
+        std::vector<hipEvent_t> ev((size_t)nsteps * 6);
+        for (auto& x : ev) hip_check(hipEventCreate(&x), "hipEventCreate");
+        hip_check(hipEventRecord(c->ev0, c->stream), "record");
+        for (int s = 0; s < nsteps; ++s) {
+            const double ca[3] = {0.0, 0.75, 1.0 / 3.0}, cb[3] = {1.0, 0.25, 2.0 / 3.0};
+            void* in[3] = {c->u.p, c->u1.p, c->u2.p};
+            void* outb[3] = {c->u1.p, c->u2.p, c->u1.p};
+            for (int k = 0; k < 3; ++k) {
+                hip_check(hipEventRecord(ev[(size_t)s * 6 + 2 * k], c->stream), "record");
+                c->stage_launch(in[k], c->u.p, outb[k], MODE_RK, dt, ca[k], cb[k]);
+                hip_check(hipEventRecord(ev[(size_t)s * 6 + 2 * k + 1], c->stream), "record");
+            }
+            std::swap(c->u, c->u1);
+        }
+        hip_check(hipEventRecord(c->ev1, c->stream), "record");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        double sum = 0.0;
+        for (int k = 0; k < nsteps * 3; ++k) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]), "elapsed");
+            sum += ms;
+        }
+        float tot = 0.f;
+        hip_check(hipEventElapsedTime(&tot, c->ev0, c->ev1), "elapsed");
+        for (auto& x : ev) (void)hipEventDestroy(x);
+        if (stage_ms) *stage_ms = sum / (3.0 * nsteps);
+        if (total_ms) *total_ms = tot;
+    });
+}
+
+dgx_status dgx_conservation_sums(dgx_ctx* c, double out[4]) {
+    return guarded([&] {
+        if (!c || !out) throw std::invalid_argument("null argument");
+        std::vector<double> h = c->reduce(0);
+        for (int v = 0; v < 4; ++v) {
+            double s = 0.0;
+            for (int b = 0; b < c->red_blocks(); ++b) s += h[b * 4 + v];
+            out[v] = s;
+        }
+    });
+}
+
+dgx_status dgx_kinetic_energy(dgx_ctx* c, double* J) {
+    return guarded([&] {
+        if (!c || !J) throw std::invalid_argument("null argument");
+        std::vector<double> h = c->reduce(1);
+        double s = 0.0;
+        for (int b = 0; b < c->red_blocks(); ++b) s += h[b * 4];
+        *J = s;
+    });
+}
+
+dgx_status dgx_max_cfl(dgx_ctx* c, double dt, double* cfl) {
+    return guarded([&] {
+        if (!c || !cfl) throw std::invalid_argument("null argument");
+        std::vector<double> h = c->reduce(2);
+        double mx = 0.0, bad = 0.0;
+        for (int b = 0; b < c->red_blocks(); ++b) { mx = std::max(mx, h[b * 4]); bad += h[b * 4 + 3]; }
+        *cfl = bad > 0 ? INFINITY : dt * mx;
+    });
+}
+
+dgx_status dgx_density_range(dgx_ctx* c, double out[2]) {
+    return guarded([&] {
+        if (!c || !out) throw std::invalid_argument("null argument");
+        std::vector<double> h = c->reduce(2);
+        double lo = INFINITY, hi = -INFINITY;
+        for (int b = 0; b < c->red_blocks(); ++b) { lo = std::min(lo, h[b * 4 + 1]); hi = std::max(hi, h[b * 4 + 2]); }
+        if (!std::isfinite(lo) || !std::isfinite(hi)) { out[0] = out[1] = NAN; return; }
+        out[0] = lo;
+        out[1] = hi;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Private definitions shared by the host API (dgx_api.cpp) and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dgx.h"
+
+namespace dgx {
+
+constexpr int kBlock = 256;  // threads per block of the element kernels (4 wave64)
+
+// Volume points of the Dunavant rule exact to degree 2p (+1 for the 2P1 variant).
+constexpr int nq_of(int p, int rule) {
+    return rule == 0 ? (p == 1 ? 3 : p == 2 ? 6 : p == 3 ? 12 : 16)
+                     : (p == 1 ? 4 : p == 2 ? 7 : p == 3 ? 13 : 19);
+}
+constexpr int np_of(int p) { return (p + 1) * (p + 2) / 2; }
+
+// Geometry rows of the per-element FP64 array geo[kGeoRows][Kp].
+enum GeoRow : int {
+    G_A00 = 0, G_A01, G_A10, G_A11,  // dxi_dx = inverse of J^T (row-major)
+    G_DET,                            // |det J|
+    G_N0X, G_N0Y, G_N1X, G_N1Y, G_N2X, G_N2Y,  // outward unit normals of faces 0..2
+    G_L0, G_L1, G_L2,                 // face lengths
+    kGeoRows
+};
+
+// Per-element face code: 4 bits per local face f at bit 4f:
+//   bits 0-1 neighbour local face (3 = boundary face), bit 2 reversed point order,
+//   bit 3 this element holds the canonical (L) orientation of the face.
+constexpr uint32_t kFaceBoundary = 3u;
+
+// Table layout (Real), offsets in elements.  "L" tables carry the reference-element mass
+// inverse folded in; "R" tables produce M dU/dt.
+struct TabLayout {
+    int np, nq, nfq;
+    int phiV, PX, PY, phiF, LF, DXV, DYV, FG, mass, total;
+    TabLayout(int p, int rule) {
+        np = np_of(p);
+        nq = nq_of(p, rule);
+        nfq = p + 1;
+        int o = 0;
+        phiV = o; o += nq * np;
+        PX = o; o += nq * np;
+        PY = o; o += nq * np;
+        phiF = o; o += 3 * nfq * np;
+        LF = o; o += 3 * nfq * np;
+        DXV = o; o += nq * np;
+        DYV = o; o += nq * np;
+        FG = o; o += 3 * nfq * 8 * np;  // face gradients [3][nfq][slot (c,d,k)][np]
+        mass = o; o += np;
+        total = o;
+    }
+};
+
+// Launch parameters of one fused stage: out = ca*un + cb*(uk + dt*L(uk)) (mode RK),
+// out = L(uk) (mode RHS) or out = M dU/dt (mode RESIDUAL).
+enum StageMode : int { MODE_RK = 0, MODE_RHS = 1, MODE_RESIDUAL = 2 };
+
+struct StageParams {
+    const void* uk;
+    const void* un;
+    void* out;
+    const double* geo;
+    const int32_t* nbr;
+    const uint32_t* code;
+    const void* tab;        // tables in the state precision
+    const double* bc;       // [n_bcs][5]: type, 4 values
+    int K, Kp;
+    double dt, ca, cb;
+    double gamma, mu, prandtl, sigma0;
+    int mode;
+    int ns_face_quirk;
+};
+
+// Kernel launchers (dgx_kernels.hip / dgx_kernels_ns.hip).  Each returns hipSuccess or the
+// launch error.
+hipError_t launch_stage(int precision, int physics, int p, int rule, const StageParams& sp,
+                        hipStream_t s);
+hipError_t launch_project(int precision, int p, int rule, const double* Uq, void* u,
+                          const void* tabL, int K, int Kp, const double* w_minv, hipStream_t s);
+hipError_t launch_eval(int precision, int p, int rule, const void* u, const void* tab, int K,
+                       int Kp, double* Uq, hipStream_t s);
+hipError_t launch_pack(int precision, int np, const double* host_layout_dev, void* u, int K,
+                       int Kp, hipStream_t s);
+hipError_t launch_unpack(int precision, int np, const void* u, double* host_layout_dev, int K,
+                         int Kp, hipStream_t s);
+// Per-block FP64 partial reductions: 0 conservation sums (4), 1 kinetic energy (1),
+// 2 CFL/density/finite (max wave/h, rho min, rho max, bad count)
+hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u, const void* tab,
+                         const double* geo, const double* hmin, const double* qw, int K, int Kp,
+                         double gamma, double* partial, int nblocks, hipStream_t s);
+hipError_t launch_apply_minv(int precision, int np, const double* minv_ref, const double* geo,
+                             double* r_dev, int K, int Kp, hipStream_t s);
+
+}  // namespace dgx

@@ -1,0 +1,376 @@
+"""Python binding of the dgx C ABI (include/dgx.h) over ctypes.
+
+Mirrors the reference's compressible-solver interface for this path:
+  * ``EulerWeakFormulation`` / ``NavierStokesWeakFormulation``  (physics parameters,
+    I/weak_forms/euler_weak_formulation.hpp, navier_stokes_weak_formulation.hpp)
+  * ``DGAssembler.assemble_euler_residual(u)``  -> M dU/dt      (S/solver/assembler.cpp:137-279)
+  * ``EulerDGSolver`` / ``NavierStokesDGSolver``: apply_mass_inv, time_step_ssp_rk3,
+    project_initial_condition, compute_max_cfl, compute_density_range, solve()
+    (S/solver/compressible_solver_base.cpp)
+State arrays use the reference layout [K][Np][4] in FP64.
+
+The compute runs in libdgx.so (HIP, gfx950).  There is no CPU fallback: constructing a
+context without the library or without a HIP device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdgx.so")
+
+DGX_OK, DGX_E_INVALID, DGX_E_UNSUPPORTED, DGX_E_DEVICE, DGX_E_NONFINITE, DGX_E_NOMEM = range(6)
+BASIS = {"monomial": 0, "dubiner": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "fp32_bf16vol": 2}
+PHYSICS = {"euler": 0, "ns": 1}
+BC_TYPES = {"far_field": 0, "slip_wall": 1, "no_slip_wall": 2, "periodic": 3, "none": -1}
+QUIRK_NS_FACE_GRAD_REFERENCE = 1
+
+# exported symbols, in the order include/dgx.h declares them
+EXPORTS = [
+    "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_create", "dgx_destroy", "dgx_last_error",
+    "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
+    "dgx_eval_volume", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
+    "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
+    "dgx_density_range", "dgx_kinetic_energy",
+]
+
+
+class DGXError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"dgx status {status}: {msg}")
+        self.status = status
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("n_vertices", C.c_int32), ("vertices", C.POINTER(C.c_double)),
+                ("n_elements", C.c_int32), ("elements", C.POINTER(C.c_int32)),
+                ("face_neighbors", C.POINTER(C.c_int32)), ("n_bcs", C.c_int32),
+                ("bc_types", C.POINTER(C.c_int32)), ("bc_values", C.POINTER(C.c_double))]
+
+
+class SpaceDesc(C.Structure):
+    _fields_ = [("order", C.c_int32), ("basis", C.c_int32), ("rule", C.c_int32)]
+
+
+class Physics(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("gamma", C.c_double), ("mu", C.c_double),
+                ("prandtl", C.c_double), ("sigma0", C.c_double), ("quirks", C.c_uint32)]
+
+
+class Info(C.Structure):
+    _fields_ = [("n_elements", C.c_int32), ("n_modes", C.c_int32), ("n_volume_points", C.c_int32),
+                ("n_face_points", C.c_int32), ("order", C.c_int32), ("basis", C.c_int32),
+                ("precision", C.c_int32), ("physics", C.c_int32), ("n_boundary_faces", C.c_int32),
+                ("device", C.c_int32), ("bytes_per_stage_alg", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libdgx.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DGXError(DGX_E_DEVICE, f"{LIB_PATH} is missing: build it (python __graft_entry__.py or make)")
+    L = C.CDLL(LIB_PATH)
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    L.dgx_mesh_split_quads.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, dp, ip]
+    L.dgx_mesh_connect.argtypes = [C.c_int32, dp, C.c_int32, ip, C.c_int32, ip]
+    L.dgx_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(SpaceDesc), C.POINTER(Physics), C.c_int32,
+                             C.POINTER(vp)]
+    L.dgx_destroy.argtypes = [vp]
+    L.dgx_destroy.restype = None
+    L.dgx_last_error.restype = C.c_char_p
+    L.dgx_info.argtypes = [vp, C.POINTER(Info)]
+    for n in ("dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project", "dgx_eval_volume",
+              "dgx_apply_minv", "dgx_conservation_sums", "dgx_density_range", "dgx_kinetic_energy"):
+        getattr(L, n).argtypes = [vp, dp]
+    L.dgx_residual.argtypes = [vp, dp, dp]
+    L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]
+    L.dgx_rk3_steps_async.argtypes = [vp, C.c_double, C.c_int32]
+    L.dgx_synchronize.argtypes = [vp]
+    L.dgx_rk3_profile.argtypes = [vp, C.c_double, C.c_int32, dp, dp]
+    L.dgx_max_cfl.argtypes = [vp, C.c_double, dp]
+    for n in EXPORTS:
+        f = getattr(L, n)
+        if n not in ("dgx_destroy", "dgx_last_error"):
+            f.restype = C.c_int32
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _check(st):
+    if st != DGX_OK:
+        raise DGXError(st, load().dgx_last_error().decode())
+
+
+# ----------------------------------------------------------------------------------------------
+# mesh helpers
+# ----------------------------------------------------------------------------------------------
+def split_quad_mesh(n, length=10.0, x0=0.0, y0=0.0):
+    verts = np.zeros(((n + 1) * (n + 1), 2))
+    tris = np.zeros((2 * n * n, 3), dtype=np.int32)
+    _check(load().dgx_mesh_split_quads(n, length, x0, y0, _dp(verts), _ip(tris)))
+    return verts, tris
+
+
+def connect(verts, tris, periodic=3):
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    nbr = np.zeros((len(tris), 3, 2), dtype=np.int32)
+    _check(load().dgx_mesh_connect(len(verts), _dp(verts), len(tris), _ip(tris), periodic, _ip(nbr)))
+    return nbr
+
+
+# ----------------------------------------------------------------------------------------------
+# weak forms (physics parameters)
+# ----------------------------------------------------------------------------------------------
+class EulerWeakFormulation:
+    """EulerWeakFormulation(gamma) (I/weak_forms/euler_weak_formulation.hpp)."""
+    kind = "euler"
+
+    def __init__(self, gamma=1.4):
+        self.gamma = gamma
+        self.mu, self.prandtl, self.sigma0, self.quirks = 0.0, 0.72, 5.0, 0
+
+    def has_viscous_terms(self):
+        return False
+
+
+class NavierStokesWeakFormulation(EulerWeakFormulation):
+    """NavierStokesWeakFormulation(gamma, mu, Pr, sigma0) with the reference defaults
+    mu = 1e-3, Pr = 0.72, sigma0 = 5 (I/solver/dg_solver.hpp:283-285).  ``face_grad`` selects
+    the reference's face-gradient read ("reference") or the mathematically correct one."""
+    kind = "ns"
+
+    def __init__(self, gamma=1.4, dynamic_viscosity=1e-3, prandtl=0.72, penalty_prefactor=5.0,
+                 face_grad="reference"):
+        super().__init__(gamma)
+        if dynamic_viscosity <= 0:
+            raise ValueError("Dynamic viscosity must be positive for Navier-Stokes weak formulation")
+        if prandtl <= 0:
+            raise ValueError("Prandtl number must be positive for Navier-Stokes weak formulation")
+        if penalty_prefactor <= 0:
+            raise ValueError("Penalty prefactor must be positive for Navier-Stokes weak formulation")
+        self.mu, self.prandtl, self.sigma0 = dynamic_viscosity, prandtl, penalty_prefactor
+        self.quirks = QUIRK_NS_FACE_GRAD_REFERENCE if face_grad == "reference" else 0
+
+    def compute_penalty_parameter(self, p, h):
+        return self.sigma0 * self.mu * (p + 1) * (p + 1) / max(h, 1e-12)
+
+    def has_viscous_terms(self):
+        return True
+
+
+# ----------------------------------------------------------------------------------------------
+# context
+# ----------------------------------------------------------------------------------------------
+class Context:
+    """One mesh + space + physics + precision on the GPU.  Layout of every host state array:
+    [K][Np][4] FP64."""
+
+    def __init__(self, verts, tris, order, weak_form=None, basis="dubiner", precision="fp32",
+                 rule=0, periodic=3, neighbors=None, bcs=None):
+        L = load()
+        wf = weak_form or EulerWeakFormulation()
+        self.weak_form = wf
+        self.order = order
+        self._verts = np.ascontiguousarray(verts, dtype=np.float64)
+        self._tris = np.ascontiguousarray(tris, dtype=np.int32)
+        self._nbr = (connect(self._verts, self._tris, periodic) if neighbors is None
+                     else np.ascontiguousarray(neighbors, dtype=np.int32))
+        # boundary-condition table indexed by tag 0..3 = Bottom, Right, Top, Left
+        bcs = bcs or {}
+        tags = ["Bottom", "Right", "Top", "Left"]
+        bc_t = np.full(4, -1, dtype=np.int32)
+        bc_v = np.zeros((4, 4))
+        for name, (typ, val) in bcs.items():
+            k = tags.index(name)
+            bc_t[k] = BC_TYPES[typ]
+            bc_v[k] = val
+        self._bc_t, self._bc_v = bc_t, bc_v
+        mesh = MeshDesc(len(self._verts), _dp(self._verts), len(self._tris), _ip(self._tris),
+                        _ip(self._nbr), 4, _ip(bc_t), _dp(bc_v))
+        space = SpaceDesc(order, BASIS[basis], rule)
+        phys = Physics(PHYSICS[wf.kind], wf.gamma, wf.mu, wf.prandtl, wf.sigma0, wf.quirks)
+        h = C.c_void_p()
+        _check(L.dgx_create(C.byref(mesh), C.byref(space), C.byref(phys), PRECISION[precision], C.byref(h)))
+        self._h = h
+        info = Info()
+        _check(L.dgx_info(h, C.byref(info)))
+        self.info = info
+        self.K, self.Np, self.Nq, self.Nfq = info.n_elements, info.n_modes, info.n_volume_points, info.n_face_points
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dgx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # state
+    def set_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.K, self.Np, 4)
+        _check(load().dgx_set_state(self._h, _dp(u)))
+
+    def get_state(self):
+        u = np.zeros((self.K, self.Np, 4))
+        _check(load().dgx_get_state(self._h, _dp(u)))
+        return u
+
+    def volume_points(self):
+        xy = np.zeros((self.K, self.Nq, 2))
+        _check(load().dgx_volume_points(self._h, _dp(xy)))
+        return xy
+
+    def project(self, Uq):
+        Uq = np.ascontiguousarray(Uq, dtype=np.float64)
+        _check(load().dgx_project(self._h, _dp(Uq)))
+
+    def eval_volume(self):
+        Uq = np.zeros((self.K, self.Nq, 4))
+        _check(load().dgx_eval_volume(self._h, _dp(Uq)))
+        return Uq
+
+    # hot path
+    def residual(self, u=None):
+        r = np.zeros((self.K, self.Np, 4))
+        if u is None:
+            _check(load().dgx_residual(self._h, None, _dp(r)))
+        else:
+            u = np.ascontiguousarray(u, dtype=np.float64)
+            _check(load().dgx_residual(self._h, _dp(u), _dp(r)))
+        return r
+
+    def apply_minv(self, r):
+        r = np.array(r, dtype=np.float64, order="C")
+        _check(load().dgx_apply_minv(self._h, _dp(r)))
+        return r
+
+    def rk3_steps(self, dt, nsteps):
+        _check(load().dgx_rk3_steps(self._h, dt, nsteps))
+
+    def rk3_steps_async(self, dt, nsteps):
+        _check(load().dgx_rk3_steps_async(self._h, dt, nsteps))
+
+    def synchronize(self):
+        _check(load().dgx_synchronize(self._h))
+
+    def rk3_profile(self, dt, nsteps):
+        s, t = C.c_double(), C.c_double()
+        _check(load().dgx_rk3_profile(self._h, dt, nsteps, C.byref(s), C.byref(t)))
+        return s.value, t.value
+
+    # diagnostics
+    def conservation_sums(self):
+        o = np.zeros(4)
+        _check(load().dgx_conservation_sums(self._h, _dp(o)))
+        return o
+
+    def max_cfl(self, dt):
+        o = C.c_double()
+        _check(load().dgx_max_cfl(self._h, dt, C.byref(o)))
+        return o.value
+
+    def density_range(self):
+        o = np.zeros(2)
+        _check(load().dgx_density_range(self._h, _dp(o)))
+        return o
+
+    def kinetic_energy(self):
+        o = np.zeros(1)
+        _check(load().dgx_kinetic_energy(self._h, _dp(o)))
+        return float(o[0])
+
+
+def stable_dt(ctx, cfl=0.3):
+    """dt = CFL h_min / ((2p+1) max(|u|+c)) for the current state: the reference's CFL
+    measure (compute_max_cfl) scaled by the DG stability factor 1/(2p+1)."""
+    lam = ctx.max_cfl(1.0)
+    return cfl / ((2 * ctx.order + 1) * lam)
+
+
+# ----------------------------------------------------------------------------------------------
+# reference-API mirror
+# ----------------------------------------------------------------------------------------------
+class DGAssembler:
+    """DGAssembler(mesh, weak_form).assemble_euler_residual(u) -> M dU/dt."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def assemble_euler_residual(self, u_coeffs):
+        return self.ctx.residual(u_coeffs)
+
+
+class CompressibleDGSolver:
+    """EulerDGSolver / NavierStokesDGSolver: solve(ic, T_final, dt, save_every) returns the
+    flattened frames (K, 4 Np) with column i*4+v (flatten_frames,
+    S/solver/compressible_solver_base.cpp:340-368), frames at step 0, every save_every steps,
+    and at the end when not aligned.  The step count is ceil(T_final/dt) explicitly instead of
+    the reference's floating accumulation of t."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.assembler = DGAssembler(ctx)
+
+    def project_initial_condition(self, ic):
+        xy = self.ctx.volume_points()
+        self.ctx.project(ic(xy))
+        return self.ctx.get_state()
+
+    def apply_mass_inv(self, r):
+        return self.ctx.apply_minv(r)
+
+    def time_step_ssp_rk3(self, u_n, dt):
+        self.ctx.set_state(u_n)
+        self.ctx.rk3_steps(dt, 1)
+        return self.ctx.get_state()
+
+    def compute_max_cfl(self, u, dt):
+        self.ctx.set_state(u)
+        return self.ctx.max_cfl(dt)
+
+    def compute_density_range(self, u):
+        self.ctx.set_state(u)
+        return tuple(self.ctx.density_range())
+
+    def solve(self, ic, T_final, dt, save_every):
+        self.project_initial_condition(ic)
+        n_steps = int(math.ceil(T_final / dt))
+        frames = [self.ctx.get_state()]
+        step = 0
+        while step < n_steps:
+            chunk = min(save_every - step % save_every, n_steps - step)
+            st = load().dgx_rk3_steps(self.ctx._h, dt, chunk)
+            step += chunk
+            if st == DGX_E_NONFINITE:
+                break
+            _check(st)
+            if step % save_every == 0:
+                frames.append(self.ctx.get_state())
+        if step % save_every != 0:
+            frames.append(self.ctx.get_state())
+        return [f.reshape(self.ctx.K, self.ctx.Np * 4) for f in frames]
+
+
+EulerDGSolver = CompressibleDGSolver
+NavierStokesDGSolver = CompressibleDGSolver

@@ -1,0 +1,179 @@
+/*
+ * dgx.h — C ABI of the MI355X-native explicit DG residual for the 2-D compressible Euler and
+ * Navier-Stokes equations (modal DG on triangles, Rusanov flux, SSP-RK3).
+ *
+ * The reference exposes this path as C++ virtuals and member functions; each entry point below
+ * names the reference interface it replaces (paths relative to the reference root:
+ * S/ = dgfem/src, I/ = dgfem/include/dgfem).
+ *
+ * Conventions
+ *   - Plain C types only.  Every call returns dgx_status; dgx_last_error() describes the most
+ *     recent failure on the calling thread (the reference throws std::runtime_error /
+ *     std::invalid_argument instead; the C++ mirror in dgfem_amd.hpp re-throws).
+ *   - Host state layout is [K][Np][4] (element, mode, variable), FP64: the reference's
+ *     DGSolution flat index e*Np*4 + i*4 + v (S/core/solution.cpp:143-145) and its frame
+ *     column order i*4+v (S/solver/compressible_solver_base.cpp:355-361).  The device layout
+ *     is private to the library.
+ *   - One context = one mesh + space + physics + precision, one HIP stream, device buffers
+ *     owned by the context.  Contexts are not thread-safe; use one per host thread.
+ */
+#ifndef DGX_H
+#define DGX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGX_ABI_VERSION 1
+
+typedef int32_t dgx_status;
+enum {
+    DGX_OK = 0,
+    DGX_E_INVALID = 1,     /* bad argument (the reference's std::invalid_argument)        */
+    DGX_E_UNSUPPORTED = 2, /* valid request outside what this build implements            */
+    DGX_E_DEVICE = 3,      /* HIP runtime error, or the HIP library/device is unavailable  */
+    DGX_E_NONFINITE = 4,   /* NaN/Inf detected in the state (the reference aborts its loop) */
+    DGX_E_NOMEM = 5
+};
+
+enum dgx_basis { DGX_BASIS_MONOMIAL = 0, DGX_BASIS_DUBINER = 1 };
+/* Volume rule: DGX_RULE_2P = Dunavant exact to degree 2p, as the reference
+ * (S/core/space.cpp:37-46); DGX_RULE_2P1 = exact to 2p+1 (variant). Face rule: GL, p+1 points. */
+enum dgx_rule { DGX_RULE_2P = 0, DGX_RULE_2P1 = 1 };
+/* Precision of the state and the bulk arithmetic. Geometry, inverse-mass scaling and the
+ * conservation sums are FP64 in every mode. DGX_FP32_BF16VOL rounds the operands of the
+ * volume contraction to bfloat16 (FP32 accumulation). */
+enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1, DGX_FP32_BF16VOL = 2 };
+enum dgx_physics_kind { DGX_EULER = 0, DGX_NAVIER_STOKES = 1 };
+/* BCTypeEuler (I/boundary/conditions.hpp:27-32). */
+enum dgx_bc_type { DGX_BC_FAR_FIELD = 0, DGX_BC_SLIP_WALL = 1, DGX_BC_NO_SLIP_WALL = 2, DGX_BC_PERIODIC = 3 };
+/* Compatibility switches for reference behaviours. */
+enum dgx_quirk {
+    /* NS face gradients read through the reference's column-major Map of the interleaved
+     * dphi_dx_face storage (S/core/space.cpp:143,153-158 vs S/core/mesh.cpp:187-195). */
+    DGX_QUIRK_NS_FACE_GRAD_REFERENCE = 1u << 0
+};
+
+typedef struct dgx_ctx dgx_ctx;
+
+/* Mesh: straight-sided triangles.  face_neighbors[(e*3+f)*2 + {0,1}] = (neighbour element,
+ * neighbour local face) for local face f = edge (v_f, v_{(f+1)%3}), or (-1, bc index) on a
+ * boundary face.  Build it with dgx_mesh_connect (DGMesh::build_face_connectivity +
+ * set_periodic_boundaries, S/core/mesh.cpp:312-359,482-588). */
+typedef struct {
+    int32_t n_vertices;
+    const double* vertices;       /* [n_vertices][2] */
+    int32_t n_elements;
+    const int32_t* elements;      /* [K][3] */
+    const int32_t* face_neighbors;/* [K][3][2] */
+    int32_t n_bcs;                /* boundary-condition table referenced by boundary faces */
+    const int32_t* bc_types;      /* [n_bcs] dgx_bc_type */
+    const double* bc_values;      /* [n_bcs][4]: FAR_FIELD conserved state, NO_SLIP primitive */
+} dgx_mesh_desc;
+
+typedef struct {
+    int32_t order; /* p = 1..4 */
+    int32_t basis; /* dgx_basis */
+    int32_t rule;  /* dgx_rule */
+} dgx_space_desc;
+
+/* EulerWeakFormulation(gamma) / NavierStokesWeakFormulation(gamma, mu, Pr, sigma0)
+ * (I/solver/dg_solver.hpp:264,283-285; S/weak_forms/navier_stokes_weak_formulation.cpp:18-34). */
+typedef struct {
+    int32_t kind;   /* dgx_physics_kind */
+    double gamma;
+    double mu;
+    double prandtl;
+    double sigma0;
+    uint32_t quirks; /* dgx_quirk bits */
+} dgx_physics;
+
+typedef struct {
+    int32_t n_elements, n_modes, n_volume_points, n_face_points;
+    int32_t order, basis, precision, physics;
+    int32_t n_boundary_faces;
+    int32_t device;
+    double bytes_per_stage_alg; /* algorithmic HBM bytes of one fused stage launch */
+} dgx_info_t;
+
+/* ---- mesh helpers (host) ------------------------------------------------------------------ */
+
+/* Structured periodic mesh: n x n squares on [x0,x0+L]^2, each split along its (i,j)->(i+1,j+1)
+ * diagonal into two CCW triangles; vertices [(n+1)^2][2], elements [2 n^2][3]. Replaces the
+ * GMSH rectangle generator (S/utils/mesh_creation.cpp:49-74) for the benchmark meshes. */
+dgx_status dgx_mesh_split_quads(int32_t n, double length, double x0, double y0, double* vertices,
+                                int32_t* elements);
+
+/* Face connectivity by sorted vertex pair, boundary faces tagged Bottom/Right/Top/Left by
+ * position (bc index 0..3), periodic pairing by transverse face-midpoint coordinate.
+ * periodic_mask bit0 pairs Left/Right, bit1 pairs Bottom/Top.
+ * Replaces DGMesh::build_face_connectivity, identify_boundary_faces and
+ * set_periodic_boundaries (S/core/mesh.cpp:312-402,482-588). */
+dgx_status dgx_mesh_connect(int32_t n_vertices, const double* vertices, int32_t n_elements,
+                            const int32_t* elements, int32_t periodic_mask,
+                            int32_t* face_neighbors);
+
+/* ---- context ------------------------------------------------------------------------------- */
+
+/* Builds the space (DGSpace ctor, S/core/space.cpp:20-67), per-element geometry
+ * (compute_element_data / compute_face_data, :69-171), face permutations (:173-228), mass
+ * inverse (S/solver/compressible_solver_base.cpp:41-52) and uploads everything to the device. */
+dgx_status dgx_create(const dgx_mesh_desc* mesh, const dgx_space_desc* space,
+                      const dgx_physics* physics, int32_t precision, dgx_ctx** out);
+void dgx_destroy(dgx_ctx* ctx);
+const char* dgx_last_error(void);
+dgx_status dgx_info(const dgx_ctx* ctx, dgx_info_t* out);
+
+/* ---- state ---------------------------------------------------------------------------------- */
+
+dgx_status dgx_set_state(dgx_ctx* ctx, const double* u);  /* [K][Np][4] -> device */
+dgx_status dgx_get_state(dgx_ctx* ctx, double* u);        /* device -> [K][Np][4] */
+
+/* Physical coordinates of the volume quadrature points, [K][Nq][2]. */
+dgx_status dgx_volume_points(dgx_ctx* ctx, double* xy);
+/* L2 projection of point values Uq [K][Nq][4] into the state (project_initial_condition,
+ * S/solver/compressible_solver_base.cpp:63-104). */
+dgx_status dgx_project(dgx_ctx* ctx, const double* Uq);
+/* Current state evaluated at the volume points, [K][Nq][4]. */
+dgx_status dgx_eval_volume(dgx_ctx* ctx, double* Uq);
+
+/* ---- the hot path -------------------------------------------------------------------------- */
+
+/* R = M dU/dt of u (or of the current state when u is NULL), [K][Np][4]:
+ * DGAssembler::assemble_euler_residual (S/solver/assembler.cpp:137-279; I/solver/assembler.hpp:58-59). */
+dgx_status dgx_residual(dgx_ctx* ctx, const double* u, double* r);
+/* r <- M^{-1} r in place, [K][Np][4]: CompressibleDGSolverBase::apply_mass_inv
+ * (S/solver/compressible_solver_base.cpp:54-61). */
+dgx_status dgx_apply_minv(dgx_ctx* ctx, double* r);
+/* nsteps SSP-RK3 steps of the device state (time_step_ssp_rk3 x nsteps,
+ * S/solver/compressible_solver_base.cpp:106-112; I/solver/time_stepping.hpp:39-55).
+ * Returns DGX_E_NONFINITE if the state holds NaN/Inf afterwards (the reference's abort check,
+ * :290-301); the state is left as computed. */
+dgx_status dgx_rk3_steps(dgx_ctx* ctx, double dt, int32_t nsteps);
+/* Same, without the host round trip of the non-finite check; for timing loops and capture. */
+dgx_status dgx_rk3_steps_async(dgx_ctx* ctx, double dt, int32_t nsteps);
+dgx_status dgx_synchronize(dgx_ctx* ctx);
+
+/* Runs nsteps steps and reports the mean device duration of one fused stage launch measured
+ * with HIP events on the context stream (ms) and the wall time of the whole loop (ms). */
+dgx_status dgx_rk3_profile(dgx_ctx* ctx, double dt, int32_t nsteps, double* stage_ms,
+                           double* total_ms);
+
+/* ---- diagnostics (FP64) -------------------------------------------------------------------- */
+
+/* Integral of each conserved variable over the mesh, FP64 deterministic reduction. */
+dgx_status dgx_conservation_sums(dgx_ctx* ctx, double out[4]);
+/* compute_max_cfl / compute_density_range (S/solver/compressible_solver_base.cpp:114-199):
+ * CFL = dt * max_q(|u|+c) / h_min(e); returns +inf for rho<=0 or p<=0 like the reference. */
+dgx_status dgx_max_cfl(dgx_ctx* ctx, double dt, double* cfl);
+dgx_status dgx_density_range(dgx_ctx* ctx, double out[2]);
+/* Kinetic energy functional J = sum_e sum_q w_q |J_e| (m_x^2+m_y^2)/(2 rho), FP64. */
+dgx_status dgx_kinetic_energy(dgx_ctx* ctx, double* J);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGX_H */
