@@ -95,6 +95,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run only a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--workload", default="euler",
+                    choices=["euler", "ns", "adjoint", "batch"],
+                    help="euler: the headline vortex case; ns: Taylor-Green Navier-Stokes; "
+                         "adjoint: gradient of the final kinetic energy; batch: 64 mixed instances")
     args = ap.parse_args()
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
@@ -102,6 +106,9 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "euler":
+        {"ns": bench_ns, "adjoint": bench_adjoint, "batch": bench_batch}[args.workload](args)
+        return
 
     n, p = args.n, args.p
     workload = f"euler_vortex_n{n}_p{p}"
@@ -168,6 +175,80 @@ def main():
                                   "host_cpus": os.cpu_count()}
     ctx32.close()
     print(json.dumps(result))
+
+
+def _line(metric, value, unit, steps, warmup, ms_per_step, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out))
+
+
+def bench_ns(args):
+    """Taylor-Green vortex, compressible Navier-Stokes, 192 x 192 split quads on [0, 2 pi]^2
+    (K = 73728), p = 4, Mach 0.1, Re 1600, Pr 0.71, FP32, reference face-gradient read."""
+    n = 192 if args.n == 256 else args.n
+    p = 4 if args.p == 3 else args.p
+    steps = args.steps if args.steps != 200 else 500
+    verts, tris = dgx.split_quad_mesh(n, 2 * np.pi)
+    wf = dgx.NavierStokesWeakFormulation(1.4, 1.0 / 1600.0, 0.71, 5.0, face_grad="reference")
+    ctx = dgx.Context(verts, tris, p, weak_form=wf, basis="dubiner", precision="fp32")
+    ctx.project(W.taylor_green(ctx.volume_points()))
+    dt = dgx.stable_dt(ctx, 0.3)
+    secs = timed_steps(ctx, dt, steps, args.warmup)
+    ctx.set_state(ctx.get_state())
+    stage_ms, _ = ctx.rk3_profile(dt, 20)
+    K, Np = ctx.K, ctx.Np
+    bytes_launch = ctx.info.bytes_per_stage_alg
+    _line("NS RHS DOF-stage updates/s (FP32)", 12.0 * Np * K * steps / secs, "DOF-stage updates/s", steps,
+          args.warmup, secs / steps * 1e3, "f32",
+          {"workload": f"ns_taylor_green_n{n}_p{p}", "elements": K, "order": p, "mu": 1 / 1600, "dt": dt},
+          roofline={"bound": "hbm", "achieved": bytes_launch / (stage_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": bytes_launch / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "traffic": load_traffic(f"ns_taylor_green_n{n}_p{p}"), "launch_ms": stage_ms})
+
+
+def bench_adjoint(args):
+    """Gradient of J = int KE(t_final) w.r.t. the initial coefficients: 128 x 128 split quads
+    (K = 32768), p = 3, FP32, 200 SSP-RK3 steps, checkpoints every 20, reverse sweep with
+    recompute.  Reports adjoint time / forward time."""
+    n = 128 if args.n == 256 else args.n
+    steps = args.steps
+    ctx, _, _, _ = make_ctx(n, args.p, "fp32", SEED)
+    dt = dgx.stable_dt(ctx, 0.3)
+    fwd_ms, adj_ms = ctx.rk3_vjp_profile(dt, steps, 20)
+    K, Np = ctx.K, ctx.Np
+    _line("adjoint time / forward time (Euler, FP32)", adj_ms / fwd_ms, "ratio", steps, 1, adj_ms / steps, "f32",
+          {"workload": f"euler_vortex_adjoint_n{n}_p{args.p}", "elements": K, "order": args.p,
+           "checkpoint_every": 20, "dt": dt},
+          forward_ms=fwd_ms, adjoint_ms=adj_ms,
+          forward_dof_stage_per_s=12.0 * Np * K * steps / (fwd_ms * 1e-3))
+
+
+def bench_batch(args):
+    """64 independent vortex instances (N in {16,32,64,128}, p in {1..4}, beta in [3,6], centre
+    in [3,7]^2, drawn from splitmix64), FP32, 100 steps, batched graph-captured steps after the
+    autotuner vs the same instances run one at a time."""
+    specs = W.batch_instances(64, seed=5)
+    meshes = [dgx.split_quad_mesh(s["n"], 10.0) for s in specs]
+    b = dgx.Batch(meshes, [s["p"] for s in specs], precision="fp32")
+    dts = []
+    for ctx, s in zip(b.instances, specs):
+        ctx.project(W.vortex(ctx.volume_points(), beta=s["beta"], xc=s["xc"], yc=s["yc"]))
+        dts.append(dgx.stable_dt(ctx, 0.3))
+    b.set_dt(dts)
+    steps = 100 if args.steps == 200 else args.steps
+    default_ms = b.profile(steps)
+    tuning = b.autotune(4)
+    ms = b.profile(steps)
+    seq_ms = b.profile(steps, sequential=True)
+    dof = sum(12.0 * c.Np * c.K for c in b.instances)
+    _line("batched Euler DOF-stage updates/s (FP32, 64 instances)", dof / (ms * 1e-3), "DOF-stage updates/s", steps,
+          2, ms, "f32", {"workload": "euler_vortex_batch64", "instances": 64,
+                         "elements_total": sum(c.K for c in b.instances)},
+          sequential_value=dof / (seq_ms * 1e-3), speedup_vs_sequential=seq_ms / ms,
+          untuned_value=dof / (default_ms * 1e-3), tuning=tuning.as_dict())
 
 
 def calibrate_copy(mib):

@@ -15,58 +15,13 @@
 #include "dgx_internal.hpp"
 #include "dgx_space.hpp"
 
-using namespace dgx;
+#include "dgx_ctx.hpp"
+
+namespace dgx_detail {
+thread_local std::string g_error;
+}  // namespace dgx_detail
 
 namespace {
-
-thread_local std::string g_error;
-
-struct Fail : std::runtime_error {
-    dgx_status code;
-    Fail(dgx_status c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw Fail(DGX_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F>
-dgx_status guarded(F&& f) {
-    try {
-        f();
-        return DGX_OK;
-    } catch (const Fail& e) {
-        g_error = e.what();
-        return e.code;
-    } catch (const std::invalid_argument& e) {
-        g_error = e.what();
-        return DGX_E_INVALID;
-    } catch (const std::bad_alloc&) {
-        g_error = "out of host memory";
-        return DGX_E_NOMEM;
-    } catch (const std::exception& e) {
-        g_error = e.what();
-        return DGX_E_INVALID;
-    }
-}
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-    void alloc(size_t bytes) {
-        release();
-        if (bytes == 0) return;
-        hip_check(hipMalloc(&p, bytes), "hipMalloc");
-        n = bytes;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    template <typename T>
-    T* as() const { return static_cast<T*>(p); }
-};
 
 // Inverse of a small dense SPD/general matrix by Gauss-Jordan with partial pivoting.
 std::vector<double> invert(std::vector<double> M, int n) {
@@ -91,84 +46,6 @@ std::vector<double> invert(std::vector<double> M, int n) {
 }
 
 }  // namespace
-
-struct dgx_ctx {
-    int K = 0, Kp = 0, p = 1, rule = 0, np = 3, nq = 3, nfq = 2, basis = 1, precision = 0, physics = 0;
-    double gamma = 1.4, mu = 0, prandtl = 0.72, sigma0 = 5;
-    uint32_t quirks = 0;
-    int device = 0;
-    int n_bfaces = 0;
-    size_t rsz = 8;
-    hipStream_t stream = nullptr;
-    DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
-    std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][3][2]
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
-    ~dgx_ctx() {
-        for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage, &partial})
-            b->release();
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    size_t state_bytes() const { return (size_t)np * 4 * Kp * rsz; }
-    int red_blocks() const { return (K + 255) / 256; }
-
-    void upload(DevBuf& b, const void* src, size_t bytes) {
-        b.alloc(bytes);
-        hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
-    }
-    template <typename T>
-    void upload_real(DevBuf& b, const std::vector<double>& v) {
-        std::vector<T> t(v.begin(), v.end());
-        upload(b, t.data(), t.size() * sizeof(T));
-    }
-
-    void stage_launch(const void* uk, const void* un, void* out, int mode, double dt, double ca, double cb) {
-        StageParams sp{};
-        sp.uk = uk;
-        sp.un = un;
-        sp.out = out;
-        sp.geo = geo.as<double>();
-        sp.nbr = nbr.as<int32_t>();
-        sp.code = code.as<uint32_t>();
-        sp.tab = (mode == MODE_RESIDUAL) ? tabR.p : tabL.p;
-        sp.bc = bc.as<double>();
-        sp.K = K;
-        sp.Kp = Kp;
-        sp.dt = dt;
-        sp.ca = ca;
-        sp.cb = cb;
-        sp.gamma = gamma;
-        sp.mu = mu;
-        sp.prandtl = prandtl;
-        sp.sigma0 = sigma0;
-        sp.mode = mode;
-        sp.ns_face_quirk = (quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) ? 1 : 0;
-        hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
-    }
-
-    // One SSP-RK3 step (SSP_RK::step_rk3, I/solver/time_stepping.hpp:39-55):
-    //   u1 = u + dt L(u); u2 = 3/4 u + 1/4 (u1 + dt L(u1)); u = 1/3 u + 2/3 (u2 + dt L(u2)).
-    void rk3_step(double dt) {
-        stage_launch(u.p, u.p, u1.p, MODE_RK, dt, 0.0, 1.0);
-        stage_launch(u1.p, u.p, u2.p, MODE_RK, dt, 0.75, 0.25);
-        stage_launch(u2.p, u.p, u1.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
-        std::swap(u, u1);
-    }
-
-    std::vector<double> reduce(int kind) {
-        int nb = red_blocks();
-        hip_check(launch_reduce(kind, precision, p, rule, u.p, tab64.p, geo.as<double>(), hmin.as<double>(),
-                                qw.as<double>(), K, Kp, gamma, partial.as<double>(), nb, stream),
-                  "reduce kernel launch");
-        std::vector<double> h((size_t)nb * 4);
-        hip_check(hipMemcpyAsync(h.data(), partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "copy");
-        hip_check(hipStreamSynchronize(stream), "sync");
-        return h;
-    }
-};
 
 namespace {
 
@@ -365,6 +242,7 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
                 int bci = fn;
                 if (bci < 0) bci = m.n_bcs;  // "no boundary condition" row
                 if (bci > m.n_bcs) throw std::invalid_argument("boundary face references a missing boundary condition");
+                if (bci < m.n_bcs && m.bc_types && m.bc_types[bci] == DGX_BC_SLIP_WALL) c.has_slip = true;
                 nbr[f * Kp + e] = bci;
                 cfield = kFaceBoundary | (1u << 3);
             } else {
@@ -746,6 +624,194 @@ dgx_status dgx_kinetic_energy(dgx_ctx* c, double* J) {
         double s = 0.0;
         for (int b = 0; b < c->red_blocks(); ++b) s += h[b * 4];
         *J = s;
+    });
+}
+
+// ------------------------------------------------------------------------------------------------
+// discrete adjoint
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+void adjoint_supported(const dgx_ctx* c) {
+    if (!c) throw std::invalid_argument("null context");
+    if (c->physics != DGX_EULER) throw Fail(DGX_E_UNSUPPORTED, "the adjoint is built for the Euler residual only");
+    if (c->has_slip) throw Fail(DGX_E_UNSUPPORTED, "the adjoint does not differentiate slip-wall ghost states");
+}
+
+// host [K][Np][4] -> device SoA in `dst`
+void to_device(dgx_ctx* c, const double* h, void* dst) {
+    size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
+    hip_check(hipMemcpyAsync(c->stage.p, h, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+    hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), dst, c->K, c->Kp, c->stream), "pack");
+}
+void to_host(dgx_ctx* c, const void* src, double* h) {
+    size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
+    hip_check(launch_unpack(c->precision, c->np, src, c->stage.as<double>(), c->K, c->Kp, c->stream), "unpack");
+    hip_check(hipMemcpyAsync(h, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+void vjp(dgx_ctx* c, const void* uk, const void* lam, const void* add, const void* acc_in, void* acc_out, void* out,
+         int mode, double dt, double ca, double cb) {
+    hip_check(launch_vjp(c->precision, c->p, c->rule, uk, lam, add, acc_in, acc_out, out, c->geo.as<double>(),
+                         c->nbr.as<int32_t>(), c->code.as<uint32_t>(), mode == MODE_RESIDUAL ? c->tabR.p : c->tabL.p,
+                         c->bc.as<double>(), c->K, c->Kp, dt, ca, cb, c->gamma, mode, c->stream),
+              "vjp kernel launch");
+}
+
+struct AdjointBuffers {
+    std::vector<DevBuf> ckpt, stages;
+    DevBuf l0, l1, l2, acc;
+};
+
+// Reverse sweep of nsteps SSP-RK3 steps from the state in c->u with checkpoints every
+// `every` steps; `lam` holds lambda_T on entry (device, in b.l0) and lambda_0 on exit.
+void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every) {
+    const size_t sb = c->state_bytes();
+    const int nck = (nsteps + every - 1) / every;
+    if (b.ckpt.size() < (size_t)nck) b.ckpt.resize(nck);
+    for (int j = 0; j < nck; ++j) if (!b.ckpt[j].p) b.ckpt[j].alloc(sb);
+    if (b.stages.size() < (size_t)(3 * every + 1)) b.stages.resize(3 * every + 1);
+    for (auto& s : b.stages) if (!s.p) s.alloc(sb);
+    // forward with checkpoints
+    for (int s = 0; s < nsteps; ++s) {
+        if (s % every == 0)
+            hip_check(hipMemcpyAsync(b.ckpt[s / every].p, c->u.p, sb, hipMemcpyDeviceToDevice, c->stream), "ckpt");
+        c->rk3_step(dt);
+    }
+    // reverse sweep, segment by segment
+    for (int j = nck - 1; j >= 0; --j) {
+        const int s0 = j * every, len = std::min(every, nsteps - s0);
+        hip_check(hipMemcpyAsync(b.stages[0].p, b.ckpt[j].p, sb, hipMemcpyDeviceToDevice, c->stream), "restore");
+        for (int k = 0; k < len; ++k) {
+            void* v0 = b.stages[3 * k].p;
+            void* v1 = b.stages[3 * k + 1].p;
+            void* v2 = b.stages[3 * k + 2].p;
+            c->stage_launch(v0, v0, v1, MODE_RK, dt, 0.0, 1.0);
+            c->stage_launch(v1, v0, v2, MODE_RK, dt, 0.75, 0.25);
+            if (k + 1 < len) c->stage_launch(v2, v0, b.stages[3 * k + 3].p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
+        }
+        for (int k = len - 1; k >= 0; --k) {
+            void* v0 = b.stages[3 * k].p;
+            void* v1 = b.stages[3 * k + 1].p;
+            void* v2 = b.stages[3 * k + 2].p;
+            vjp(c, v2, b.l0.p, nullptr, nullptr, b.acc.p, b.l1.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
+            vjp(c, v1, b.l1.p, nullptr, b.acc.p, b.acc.p, b.l2.p, MODE_RK, dt, 0.75, 0.25);
+            vjp(c, v0, b.l2.p, b.acc.p, nullptr, nullptr, b.l0.p, MODE_RK, dt, 0.0, 1.0);
+        }
+    }
+}
+
+}  // namespace
+
+dgx_status dgx_residual_vjp(dgx_ctx* c, const double* u, const double* lam, double* out) {
+    return guarded([&] {
+        adjoint_supported(c);
+        if (!lam || !out) throw std::invalid_argument("null argument");
+        const void* src = c->u.p;
+        if (u) { to_device(c, u, c->u2.p); src = c->u2.p; }
+        to_device(c, lam, c->u1.p);
+        DevBuf o;
+        o.alloc(c->state_bytes());
+        vjp(c, src, c->u1.p, nullptr, nullptr, nullptr, o.p, MODE_RESIDUAL, 0, 0, 0);
+        to_host(c, o.p, out);
+    });
+}
+
+dgx_status dgx_rhs_vjp(dgx_ctx* c, const double* u, const double* lam, double* out) {
+    return guarded([&] {
+        adjoint_supported(c);
+        if (!lam || !out) throw std::invalid_argument("null argument");
+        const void* src = c->u.p;
+        if (u) { to_device(c, u, c->u2.p); src = c->u2.p; }
+        to_device(c, lam, c->u1.p);
+        DevBuf o;
+        o.alloc(c->state_bytes());
+        vjp(c, src, c->u1.p, nullptr, nullptr, nullptr, o.p, MODE_RHS, 0, 0, 0);
+        to_host(c, o.p, out);
+    });
+}
+
+dgx_status dgx_kinetic_energy_grad(dgx_ctx* c, double* J, double* grad) {
+    return guarded([&] {
+        if (!c) throw std::invalid_argument("null context");
+        if (J) {
+            std::vector<double> h = c->reduce(1);
+            double s = 0.0;
+            for (int b = 0; b < c->red_blocks(); ++b) s += h[b * 4];
+            *J = s;
+        }
+        if (grad) {
+            TabLayout L(c->p, c->rule);
+            hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
+                                     c->geo.as<double>(), c->u1.p, c->K, c->Kp, c->stream),
+                      "ke grad");
+            to_host(c, c->u1.p, grad);
+        }
+    });
+}
+
+dgx_status dgx_rk3_vjp(dgx_ctx* c, double dt, int32_t nsteps, int32_t every, const double* lambda_T, double* lambda_0) {
+    return guarded([&] {
+        adjoint_supported(c);
+        if (nsteps <= 0 || every <= 0 || !lambda_0) throw std::invalid_argument("bad adjoint arguments");
+        AdjointBuffers b;
+        for (DevBuf* x : {&b.l0, &b.l1, &b.l2, &b.acc}) x->alloc(c->state_bytes());
+        // seed: lambda_T given, or the kinetic-energy gradient at u_T (computed after the forward)
+        if (lambda_T) {
+            to_device(c, lambda_T, b.l0.p);
+            rk3_adjoint(c, b, dt, nsteps, every);
+        } else {
+            // forward once to reach u_T, seed, then restart from the saved initial state
+            DevBuf u0;
+            u0.alloc(c->state_bytes());
+            hip_check(hipMemcpyAsync(u0.p, c->u.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+            for (int s = 0; s < nsteps; ++s) c->rk3_step(dt);
+            hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
+                                     c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->stream),
+                      "ke grad");
+            hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+            rk3_adjoint(c, b, dt, nsteps, every);
+        }
+        to_host(c, b.l0.p, lambda_0);
+    });
+}
+
+dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t every, double* fwd_ms, double* adj_ms) {
+    return guarded([&] {
+        adjoint_supported(c);
+        if (nsteps <= 0 || every <= 0) throw std::invalid_argument("bad adjoint arguments");
+        AdjointBuffers b;
+        for (DevBuf* x : {&b.l0, &b.l1, &b.l2, &b.acc}) x->alloc(c->state_bytes());
+        DevBuf u0;
+        u0.alloc(c->state_bytes());
+        hip_check(hipMemcpyAsync(u0.p, c->u.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        // warm-up allocations of the checkpoint and stage stores with a one-segment sweep
+        hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
+                                 c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->stream), "ke grad");
+        rk3_adjoint(c, b, dt, std::min(nsteps, every), every);
+        hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        // forward only
+        hip_check(hipEventRecord(c->ev0, c->stream), "record");
+        for (int s = 0; s < nsteps; ++s) c->rk3_step(dt);
+        hip_check(hipEventRecord(c->ev1, c->stream), "record");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        float f = 0;
+        hip_check(hipEventElapsedTime(&f, c->ev0, c->ev1), "elapsed");
+        // adjoint (forward with checkpoints + recompute + reverse), seeded at u_T
+        hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
+                                 c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->stream), "ke grad");
+        hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        hip_check(hipEventRecord(c->ev0, c->stream), "record");
+        rk3_adjoint(c, b, dt, nsteps, every);
+        hip_check(hipEventRecord(c->ev1, c->stream), "record");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        float a = 0;
+        hip_check(hipEventElapsedTime(&a, c->ev0, c->ev1), "elapsed");
+        hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        if (fwd_ms) *fwd_ms = f;
+        if (adj_ms) *adj_ms = a;
     });
 }
 

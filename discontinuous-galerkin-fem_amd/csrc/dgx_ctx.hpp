@@ -1,0 +1,164 @@
+// Private: the context object behind the dgx C ABI, device buffers and error plumbing shared by
+// the host translation units (dgx_api.cpp, dgx_batch_api.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/dgx.h"
+#include "dgx_internal.hpp"
+
+using namespace dgx;
+
+namespace dgx_detail {
+
+extern thread_local std::string g_error;
+
+struct Fail : std::runtime_error {
+    dgx_status code;
+    Fail(dgx_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Fail(DGX_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+inline dgx_status guarded(F&& f) {
+    try {
+        f();
+        return DGX_OK;
+    } catch (const Fail& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return DGX_E_INVALID;
+    } catch (const std::bad_alloc&) {
+        g_error = "out of host memory";
+        return DGX_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return DGX_E_INVALID;
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t bytes) {
+        release();
+        if (bytes == 0) return;
+        hip_check(hipMalloc(&p, bytes), "hipMalloc");
+        n = bytes;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace dgx_detail
+
+using namespace dgx_detail;
+
+struct dgx_ctx {
+    int K = 0, Kp = 0, p = 1, rule = 0, np = 3, nq = 3, nfq = 2, basis = 1, precision = 0, physics = 0;
+    double gamma = 1.4, mu = 0, prandtl = 0.72, sigma0 = 5;
+    uint32_t quirks = 0;
+    int device = 0;
+    int n_bfaces = 0;
+    bool has_slip = false;  // a boundary face uses a slip-wall condition
+    double dt = 0.0;        // time step of this instance inside a batch
+    size_t rsz = 8;
+    hipStream_t stream = nullptr;
+    DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
+    std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][3][2]
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~dgx_ctx() {
+        for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage, &partial})
+            b->release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    size_t state_bytes() const { return (size_t)np * 4 * Kp * rsz; }
+    int red_blocks() const { return (K + 255) / 256; }
+
+    void upload(DevBuf& b, const void* src, size_t bytes) {
+        b.alloc(bytes);
+        hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    template <typename T>
+    void upload_real(DevBuf& b, const std::vector<double>& v) {
+        std::vector<T> t(v.begin(), v.end());
+        upload(b, t.data(), t.size() * sizeof(T));
+    }
+
+    void stage_launch(const void* uk, const void* un, void* out, int mode, double dt, double ca, double cb) {
+        StageParams sp{};
+        sp.uk = uk;
+        sp.un = un;
+        sp.out = out;
+        sp.geo = geo.as<double>();
+        sp.nbr = nbr.as<int32_t>();
+        sp.code = code.as<uint32_t>();
+        sp.tab = (mode == MODE_RESIDUAL) ? tabR.p : tabL.p;
+        sp.bc = bc.as<double>();
+        sp.K = K;
+        sp.Kp = Kp;
+        sp.dt = dt;
+        sp.ca = ca;
+        sp.cb = cb;
+        sp.gamma = gamma;
+        sp.mu = mu;
+        sp.prandtl = prandtl;
+        sp.sigma0 = sigma0;
+        sp.mode = mode;
+        sp.ns_face_quirk = (quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) ? 1 : 0;
+        hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
+    }
+
+    // One SSP-RK3 step (SSP_RK::step_rk3, I/solver/time_stepping.hpp:39-55):
+    //   u1 = u + dt L(u); u2 = 3/4 u + 1/4 (u1 + dt L(u1)); u = 1/3 u + 2/3 (u2 + dt L(u2)).
+    void rk3_step(double dt) {
+        stage_launch(u.p, u.p, u1.p, MODE_RK, dt, 0.0, 1.0);
+        stage_launch(u1.p, u.p, u2.p, MODE_RK, dt, 0.75, 0.25);
+        stage_launch(u2.p, u.p, u1.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
+        std::swap(u, u1);
+    }
+
+    std::vector<double> reduce(int kind) {
+        int nb = red_blocks();
+        hip_check(launch_reduce(kind, precision, p, rule, u.p, tab64.p, geo.as<double>(), hmin.as<double>(),
+                                qw.as<double>(), K, Kp, gamma, partial.as<double>(), nb, stream),
+                  "reduce kernel launch");
+        std::vector<double> h((size_t)nb * 4);
+        hip_check(hipMemcpyAsync(h.data(), partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "copy");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        return h;
+    }
+};

@@ -95,5 +95,24 @@ hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u
                          double gamma, double* partial, int nblocks, hipStream_t s);
 hipError_t launch_apply_minv(int precision, int np, const double* minv_ref, const double* geo,
                              double* r_dev, int K, int Kp, hipStream_t s);
+// Euler stage adjoint (dgx_vjp.hip): out = cb (lam + dt (dL/du)^T lam) + add,
+// acc_out = acc_in + ca lam (mode RK); out = (dL/du)^T lam (mode RHS) or (dR/du)^T lam
+// (mode RESIDUAL, unfolded tables).
+hipError_t launch_vjp(int precision, int p, int rule, const void* uk, const void* lam, const void* add,
+                      const void* acc_in, void* acc_out, void* out, const double* geo, const int32_t* nbr,
+                      const uint32_t* code, const void* tab, const double* bc, int K, int Kp, double dt,
+                      double ca, double cb, double gamma, int mode, hipStream_t s);
+// Batched persistent stage (dgx_batch.hip): items of (instance, first element, count, order);
+// params[instance] holds that instance's stage parameters; tabs[p-1] the order-p tables.
+struct BatchItem {
+    int32_t instance, e0, count, order;
+};
+hipError_t launch_batch_stage(int precision, const StageParams* params, const BatchItem* items, int n_items,
+                              const void* const tabs[4], int order_mask, int grid, hipStream_t s);
+hipError_t launch_batch_stage_order(int precision, int p, const StageParams* params, const BatchItem* items,
+                                    int n_items, const void* tab, int grid, hipStream_t s);
+// gradient of the kinetic-energy functional with respect to the coefficients (device layout)
+hipError_t launch_ke_grad(int precision, int p, int rule, const void* u, const double* tab64, const double* qw,
+                          const double* geo, void* grad, int K, int Kp, hipStream_t s);
 
 }  // namespace dgx

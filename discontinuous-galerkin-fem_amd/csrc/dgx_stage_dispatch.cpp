@@ -1,0 +1,20 @@
+// Stage-kernel dispatch over the (precision, physics) instantiation units.
+#include "dgx_internal.hpp"
+
+namespace dgx {
+
+hipError_t launch_stage_f32_euler(int p, int rule, bool quirk, const StageParams& sp, hipStream_t s);
+hipError_t launch_stage_f64_euler(int p, int rule, bool quirk, const StageParams& sp, hipStream_t s);
+hipError_t launch_stage_f32_ns(int p, int rule, bool quirk, const StageParams& sp, hipStream_t s);
+hipError_t launch_stage_f64_ns(int p, int rule, bool quirk, const StageParams& sp, hipStream_t s);
+
+hipError_t launch_stage(int precision, int physics, int p, int rule, const StageParams& sp, hipStream_t s) {
+    if (sp.K <= 0) return hipSuccess;
+    const bool quirk = sp.ns_face_quirk != 0;
+    const bool f64 = precision == DGX_FP64;
+    if (physics == DGX_EULER)
+        return f64 ? launch_stage_f64_euler(p, rule, quirk, sp, s) : launch_stage_f32_euler(p, rule, quirk, sp, s);
+    return f64 ? launch_stage_f64_ns(p, rule, quirk, sp, s) : launch_stage_f32_ns(p, rule, quirk, sp, s);
+}
+
+}  // namespace dgx

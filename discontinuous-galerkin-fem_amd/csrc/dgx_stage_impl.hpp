@@ -1,4 +1,5 @@
-// Fused DG stage kernel for gfx950: volume residual + face residuals (Rusanov, and for
+// Fused DG stage kernel for gfx950 (implementation header, included once per
+// precision/physics family by dgx_stage_*.hip): volume residual + face residuals (Rusanov, and for
 // Navier-Stokes the viscous volume/face terms) + inverse mass + SSP-RK3 stage update, one
 // thread per element, one launch per RK stage.
 //
@@ -18,6 +19,7 @@
 // the neighbour's trace from the neighbour's coefficients.  The flux of a face is always formed
 // in the orientation of its canonical (L) element, with the L-side normal, and enters the R
 // element with the opposite sign, so both copies come from the same arithmetic.
+#pragma once
 #include "dgx_internal.hpp"
 
 namespace dgx {
@@ -130,21 +132,55 @@ __device__ __forceinline__ void face_gradient(const Real (&u)[NP][4], const doub
     }
 }
 
+// Compile-time table offsets (TabLayout).
+template <int P, int RULE, bool NS>
+struct StageTab {
+    static constexpr int NP = np_of(P);
+    static constexpr int NQ = nq_of(P, RULE);
+    static constexpr int NFQ = P + 1;
+    static constexpr int phiV = 0;
+    static constexpr int PX = phiV + NQ * NP;
+    static constexpr int PY = PX + NQ * NP;
+    static constexpr int phiF = PY + NQ * NP;
+    static constexpr int LF = phiF + 3 * NFQ * NP;
+    static constexpr int DXV = LF + 3 * NFQ * NP;
+    static constexpr int DYV = DXV + NQ * NP;
+    static constexpr int FG = DYV + NQ * NP;  // face-gradient tables [3][NFQ][8][NP]
+    static constexpr int NFG = NS ? 3 * NFQ * 8 * NP : 1;
+    static constexpr int NF = 3 * NFQ * NP;
+};
+
+// Neighbour-side tables into LDS (the neighbour's local face and point order vary per lane):
+// s_phiF [face][point][mode], s_phiFT [face][mode][point], s_FG face gradients (NS).
+template <int P, int RULE, typename Real, bool NS>
+__device__ __forceinline__ void stage_tables_to_lds(const Real* __restrict__ tab, Real* s_phiF, Real* s_phiFT,
+                                                    Real* s_FG, int tid, int nthreads) {
+    using T = StageTab<P, RULE, NS>;
+    for (int t = tid; t < T::NF; t += nthreads) {
+        s_phiF[t] = tab[T::phiF + t];
+        const int f = t / (T::NFQ * T::NP), q = (t / T::NP) % T::NFQ, i = t % T::NP;
+        s_phiFT[(f * T::NP + i) * T::NFQ + q] = tab[T::phiF + t];
+    }
+    if constexpr (NS)
+        for (int t = tid; t < T::NFG; t += nthreads) s_FG[t] = tab[T::FG + t];
+}
+
+// One element's fused stage: volume + faces + M^-1 + RK update (or RHS / residual output).
 template <int P, int RULE, typename Real, bool NS, bool QUIRK>
-__global__ __launch_bounds__(kBlock) void stage_kernel(StageParams sp) {
-    constexpr int NP = np_of(P);
-    constexpr int NQ = nq_of(P, RULE);
-    constexpr int NFQ = P + 1;
-    // table offsets (TabLayout, compile-time)
-    constexpr int O_phiV = 0;
-    constexpr int O_PX = O_phiV + NQ * NP;
-    constexpr int O_PY = O_PX + NQ * NP;
-    constexpr int O_phiF = O_PY + NQ * NP;
-    constexpr int O_LF = O_phiF + 3 * NFQ * NP;
-    constexpr int O_DXV = O_LF + 3 * NFQ * NP;
-    constexpr int O_DYV = O_DXV + NQ * NP;
-    constexpr int O_FG = O_DYV + NQ * NP;  // face-gradient tables [3][NFQ][8][NP]
-    constexpr int NFG = NS ? 3 * NFQ * 8 * NP : 1;
+__device__ __forceinline__ void stage_element(const StageParams& sp, const int e, const double dt_e,
+                                              const Real* s_phiF, const Real* s_phiFT, const Real* s_FG) {
+    using T = StageTab<P, RULE, NS>;
+    constexpr int NP = T::NP;
+    constexpr int NQ = T::NQ;
+    constexpr int NFQ = T::NFQ;
+    constexpr int O_phiV = T::phiV;
+    constexpr int O_PX = T::PX;
+    constexpr int O_PY = T::PY;
+    constexpr int O_phiF = T::phiF;
+    constexpr int O_LF = T::LF;
+    constexpr int O_DXV = T::DXV;
+    constexpr int O_DYV = T::DYV;
+    constexpr int O_FG = T::FG;
 
     const Real* __restrict__ tab = static_cast<const Real*>(sp.tab);
     const Real* __restrict__ uk = static_cast<const Real*>(sp.uk);
@@ -152,17 +188,6 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(StageParams sp) {
     Real* __restrict__ out = static_cast<Real*>(sp.out);
     const double* __restrict__ geo = sp.geo;
     const int Kp = sp.Kp;
-
-    // neighbour-side tables in LDS (lane-varying local face / point order)
-    __shared__ Real s_phiF[3 * NFQ * NP];
-    __shared__ Real s_FG[NFG];
-    for (int t = threadIdx.x; t < 3 * NFQ * NP; t += kBlock) s_phiF[t] = tab[O_phiF + t];
-    if constexpr (NS)
-        for (int t = threadIdx.x; t < NFG; t += kBlock) s_FG[t] = tab[O_FG + t];
-    __syncthreads();
-
-    const int e = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (e >= sp.K) return;
 
     const Real gam = Real(sp.gamma);
     const Real mu = Real(sp.mu);
@@ -244,7 +269,9 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(StageParams sp) {
 
     // ---------------- faces ----------------------------------------------------------------
     const uint32_t code = sp.code[e];
-#pragma unroll
+    // the face index stays a (wave-uniform) runtime value: table reads remain scalar loads and
+    // only one face's traces are live at a time
+#pragma unroll 1
     for (int f = 0; f < 3; ++f) {
         const uint32_t c = (code >> (4 * f)) & 15u;
         const int fn = int(c & 3u);
@@ -271,25 +298,49 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(StageParams sp) {
         }
 
         Real Un[NFQ][4];
-        Real un_c[NP][4];
+        Real un_c[NS ? NP : 1][4];
         double An[4] = {0, 0, 0, 0};
         bool boundary = (fn == int(kFaceBoundary));
         if (!boundary) {
+            if constexpr (!NS) {
+                // stream the neighbour's coefficients mode by mode: only its trace stays live
 #pragma unroll
-            for (int i = 0; i < NP; ++i)
+                for (int q = 0; q < NFQ; ++q)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) un_c[i][v] = uk[(i * 4 + v) * Kp + nb];
-#pragma unroll
-            for (int q = 0; q < NFQ; ++q) {
-                const int qq = flip ? (NFQ - 1 - q) : q;
-                const Real* row = &s_phiF[(fn * NFQ + qq) * NP];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) Un[q][v] = Real(0);
+                    for (int v = 0; v < 4; ++v) Un[q][v] = Real(0);
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
-                    Real ph = row[i];
+                    Real cf[4];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) Un[q][v] = fma(ph, un_c[i][v], Un[q][v]);
+                    for (int v = 0; v < 4; ++v) cf[v] = uk[(i * 4 + v) * Kp + nb];
+                    const Real* col = &s_phiFT[(fn * NP + i) * NFQ];
+                    Real ph[NFQ];
+#pragma unroll
+                    for (int q = 0; q < NFQ; ++q) ph[q] = col[q];
+#pragma unroll
+                    for (int q = 0; q < NFQ; ++q) {
+                        const Real t = flip ? ph[NFQ - 1 - q] : ph[q];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) Un[q][v] = fma(t, cf[v], Un[q][v]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) un_c[i][v] = uk[(i * 4 + v) * Kp + nb];
+#pragma unroll
+                for (int q = 0; q < NFQ; ++q) {
+                    const int qq = flip ? (NFQ - 1 - q) : q;
+                    const Real* row = &s_phiF[(fn * NFQ + qq) * NP];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Un[q][v] = Real(0);
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        Real ph = row[i];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) Un[q][v] = fma(ph, un_c[i][v], Un[q][v]);
+                    }
                 }
             }
             if constexpr (NS) {
@@ -408,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(StageParams sp) {
 
     // ---------------- output ---------------------------------------------------------------
     if (sp.mode == MODE_RK) {
-        const Real dt = Real(sp.dt), ca = Real(sp.ca), cb = Real(sp.cb);
+        const Real dt = Real(dt_e), ca = Real(sp.ca), cb = Real(sp.cb);
         if (sp.ca == 0.0) {
 #pragma unroll
             for (int i = 0; i < NP; ++i)
@@ -431,46 +482,39 @@ __global__ __launch_bounds__(kBlock) void stage_kernel(StageParams sp) {
     }
 }
 
-template <int P, int RULE, typename Real>
-hipError_t launch_p(int physics, bool quirk, const StageParams& sp, hipStream_t s) {
+template <int P, int RULE, typename Real, bool NS, bool QUIRK>
+__global__ __launch_bounds__(kBlock) void stage_kernel(StageParams sp) {
+    using T = StageTab<P, RULE, NS>;
+    __shared__ Real s_phiF[T::NF];
+    __shared__ Real s_phiFT[T::NF];
+    __shared__ Real s_FG[T::NFG];
+    stage_tables_to_lds<P, RULE, Real, NS>(static_cast<const Real*>(sp.tab), s_phiF, s_phiFT, s_FG, threadIdx.x, kBlock);
+    __syncthreads();
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (e >= sp.K) return;
+    stage_element<P, RULE, Real, NS, QUIRK>(sp, e, sp.dt, s_phiF, s_phiFT, s_FG);
+}
+
+// Launch of one (precision, physics) family for any p and volume rule.
+template <typename Real, bool NS, bool QUIRK>
+hipError_t launch_family(int p, int rule, const StageParams& sp, hipStream_t s) {
     dim3 grid((sp.K + kBlock - 1) / kBlock), block(kBlock);
-    if (physics == DGX_EULER)
-        hipLaunchKernelGGL((stage_kernel<P, RULE, Real, false, false>), grid, block, 0, s, sp);
-    else if (quirk)
-        hipLaunchKernelGGL((stage_kernel<P, RULE, Real, true, true>), grid, block, 0, s, sp);
-    else
-        hipLaunchKernelGGL((stage_kernel<P, RULE, Real, true, false>), grid, block, 0, s, sp);
+#define DGX_LAUNCH(PP, RR) \
+    hipLaunchKernelGGL((stage_kernel<PP, RR, Real, NS, QUIRK>), grid, block, 0, s, sp)
+    switch (rule * 8 + p) {
+    case 1: DGX_LAUNCH(1, 0); break;
+    case 2: DGX_LAUNCH(2, 0); break;
+    case 3: DGX_LAUNCH(3, 0); break;
+    case 4: DGX_LAUNCH(4, 0); break;
+    case 9: DGX_LAUNCH(1, 1); break;
+    case 10: DGX_LAUNCH(2, 1); break;
+    case 11: DGX_LAUNCH(3, 1); break;
+    case 12: DGX_LAUNCH(4, 1); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef DGX_LAUNCH
     return hipGetLastError();
 }
 
-template <typename Real>
-hipError_t launch_real(int physics, int p, int rule, const StageParams& sp, hipStream_t s) {
-    bool q = sp.ns_face_quirk != 0;
-    if (rule == 0) {
-        switch (p) {
-        case 1: return launch_p<1, 0, Real>(physics, q, sp, s);
-        case 2: return launch_p<2, 0, Real>(physics, q, sp, s);
-        case 3: return launch_p<3, 0, Real>(physics, q, sp, s);
-        case 4: return launch_p<4, 0, Real>(physics, q, sp, s);
-        }
-    } else {
-        switch (p) {
-        case 1: return launch_p<1, 1, Real>(physics, q, sp, s);
-        case 2: return launch_p<2, 1, Real>(physics, q, sp, s);
-        case 3: return launch_p<3, 1, Real>(physics, q, sp, s);
-        case 4: return launch_p<4, 1, Real>(physics, q, sp, s);
-        }
-    }
-    return hipErrorInvalidValue;
-}
-
 }  // namespace
-
-hipError_t launch_stage(int precision, int physics, int p, int rule, const StageParams& sp,
-                        hipStream_t s) {
-    if (sp.K <= 0) return hipSuccess;
-    if (precision == DGX_FP64) return launch_real<double>(physics, p, rule, sp, s);
-    return launch_real<float>(physics, p, rule, sp, s);
-}
-
 }  // namespace dgx

@@ -36,7 +36,11 @@ EXPORTS = [
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
     "dgx_eval_volume", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
     "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
-    "dgx_density_range", "dgx_kinetic_energy",
+    "dgx_density_range", "dgx_kinetic_energy", "dgx_residual_vjp", "dgx_rhs_vjp",
+    "dgx_kinetic_energy_grad", "dgx_rk3_vjp", "dgx_rk3_vjp_profile",
+    "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_instance", "dgx_batch_set_dt",
+    "dgx_batch_rk3_steps", "dgx_batch_profile", "dgx_batch_get_tuning", "dgx_batch_set_tuning",
+    "dgx_batch_autotune",
 ]
 
 
@@ -60,6 +64,14 @@ class SpaceDesc(C.Structure):
 class Physics(C.Structure):
     _fields_ = [("kind", C.c_int32), ("gamma", C.c_double), ("mu", C.c_double),
                 ("prandtl", C.c_double), ("sigma0", C.c_double), ("quirks", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("layout", C.c_int32), ("chunk", C.c_int32 * 4), ("blocks_per_cu", C.c_int32)]
+
+    def as_dict(self):
+        return {"layout": ["fused", "per_order"][self.layout], "chunk": list(self.chunk),
+                "blocks_per_cu": self.blocks_per_cu}
 
 
 class Info(C.Structure):
@@ -99,9 +111,25 @@ def load():
     L.dgx_synchronize.argtypes = [vp]
     L.dgx_rk3_profile.argtypes = [vp, C.c_double, C.c_int32, dp, dp]
     L.dgx_max_cfl.argtypes = [vp, C.c_double, dp]
+    L.dgx_residual_vjp.argtypes = [vp, dp, dp, dp]
+    L.dgx_rhs_vjp.argtypes = [vp, dp, dp, dp]
+    L.dgx_kinetic_energy_grad.argtypes = [vp, dp, dp]
+    L.dgx_rk3_vjp.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, dp, dp]
+    L.dgx_rk3_vjp_profile.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, dp, dp]
+    L.dgx_batch_create.argtypes = [C.c_int32, C.POINTER(MeshDesc), C.POINTER(SpaceDesc), C.POINTER(Physics),
+                                   C.c_int32, C.POINTER(vp)]
+    L.dgx_batch_destroy.argtypes = [vp]
+    L.dgx_batch_destroy.restype = None
+    L.dgx_batch_instance.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    L.dgx_batch_set_dt.argtypes = [vp, dp]
+    L.dgx_batch_rk3_steps.argtypes = [vp, C.c_int32]
+    L.dgx_batch_profile.argtypes = [vp, C.c_int32, C.c_int32, dp]
+    L.dgx_batch_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+    L.dgx_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+    L.dgx_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(BatchTuning)]
     for n in EXPORTS:
         f = getattr(L, n)
-        if n not in ("dgx_destroy", "dgx_last_error"):
+        if n not in ("dgx_destroy", "dgx_last_error", "dgx_batch_destroy"):
             f.restype = C.c_int32
     _lib = L
     return L
@@ -299,6 +327,123 @@ class Context:
         o = np.zeros(1)
         _check(load().dgx_kinetic_energy(self._h, _dp(o)))
         return float(o[0])
+
+    # discrete adjoint (Euler)
+    def residual_vjp(self, lam, u=None):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        out = np.zeros((self.K, self.Np, 4))
+        up = None if u is None else _dp(np.ascontiguousarray(u, dtype=np.float64))
+        _check(load().dgx_residual_vjp(self._h, up, _dp(lam), _dp(out)))
+        return out
+
+    def rhs_vjp(self, lam, u=None):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        out = np.zeros((self.K, self.Np, 4))
+        up = None if u is None else _dp(np.ascontiguousarray(u, dtype=np.float64))
+        _check(load().dgx_rhs_vjp(self._h, up, _dp(lam), _dp(out)))
+        return out
+
+    def kinetic_energy_grad(self):
+        J = np.zeros(1)
+        g = np.zeros((self.K, self.Np, 4))
+        _check(load().dgx_kinetic_energy_grad(self._h, _dp(J), _dp(g)))
+        return float(J[0]), g
+
+    def rk3_vjp(self, dt, nsteps, ckpt_every, lambda_T=None):
+        """lambda_0 = (du_T/du_0)^T lambda_T from the current state; leaves u_T in the context.
+        lambda_T None seeds with the kinetic-energy gradient at u_T."""
+        out = np.zeros((self.K, self.Np, 4))
+        lt = None if lambda_T is None else _dp(np.ascontiguousarray(lambda_T, dtype=np.float64))
+        _check(load().dgx_rk3_vjp(self._h, dt, nsteps, ckpt_every, lt, _dp(out)))
+        return out
+
+    def rk3_vjp_profile(self, dt, nsteps, ckpt_every):
+        f, a = C.c_double(), C.c_double()
+        _check(load().dgx_rk3_vjp_profile(self._h, dt, nsteps, ckpt_every, C.byref(f), C.byref(a)))
+        return f.value, a.value
+
+
+class _Borrowed(Context):
+    """A Context view of one batch instance (the batch owns the device buffers)."""
+
+    def __init__(self, handle, order, batch):
+        self._h = handle
+        self._batch = batch  # keep the owner alive
+        self.order = order
+        info = Info()
+        _check(load().dgx_info(handle, C.byref(info)))
+        self.info = info
+        self.K, self.Np, self.Nq, self.Nfq = info.n_elements, info.n_modes, info.n_volume_points, info.n_face_points
+        self.precision = ["fp64", "fp32"][info.precision]
+
+    def close(self):
+        self._h = None
+
+
+class Batch:
+    """Many independent Euler instances stepped together (one persistent launch per RK stage,
+    graph-captured).  meshes: list of (verts, tris); orders: list of p."""
+
+    def __init__(self, meshes, orders, gamma=1.4, precision="fp32", basis="dubiner", periodic=3):
+        L = load()
+        n = len(meshes)
+        self._keep = []
+        descs = (MeshDesc * n)()
+        spaces = (SpaceDesc * n)()
+        for i, ((v, t), p) in enumerate(zip(meshes, orders)):
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            t = np.ascontiguousarray(t, dtype=np.int32)
+            nb = connect(v, t, periodic)
+            bt = np.full(4, -1, dtype=np.int32)
+            bv = np.zeros((4, 4))
+            self._keep += [v, t, nb, bt, bv]
+            descs[i] = MeshDesc(len(v), _dp(v), len(t), _ip(t), _ip(nb), 4, _ip(bt), _dp(bv))
+            spaces[i] = SpaceDesc(p, BASIS[basis], 0)
+        phys = Physics(0, gamma, 0.0, 0.72, 5.0, 0)
+        h = C.c_void_p()
+        _check(L.dgx_batch_create(n, descs, spaces, C.byref(phys), PRECISION[precision], C.byref(h)))
+        self._h = h
+        self.instances = []
+        for i in range(n):
+            ch = C.c_void_p()
+            _check(L.dgx_batch_instance(h, i, C.byref(ch)))
+            self.instances.append(_Borrowed(ch, orders[i], self))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for c in self.instances:
+                c.close()
+            load().dgx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_dt(self, dts):
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        _check(load().dgx_batch_set_dt(self._h, _dp(d)))
+
+    def rk3_steps(self, nsteps):
+        _check(load().dgx_batch_rk3_steps(self._h, nsteps))
+
+    def profile(self, nsteps, sequential=False):
+        ms = C.c_double()
+        _check(load().dgx_batch_profile(self._h, nsteps, int(sequential), C.byref(ms)))
+        return ms.value
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(load().dgx_batch_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_tuning(self, layout="fused", chunk=(4, 4, 2, 2), blocks_per_cu=2):
+        t = BatchTuning({"fused": 0, "per_order": 1}[layout], (C.c_int32 * 4)(*chunk), blocks_per_cu)
+        _check(load().dgx_batch_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, trial_steps=4):
+        t = BatchTuning()
+        _check(load().dgx_batch_autotune(self._h, trial_steps, C.byref(t)))
+        return t
 
 
 def stable_dt(ctx, cfl=0.3):

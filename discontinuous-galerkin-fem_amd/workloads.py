@@ -51,6 +51,24 @@ def vortex(xy, beta=5.0, xc=5.0, yc=5.0, gamma=1.4, t=0.0, length=10.0, x0=0.0, 
     return np.stack([rho, rho * u, rho * v, E], axis=-1)
 
 
+def _unit(z):
+    return (np.asarray(z, dtype=np.uint64) >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+
+
+def batch_instances(n_instances=64, seed=5, sizes=(16, 32, 64, 128), orders=(1, 2, 3, 4)):
+    """Instance parameters of the batched vortex workload, each drawn from splitmix64(seed ^ (8 i + k)):
+    mesh size N uniform in `sizes`, order p uniform in `orders`, beta ~ U[3,6], centre ~ U[3,7]^2."""
+    out = []
+    for i in range(n_instances):
+        r = splitmix64(np.uint64(seed) ^ np.arange(8 * i, 8 * i + 5, dtype=np.uint64))
+        out.append({"n": int(sizes[int(r[0] % np.uint64(len(sizes)))]),
+                    "p": int(orders[int(r[1] % np.uint64(len(orders)))]),
+                    "beta": 3.0 + 3.0 * float(_unit(r[2])),
+                    "xc": 3.0 + 4.0 * float(_unit(r[3])),
+                    "yc": 3.0 + 4.0 * float(_unit(r[4]))})
+    return out
+
+
 def taylor_green(xy, gamma=1.4, mach=0.1, rho0=1.0, u0=1.0):
     x, y = xy[..., 0], xy[..., 1]
     c0 = u0 / mach

@@ -1,0 +1,269 @@
+// dgfem_amd.hpp — C++20 mirror of the reference's compressible-solver interface on top of the
+// dgx C ABI (dgx.h).  Class and method names follow the reference so a caller of
+//   EulerWeakFormulation / NavierStokesWeakFormulation   (I/weak_forms/*.hpp)
+//   DGAssembler::assemble_euler_residual                  (I/solver/assembler.hpp:58-59)
+//   EulerDGSolver / NavierStokesDGSolver::solve           (I/solver/dg_solver.hpp:223-296)
+//   apply_mass_inv / time_step_ssp_rk3 / compute_max_cfl / compute_density_range
+//                                                         (S/solver/compressible_solver_base.cpp)
+// can switch to the GPU path.  Eigen is not required: a state is std::vector<double> in the
+// reference's DGSolution layout [K][Np][4] (S/core/solution.cpp:143-145), and a frame is the
+// flattened (K, 4 Np) row-major matrix with column i*4+v (flatten_frames, :340-368).
+// Errors are re-thrown as std::runtime_error / std::invalid_argument like the reference.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dgx.h"
+
+namespace dgfem_amd {
+
+using StateVector = std::vector<double>;  // [K][Np][4]
+
+inline void check(dgx_status st) {
+    if (st == DGX_OK) return;
+    std::string msg = dgx_last_error();
+    if (st == DGX_E_INVALID) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+// --------------------------------------------------------------------------------------------
+// Mesh: vertices + triangles + face connectivity (periodic pairing or boundary tags).
+// --------------------------------------------------------------------------------------------
+struct BoundaryConditionEuler {
+    dgx_bc_type type = DGX_BC_FAR_FIELD;
+    std::array<double, 4> value{};  // FAR_FIELD: conserved state; NO_SLIP: primitive (rho,u,v,p)
+};
+
+class DGMesh {
+public:
+    DGMesh(std::vector<double> vertices, std::vector<int32_t> elements)
+        : vertices_(std::move(vertices)), elements_(std::move(elements)) {
+        neighbors_.assign(elements_.size() * 2, -1);
+        check(dgx_mesh_connect(n_vertices(), vertices_.data(), n_elements(), elements_.data(), 0,
+                               neighbors_.data()));
+    }
+    // n x n squares on [x0, x0+L]^2 split into 2 n^2 CCW triangles.
+    static DGMesh split_quads(int n, double L, double x0 = 0.0, double y0 = 0.0) {
+        std::vector<double> v(2 * (n + 1) * (n + 1));
+        std::vector<int32_t> t(6 * n * n);
+        check(dgx_mesh_split_quads(n, L, x0, y0, v.data(), t.data()));
+        return DGMesh(std::move(v), std::move(t));
+    }
+    // DGMesh::set_periodic_boundaries("Left","Right") / ("Bottom","Top")
+    void set_periodic_boundaries(const std::string& a, const std::string& b) {
+        bool lr = (a == "Left" || a == "Right") && (b == "Left" || b == "Right");
+        bool bt = (a == "Bottom" || a == "Top") && (b == "Bottom" || b == "Top");
+        if (!lr && !bt) throw std::invalid_argument("Unknown boundary tag pair: " + a + "/" + b);
+        periodic_ |= lr ? 1 : 2;
+        check(dgx_mesh_connect(n_vertices(), vertices_.data(), n_elements(), elements_.data(), periodic_,
+                               neighbors_.data()));
+    }
+    // DGMesh::set_boundary_condition_euler(tag, bc)
+    void set_boundary_condition_euler(const std::string& tag, const BoundaryConditionEuler& bc) {
+        static const char* names[4] = {"Bottom", "Right", "Top", "Left"};
+        for (int k = 0; k < 4; ++k)
+            if (tag == names[k]) {
+                if (bc.type == DGX_BC_PERIODIC)
+                    throw std::runtime_error("PERIODIC BC: use set_periodic_boundaries");
+                bc_types_[k] = bc.type;
+                for (int v = 0; v < 4; ++v) bc_values_[4 * k + v] = bc.value[v];
+                return;
+            }
+        throw std::invalid_argument("Unknown boundary tag: " + tag);
+    }
+    int n_vertices() const { return int(vertices_.size() / 2); }
+    int n_elements() const { return int(elements_.size() / 3); }
+    const std::vector<double>& vertices() const { return vertices_; }
+
+    dgx_mesh_desc desc() const {
+        dgx_mesh_desc d{};
+        d.n_vertices = n_vertices();
+        d.vertices = vertices_.data();
+        d.n_elements = n_elements();
+        d.elements = elements_.data();
+        d.face_neighbors = neighbors_.data();
+        d.n_bcs = 4;
+        d.bc_types = bc_types_.data();
+        d.bc_values = bc_values_.data();
+        return d;
+    }
+
+private:
+    std::vector<double> vertices_;
+    std::vector<int32_t> elements_;
+    std::vector<int32_t> neighbors_;
+    int periodic_ = 0;
+    std::array<int32_t, 4> bc_types_{-1, -1, -1, -1};
+    std::array<double, 16> bc_values_{};
+};
+
+// --------------------------------------------------------------------------------------------
+// Weak forms: physics parameters and constructor checks.
+// --------------------------------------------------------------------------------------------
+class EulerWeakFormulation {
+public:
+    explicit EulerWeakFormulation(double gamma = 1.4) : gamma_(gamma) {}
+    virtual ~EulerWeakFormulation() = default;
+    double get_gamma() const { return gamma_; }
+    int get_n_vars() const { return 4; }
+    virtual bool has_viscous_terms() const { return false; }
+    virtual dgx_physics physics() const { return dgx_physics{DGX_EULER, gamma_, 0.0, 0.72, 5.0, 0u}; }
+
+protected:
+    double gamma_;
+};
+
+class NavierStokesWeakFormulation : public EulerWeakFormulation {
+public:
+    NavierStokesWeakFormulation(double gamma = 1.4, double dynamic_viscosity = 1.0e-3,
+                                double prandtl = 0.72, double penalty_prefactor = 5.0,
+                                bool reference_face_gradients = true)
+        : EulerWeakFormulation(gamma), mu_(dynamic_viscosity), pr_(prandtl), sigma0_(penalty_prefactor),
+          quirk_(reference_face_gradients) {
+        if (mu_ <= 0.0) throw std::invalid_argument("Dynamic viscosity must be positive for Navier-Stokes weak formulation");
+        if (pr_ <= 0.0) throw std::invalid_argument("Prandtl number must be positive for Navier-Stokes weak formulation");
+        if (sigma0_ <= 0.0) throw std::invalid_argument("Penalty prefactor must be positive for Navier-Stokes weak formulation");
+    }
+    bool has_viscous_terms() const override { return true; }
+    double compute_penalty_parameter(int p, double h) const {
+        return sigma0_ * mu_ * (p + 1) * (p + 1) / std::max(h, 1e-12);
+    }
+    dgx_physics physics() const override {
+        return dgx_physics{DGX_NAVIER_STOKES, gamma_, mu_, pr_, sigma0_,
+                           quirk_ ? uint32_t(DGX_QUIRK_NS_FACE_GRAD_REFERENCE) : 0u};
+    }
+
+private:
+    double mu_, pr_, sigma0_;
+    bool quirk_;
+};
+
+// --------------------------------------------------------------------------------------------
+// Context (owns the device state) and the reference-named operations.
+// --------------------------------------------------------------------------------------------
+class DGContext {
+public:
+    DGContext(const DGMesh& mesh, int order, const EulerWeakFormulation& wf, dgx_precision precision = DGX_FP32,
+              dgx_basis basis = DGX_BASIS_DUBINER, dgx_rule rule = DGX_RULE_2P)
+        : mesh_(mesh) {
+        dgx_mesh_desc m = mesh_.desc();
+        dgx_space_desc s{order, basis, rule};
+        dgx_physics ph = wf.physics();
+        dgx_ctx* c = nullptr;
+        check(dgx_create(&m, &s, &ph, precision, &c));
+        ctx_.reset(c);
+        check(dgx_info(c, &info_));
+    }
+    dgx_ctx* get() const { return ctx_.get(); }
+    const dgx_info_t& info() const { return info_; }
+    const DGMesh& mesh() const { return mesh_; }
+
+private:
+    struct Del { void operator()(dgx_ctx* c) const { dgx_destroy(c); } };
+    DGMesh mesh_;
+    std::unique_ptr<dgx_ctx, Del> ctx_;
+    dgx_info_t info_{};
+};
+
+class DGAssembler {
+public:
+    explicit DGAssembler(DGContext& ctx) : ctx_(ctx) {}
+    // residuals_out = M dU/dt, layout [K][Np][4]
+    void assemble_euler_residual(const StateVector& u_coeffs, StateVector& residuals_out) {
+        size_t n = size_t(ctx_.info().n_elements) * ctx_.info().n_modes * 4;
+        if (u_coeffs.size() != n) throw std::invalid_argument("state has the wrong size");
+        residuals_out.assign(n, 0.0);
+        check(dgx_residual(ctx_.get(), u_coeffs.data(), residuals_out.data()));
+    }
+
+private:
+    DGContext& ctx_;
+};
+
+// CompressibleDGSolverBase / EulerDGSolver / NavierStokesDGSolver.
+class CompressibleDGSolver {
+public:
+    using InitialCondition = std::function<std::array<double, 4>(double x, double y)>;
+
+    explicit CompressibleDGSolver(DGContext& ctx) : ctx_(ctx), assembler_(ctx) {}
+
+    DGAssembler& assembler() { return assembler_; }
+    size_t state_size() const { return size_t(ctx_.info().n_elements) * ctx_.info().n_modes * 4; }
+
+    StateVector project_initial_condition(const InitialCondition& ic) {
+        const auto& I = ctx_.info();
+        std::vector<double> xy(size_t(I.n_elements) * I.n_volume_points * 2);
+        check(dgx_volume_points(ctx_.get(), xy.data()));
+        std::vector<double> Uq(size_t(I.n_elements) * I.n_volume_points * 4);
+        for (size_t k = 0; k < xy.size() / 2; ++k) {
+            auto U = ic(xy[2 * k], xy[2 * k + 1]);
+            for (int v = 0; v < 4; ++v) Uq[4 * k + v] = U[v];
+        }
+        check(dgx_project(ctx_.get(), Uq.data()));
+        return get_state();
+    }
+    StateVector apply_mass_inv(const StateVector& vec) {
+        StateVector r = vec;
+        check(dgx_apply_minv(ctx_.get(), r.data()));
+        return r;
+    }
+    StateVector time_step_ssp_rk3(const StateVector& u_n, double dt) {
+        check(dgx_set_state(ctx_.get(), u_n.data()));
+        check(dgx_rk3_steps(ctx_.get(), dt, 1));
+        return get_state();
+    }
+    double compute_max_cfl(const StateVector& u, double dt) {
+        check(dgx_set_state(ctx_.get(), u.data()));
+        double c = 0;
+        check(dgx_max_cfl(ctx_.get(), dt, &c));
+        return c;
+    }
+    std::pair<double, double> compute_density_range(const StateVector& u) {
+        check(dgx_set_state(ctx_.get(), u.data()));
+        double r[2];
+        check(dgx_density_range(ctx_.get(), r));
+        return {r[0], r[1]};
+    }
+    StateVector get_state() {
+        StateVector u(state_size());
+        check(dgx_get_state(ctx_.get(), u.data()));
+        return u;
+    }
+
+    // solve(ic, T_final, dt, save_every): frames at step 0, every save_every steps and at the
+    // end if not aligned, each flattened to (K, 4 Np) with column i*4+v.  The step count is
+    // ceil(T_final/dt); a non-finite state ends the loop early like the reference's NaN abort.
+    std::vector<std::vector<double>> solve(const InitialCondition& ic, double T_final, double dt, int save_every) {
+        if (save_every <= 0) throw std::invalid_argument("save_every must be positive");
+        std::vector<std::vector<double>> frames;
+        frames.push_back(project_initial_condition(ic));
+        const int n_steps = int(std::ceil(T_final / dt));
+        int step = 0;
+        while (step < n_steps) {
+            int chunk = std::min(save_every - step % save_every, n_steps - step);
+            dgx_status st = dgx_rk3_steps(ctx_.get(), dt, chunk);
+            step += chunk;
+            if (st == DGX_E_NONFINITE) break;
+            check(st);
+            if (step % save_every == 0) frames.push_back(get_state());
+        }
+        if (step % save_every != 0) frames.push_back(get_state());
+        return frames;  // [K][Np][4] is already the (K, 4 Np) row-major frame with column i*4+v
+    }
+
+private:
+    DGContext& ctx_;
+    DGAssembler assembler_;
+};
+
+using EulerDGSolver = CompressibleDGSolver;
+using NavierStokesDGSolver = CompressibleDGSolver;
+
+}  // namespace dgfem_amd

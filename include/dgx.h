@@ -172,6 +172,61 @@ dgx_status dgx_density_range(dgx_ctx* ctx, double out[2]);
 /* Kinetic energy functional J = sum_e sum_q w_q |J_e| (m_x^2+m_y^2)/(2 rho), FP64. */
 dgx_status dgx_kinetic_energy(dgx_ctx* ctx, double* J);
 
+/* ---- discrete adjoint (Euler) ---------------------------------------------------------------
+ * No counterpart in the reference, which has no adjoint; the backward calls of the residual
+ * (assemble_euler_residual), of apply_mass_inv o assemble_euler_residual and of
+ * time_step_ssp_rk3.  The Rusanov max() takes the L-side branch at ties.  DGX_E_UNSUPPORTED for
+ * Navier-Stokes contexts and for meshes with slip-wall faces. */
+
+/* out = (dR/du)^T lam at u (or at the current state when u is NULL), host layout [K][Np][4]. */
+dgx_status dgx_residual_vjp(dgx_ctx* ctx, const double* u, const double* lam, double* out);
+/* out = (dL/du)^T lam with L = M^{-1} R (the RK right-hand side). */
+dgx_status dgx_rhs_vjp(dgx_ctx* ctx, const double* u, const double* lam, double* out);
+/* J of the current state and dJ/du (host layout; grad may be NULL). */
+dgx_status dgx_kinetic_energy_grad(dgx_ctx* ctx, double* J, double* grad);
+/* Gradient of a functional of the state after nsteps SSP-RK3 steps from the current state with
+ * respect to that state: lambda_0 = (du_T/du_0)^T lambda_T.  The forward trajectory is stored
+ * at every ckpt_every-th step and recomputed segment by segment in the reverse sweep.  On
+ * return the context holds u_T.  lambda_T NULL seeds with dJ/du_T of the kinetic energy. */
+dgx_status dgx_rk3_vjp(dgx_ctx* ctx, double dt, int32_t nsteps, int32_t ckpt_every,
+                       const double* lambda_T, double* lambda_0);
+/* Times a plain forward run of nsteps (fwd_ms) and dgx_rk3_vjp of the same run seeded with the
+ * kinetic-energy gradient (adj_ms, recompute included), from the current state, on the context
+ * stream.  The state is restored to the starting state afterwards. */
+dgx_status dgx_rk3_vjp_profile(dgx_ctx* ctx, double dt, int32_t nsteps, int32_t ckpt_every,
+                               double* fwd_ms, double* adj_ms);
+
+/* ---- batched instances ---------------------------------------------------------------------
+ * No counterpart in the reference (one solver per process there).  Many independent Euler
+ * instances of mixed mesh size and order (periodic or with boundary conditions), one state and
+ * dt per instance, stepped together: each RK stage is ONE persistent launch over all instances
+ * (or one per order, as the tuner decides), and a two-step window of launches is captured in a
+ * hipGraph and replayed. */
+typedef struct dgx_batch dgx_batch;
+
+typedef struct {
+    int32_t layout;      /* 0 = one launch per stage for all orders, 1 = one launch per order */
+    int32_t chunk[4];    /* elements per work item = 64 * chunk[p-1] */
+    int32_t blocks_per_cu;
+} dgx_batch_tuning;
+
+dgx_status dgx_batch_create(int32_t n_instances, const dgx_mesh_desc* meshes, const dgx_space_desc* spaces,
+                            const dgx_physics* physics, int32_t precision, dgx_batch** out);
+void dgx_batch_destroy(dgx_batch* batch);
+/* The context of one instance (owned by the batch): its state, projection and diagnostics go
+ * through the single-context calls. */
+dgx_status dgx_batch_instance(dgx_batch* batch, int32_t instance, dgx_ctx** ctx);
+dgx_status dgx_batch_set_dt(dgx_batch* batch, const double* dt);  /* [n_instances] */
+dgx_status dgx_batch_rk3_steps(dgx_batch* batch, int32_t nsteps);
+/* Mean wall time per step (ms) of nsteps batched steps, or of the same steps run instance after
+ * instance with the single-context launches (sequential != 0); the states are restored. */
+dgx_status dgx_batch_profile(dgx_batch* batch, int32_t nsteps, int32_t sequential, double* ms_per_step);
+dgx_status dgx_batch_get_tuning(const dgx_batch* batch, dgx_batch_tuning* t);
+dgx_status dgx_batch_set_tuning(dgx_batch* batch, const dgx_batch_tuning* t);
+/* Autotuner: times trial_steps for each candidate (layout, per-order chunk, blocks per CU) by
+ * coordinate descent, keeps the fastest and restores the states. */
+dgx_status dgx_batch_autotune(dgx_batch* batch, int32_t trial_steps, dgx_batch_tuning* chosen);
+
 #ifdef __cplusplus
 }
 #endif

