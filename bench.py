@@ -75,12 +75,17 @@ def cpu_baseline(verts, tris, p, u0, dt, steps):
     return 12.0 * ora.Np * ora.K * steps / secs, secs
 
 
-def load_traffic(workload):
+def load_traffic(workload, kernel=None):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None when the
+    summary is missing or was measured on another kernel."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        t = json.load(f)
+    if kernel is not None and t.get("kernel") != kernel:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def main():
@@ -127,6 +132,7 @@ def main():
     stage_ms, _ = ctx32.rk3_profile(dt, min(args.steps, 50))
     bytes_launch = ctx32.info.bytes_per_stage_alg
     achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
+    kname = ("stage_mfma_kernel_p%d" % p) if ctx32.info.stage_kernel == 1 else "stage_kernel<%d,float,Euler>" % p
 
     result = {
         "metric": "Euler RHS DOF-stage updates/s (FP32, L2 err vs FP64 stated)",
@@ -145,8 +151,8 @@ def main():
                    "modes": Np, "unknowns": K * Np * 4, "basis": "dubiner", "volume_rule": "degree 2p",
                    "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload),
-                     "kernel": "stage_kernel<P=3,float,Euler>", "launch_ms": stage_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload, kname),
+                     "kernel": kname, "launch_ms": stage_ms,
                      "alg_bytes_per_launch": bytes_launch},
     }
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -14,8 +15,8 @@
 #include "../../include/dgx.h"
 #include "dgx_internal.hpp"
 #include "dgx_space.hpp"
-
 #include "dgx_ctx.hpp"
+#include "dgx_mfma.hpp"
 
 namespace dgx_detail {
 thread_local std::string g_error;
@@ -97,6 +98,9 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         B.values(vr.xi[q], vr.eta[q], &phiV[q * np]);
         B.gradients(vr.xi[q], vr.eta[q], &gV[q * np * 2]);
     }
+    c.phi0 = phiV[0];
+    int ndummy = 0;
+    if (std::frexp(c.phi0, &ndummy) != 0.5) throw std::runtime_error("constant basis function is not a power of two");
     std::vector<double> Mref(np * np, 0.0);
     for (int q = 0; q < nq; ++q)
         for (int i = 0; i < np; ++i)
@@ -315,6 +319,17 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         c.upload_real<float>(c.tabL, tL);
         c.upload_real<float>(c.tabR, tR);
     }
+    // matrix-core stage kernel: FP32 Euler, degree-2p volume rule, p = 2..4
+    // (DGX_STAGE_KERNEL=lane keeps the one-element-per-lane kernel)
+    const char* kv = std::getenv("DGX_STAGE_KERNEL");
+    const bool lane_only = kv && std::string(kv) == "lane";
+    if (precision == DGX_FP32 && c.physics == DGX_EULER && c.rule == 0 && mfma_order(c.p) && !lane_only) {
+        std::vector<float> m;
+        build_mfma_tables(c.p, tL, m);
+        c.upload(c.tabML, m.data(), m.size() * sizeof(float));
+        build_mfma_tables(c.p, tR, m);
+        c.upload(c.tabMR, m.data(), m.size() * sizeof(float));
+    }
     c.u.alloc(c.state_bytes());
     c.u1.alloc(c.state_bytes());
     c.u2.alloc(c.state_bytes());
@@ -451,6 +466,7 @@ dgx_status dgx_info(const dgx_ctx* c, dgx_info_t* o) {
     // per stage: read u_k, (stages 2,3) u^n, write u_{k+1}: 8 state touches per 3 stages,
     // plus 128 B of FP64 geometry + connectivity per element
     o->bytes_per_stage_alg = (double)c->K * (8.0 / 3.0 * 4.0 * c->np * (double)c->rsz + 128.0);
+    o->stage_kernel = c->tabML.p ? 1 : 0;
     return DGX_OK;
 }
 
@@ -458,8 +474,14 @@ dgx_status dgx_set_state(dgx_ctx* c, const double* u) {
     return guarded([&] {
         if (!c || !u) throw std::invalid_argument("null argument");
         size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
+        // storage offset: mean of the element averages' constant-mode values
+        double U[4] = {0, 0, 0, 0};
+        for (int e = 0; e < c->K; ++e)
+            for (int v = 0; v < 4; ++v) U[v] += u[(size_t)e * c->np * 4 + v];
+        for (int v = 0; v < 4; ++v) U[v] *= c->phi0 / c->K;
+        c->set_offset(U);
         hip_check(hipMemcpyAsync(c->stage.p, u, bytes, hipMemcpyHostToDevice, c->stream), "copy");
-        hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u.p, c->K, c->Kp, c->stream), "pack");
+        hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u.p, c->K, c->Kp, c->off, c->stream), "pack");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
 }
@@ -468,7 +490,7 @@ dgx_status dgx_get_state(dgx_ctx* c, double* u) {
     return guarded([&] {
         if (!c || !u) throw std::invalid_argument("null argument");
         size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
-        hip_check(launch_unpack(c->precision, c->np, c->u.p, c->stage.as<double>(), c->K, c->Kp, c->stream), "unpack");
+        hip_check(launch_unpack(c->precision, c->np, c->u.p, c->stage.as<double>(), c->K, c->Kp, c->off, c->stream), "unpack");
         hip_check(hipMemcpyAsync(u, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
@@ -494,9 +516,15 @@ dgx_status dgx_project(dgx_ctx* c, const double* Uq) {
     return guarded([&] {
         if (!c || !Uq) throw std::invalid_argument("null argument");
         size_t bytes = (size_t)c->K * c->nq * 4 * sizeof(double);
+        double U[4] = {0, 0, 0, 0};
+        const size_t npts = (size_t)c->K * c->nq;
+        for (size_t k = 0; k < npts; ++k)
+            for (int v = 0; v < 4; ++v) U[v] += Uq[k * 4 + v];
+        for (int v = 0; v < 4; ++v) U[v] /= double(npts);
+        c->set_offset(U);
         hip_check(hipMemcpyAsync(c->stage.p, Uq, bytes, hipMemcpyHostToDevice, c->stream), "copy");
         hip_check(launch_project(c->precision, c->p, c->rule, c->stage.as<double>(), c->u.p, nullptr, c->K, c->Kp,
-                                 c->pm.as<double>(), c->stream),
+                                 c->pm.as<double>(), c->off, c->stream),
                   "project");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
@@ -508,7 +536,7 @@ dgx_status dgx_eval_volume(dgx_ctx* c, double* Uq) {
         TabLayout L(c->p, c->rule);
         size_t bytes = (size_t)c->K * c->nq * 4 * sizeof(double);
         hip_check(launch_eval(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>() + L.phiV, c->K, c->Kp,
-                              c->stage.as<double>(), c->stream),
+                              c->stage.as<double>(), c->uref, c->stream),
                   "eval");
         hip_check(hipMemcpyAsync(Uq, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
@@ -522,11 +550,11 @@ dgx_status dgx_residual(dgx_ctx* c, const double* u, double* r) {
         const void* src = c->u.p;
         if (u) {
             hip_check(hipMemcpyAsync(c->stage.p, u, bytes, hipMemcpyHostToDevice, c->stream), "copy");
-            hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u2.p, c->K, c->Kp, c->stream), "pack");
+            hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u2.p, c->K, c->Kp, c->off, c->stream), "pack");
             src = c->u2.p;
         }
         c->stage_launch(src, src, c->u1.p, MODE_RESIDUAL, 0.0, 0.0, 0.0);
-        hip_check(launch_unpack(c->precision, c->np, c->u1.p, c->stage.as<double>(), c->K, c->Kp, c->stream), "unpack");
+        hip_check(launch_unpack(c->precision, c->np, c->u1.p, c->stage.as<double>(), c->K, c->Kp, dgx_ctx::kNoOff, c->stream), "unpack");
         hip_check(hipMemcpyAsync(r, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
@@ -574,33 +602,24 @@ dgx_status dgx_rk3_steps(dgx_ctx* c, double dt, int32_t nsteps) {
 dgx_status dgx_rk3_profile(dgx_ctx* c, double dt, int32_t nsteps, double* stage_ms, double* total_ms) {
     return guarded([&] {
         if (!c || nsteps <= 0) throw std::invalid_argument("bad arguments");
-        // per-launch events around every stage launch on the context stream
-        std::vector<hipEvent_t> ev((size_t)nsteps * 6);
-        for (auto& x : ev) hip_check(hipEventCreate(&x), "hipEventCreate");
-        hip_check(hipEventRecord(c->ev0, c->stream), "record");
-        for (int s = 0; s < nsteps; ++s) {
+        // one warm-up step, then events around nsteps back-to-back steps on the context stream:
+        // the mean stage time is the total over the 3 nsteps launches (a per-launch event pair
+        // would add its own record overhead to every launch)
+        auto step = [&] {
             const double ca[3] = {0.0, 0.75, 1.0 / 3.0}, cb[3] = {1.0, 0.25, 2.0 / 3.0};
             void* in[3] = {c->u.p, c->u1.p, c->u2.p};
             void* outb[3] = {c->u1.p, c->u2.p, c->u1.p};
-            for (int k = 0; k < 3; ++k) {
-                hip_check(hipEventRecord(ev[(size_t)s * 6 + 2 * k], c->stream), "record");
-                c->stage_launch(in[k], c->u.p, outb[k], MODE_RK, dt, ca[k], cb[k]);
-                hip_check(hipEventRecord(ev[(size_t)s * 6 + 2 * k + 1], c->stream), "record");
-            }
+            for (int k = 0; k < 3; ++k) c->stage_launch(in[k], c->u.p, outb[k], MODE_RK, dt, ca[k], cb[k]);
             std::swap(c->u, c->u1);
-        }
+        };
+        step();
+        hip_check(hipEventRecord(c->ev0, c->stream), "record");
+        for (int s = 0; s < nsteps; ++s) step();
         hip_check(hipEventRecord(c->ev1, c->stream), "record");
         hip_check(hipStreamSynchronize(c->stream), "sync");
-        double sum = 0.0;
-        for (int k = 0; k < nsteps * 3; ++k) {
-            float ms = 0.f;
-            hip_check(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]), "elapsed");
-            sum += ms;
-        }
         float tot = 0.f;
         hip_check(hipEventElapsedTime(&tot, c->ev0, c->ev1), "elapsed");
-        for (auto& x : ev) (void)hipEventDestroy(x);
-        if (stage_ms) *stage_ms = sum / (3.0 * nsteps);
+        if (stage_ms) *stage_ms = tot / (3.0 * nsteps);
         if (total_ms) *total_ms = tot;
     });
 }
@@ -638,15 +657,17 @@ void adjoint_supported(const dgx_ctx* c) {
     if (c->has_slip) throw Fail(DGX_E_UNSUPPORTED, "the adjoint does not differentiate slip-wall ghost states");
 }
 
-// host [K][Np][4] -> device SoA in `dst`
-void to_device(dgx_ctx* c, const double* h, void* dst) {
+// host [K][Np][4] -> device SoA in `dst`; states carry the storage offset, adjoints do not
+void to_device(dgx_ctx* c, const double* h, void* dst, bool state) {
     size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
     hip_check(hipMemcpyAsync(c->stage.p, h, bytes, hipMemcpyHostToDevice, c->stream), "copy");
-    hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), dst, c->K, c->Kp, c->stream), "pack");
+    hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), dst, c->K, c->Kp,
+                          state ? c->off : dgx_ctx::kNoOff, c->stream),
+              "pack");
 }
 void to_host(dgx_ctx* c, const void* src, double* h) {
     size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
-    hip_check(launch_unpack(c->precision, c->np, src, c->stage.as<double>(), c->K, c->Kp, c->stream), "unpack");
+    hip_check(launch_unpack(c->precision, c->np, src, c->stage.as<double>(), c->K, c->Kp, dgx_ctx::kNoOff, c->stream), "unpack");
     hip_check(hipMemcpyAsync(h, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
 }
@@ -655,7 +676,7 @@ void vjp(dgx_ctx* c, const void* uk, const void* lam, const void* add, const voi
          int mode, double dt, double ca, double cb) {
     hip_check(launch_vjp(c->precision, c->p, c->rule, uk, lam, add, acc_in, acc_out, out, c->geo.as<double>(),
                          c->nbr.as<int32_t>(), c->code.as<uint32_t>(), mode == MODE_RESIDUAL ? c->tabR.p : c->tabL.p,
-                         c->bc.as<double>(), c->K, c->Kp, dt, ca, cb, c->gamma, mode, c->stream),
+                         c->bc.as<double>(), c->K, c->Kp, dt, ca, cb, c->gamma, mode, c->uref, c->stream),
               "vjp kernel launch");
 }
 
@@ -709,8 +730,8 @@ dgx_status dgx_residual_vjp(dgx_ctx* c, const double* u, const double* lam, doub
         adjoint_supported(c);
         if (!lam || !out) throw std::invalid_argument("null argument");
         const void* src = c->u.p;
-        if (u) { to_device(c, u, c->u2.p); src = c->u2.p; }
-        to_device(c, lam, c->u1.p);
+        if (u) { to_device(c, u, c->u2.p, true); src = c->u2.p; }
+        to_device(c, lam, c->u1.p, false);
         DevBuf o;
         o.alloc(c->state_bytes());
         vjp(c, src, c->u1.p, nullptr, nullptr, nullptr, o.p, MODE_RESIDUAL, 0, 0, 0);
@@ -723,8 +744,8 @@ dgx_status dgx_rhs_vjp(dgx_ctx* c, const double* u, const double* lam, double* o
         adjoint_supported(c);
         if (!lam || !out) throw std::invalid_argument("null argument");
         const void* src = c->u.p;
-        if (u) { to_device(c, u, c->u2.p); src = c->u2.p; }
-        to_device(c, lam, c->u1.p);
+        if (u) { to_device(c, u, c->u2.p, true); src = c->u2.p; }
+        to_device(c, lam, c->u1.p, false);
         DevBuf o;
         o.alloc(c->state_bytes());
         vjp(c, src, c->u1.p, nullptr, nullptr, nullptr, o.p, MODE_RHS, 0, 0, 0);
@@ -744,7 +765,7 @@ dgx_status dgx_kinetic_energy_grad(dgx_ctx* c, double* J, double* grad) {
         if (grad) {
             TabLayout L(c->p, c->rule);
             hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
-                                     c->geo.as<double>(), c->u1.p, c->K, c->Kp, c->stream),
+                                     c->geo.as<double>(), c->u1.p, c->K, c->Kp, c->uref, c->stream),
                       "ke grad");
             to_host(c, c->u1.p, grad);
         }
@@ -759,7 +780,7 @@ dgx_status dgx_rk3_vjp(dgx_ctx* c, double dt, int32_t nsteps, int32_t every, con
         for (DevBuf* x : {&b.l0, &b.l1, &b.l2, &b.acc}) x->alloc(c->state_bytes());
         // seed: lambda_T given, or the kinetic-energy gradient at u_T (computed after the forward)
         if (lambda_T) {
-            to_device(c, lambda_T, b.l0.p);
+            to_device(c, lambda_T, b.l0.p, false);
             rk3_adjoint(c, b, dt, nsteps, every);
         } else {
             // forward once to reach u_T, seed, then restart from the saved initial state
@@ -768,7 +789,7 @@ dgx_status dgx_rk3_vjp(dgx_ctx* c, double dt, int32_t nsteps, int32_t every, con
             hip_check(hipMemcpyAsync(u0.p, c->u.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
             for (int s = 0; s < nsteps; ++s) c->rk3_step(dt);
             hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
-                                     c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->stream),
+                                     c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->uref, c->stream),
                       "ke grad");
             hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
             rk3_adjoint(c, b, dt, nsteps, every);
@@ -788,7 +809,7 @@ dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t ev
         hip_check(hipMemcpyAsync(u0.p, c->u.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
         // warm-up allocations of the checkpoint and stage stores with a one-segment sweep
         hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
-                                 c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->stream), "ke grad");
+                                 c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->uref, c->stream), "ke grad");
         rk3_adjoint(c, b, dt, std::min(nsteps, every), every);
         hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
         // forward only
@@ -800,7 +821,7 @@ dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t ev
         hip_check(hipEventElapsedTime(&f, c->ev0, c->ev1), "elapsed");
         // adjoint (forward with checkpoints + recompute + reverse), seeded at u_T
         hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
-                                 c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->stream), "ke grad");
+                                 c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->uref, c->stream), "ke grad");
         hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
         hip_check(hipEventRecord(c->ev0, c->stream), "record");
         rk3_adjoint(c, b, dt, nsteps, every);

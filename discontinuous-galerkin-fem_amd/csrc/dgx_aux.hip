@@ -8,17 +8,23 @@ namespace dgx {
 namespace {
 
 template <typename Real>
-__global__ void pack_kernel(const double* __restrict__ h, Real* __restrict__ u, int np, int K, int Kp) {
+__global__ void pack_kernel(const double* __restrict__ h, Real* __restrict__ u, int np, int K, int Kp, Off4 off) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= K) return;
-    for (int k = 0; k < np * 4; ++k) u[k * Kp + e] = Real(h[(size_t)e * np * 4 + k]);
+    for (int k = 0; k < np * 4; ++k) {
+        double x = h[(size_t)e * np * 4 + k];
+        u[sidx(k >> 2, k & 3, e, Kp)] = Real(k < 4 ? x - off.v[k] : x);
+    }
 }
 
 template <typename Real>
-__global__ void unpack_kernel(const Real* __restrict__ u, double* __restrict__ h, int np, int K, int Kp) {
+__global__ void unpack_kernel(const Real* __restrict__ u, double* __restrict__ h, int np, int K, int Kp, Off4 off) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= K) return;
-    for (int k = 0; k < np * 4; ++k) h[(size_t)e * np * 4 + k] = double(u[k * Kp + e]);
+    for (int k = 0; k < np * 4; ++k) {
+        double x = double(u[sidx(k >> 2, k & 3, e, Kp)]);
+        h[(size_t)e * np * 4 + k] = k < 4 ? x + off.v[k] : x;
+    }
 }
 
 // u_i = sum_q Pm[q][i] U(x_q), Pm[q][i] = sum_j Minv_ref[i][j] w_q phi_j(x_q): the L2 projection
@@ -26,26 +32,26 @@ __global__ void unpack_kernel(const Real* __restrict__ u, double* __restrict__ h
 // S/solver/compressible_solver_base.cpp:63-104).  FP64 arithmetic.
 template <typename Real>
 __global__ void project_kernel(const double* __restrict__ Uq, Real* __restrict__ u,
-                               const double* __restrict__ Pm, int np, int nq, int K, int Kp) {
+                               const double* __restrict__ Pm, int np, int nq, int K, int Kp, Off4 off) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= K) return;
     for (int i = 0; i < np; ++i)
         for (int v = 0; v < 4; ++v) {
             double s = 0.0;
             for (int q = 0; q < nq; ++q) s += Pm[q * np + i] * Uq[((size_t)e * nq + q) * 4 + v];
-            u[(i * 4 + v) * Kp + e] = Real(s);
+            u[sidx(i, v, e, Kp)] = Real(i == 0 ? s - off.v[v] : s);
         }
 }
 
 template <typename Real>
 __global__ void eval_kernel(const Real* __restrict__ u, const double* __restrict__ phiV,
-                            double* __restrict__ Uq, int np, int nq, int K, int Kp) {
+                            double* __restrict__ Uq, int np, int nq, int K, int Kp, Off4 uref) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= K) return;
     for (int q = 0; q < nq; ++q)
         for (int v = 0; v < 4; ++v) {
-            double s = 0.0;
-            for (int i = 0; i < np; ++i) s += phiV[q * np + i] * double(u[(i * 4 + v) * Kp + e]);
+            double s = uref.v[v];
+            for (int i = 0; i < np; ++i) s += phiV[q * np + i] * double(u[sidx(i, v, e, Kp)]);
             Uq[((size_t)e * nq + q) * 4 + v] = s;
         }
 }
@@ -81,7 +87,8 @@ __global__ __launch_bounds__(kRed) void reduce_kernel(int kind, const Real* __re
                                                       const double* __restrict__ hmin,
                                                       const double* __restrict__ qw, int np, int nq,
                                                       int phiV_off, int mass_off, int K, int Kp,
-                                                      double gamma, double* __restrict__ partial) {
+                                                      double gamma, Off4 uref, Off4 off,
+                                                      double* __restrict__ partial) {
     __shared__ double sh[kMaxOut][kRed];
     int e = blockIdx.x * kRed + threadIdx.x;
     double val[kMaxOut] = {0.0, 0.0, 0.0, 0.0};
@@ -89,23 +96,24 @@ __global__ __launch_bounds__(kRed) void reduce_kernel(int kind, const Real* __re
     if (e < K) {
         double det = geo[G_DET * Kp + e];
         if (kind == 0) {
+            for (int v = 0; v < 4; ++v) val[v] = tab64[mass_off] * off.v[v];
             for (int i = 0; i < np; ++i) {
                 double m = tab64[mass_off + i];
-                for (int v = 0; v < 4; ++v) val[v] += m * double(u[(i * 4 + v) * Kp + e]);
+                for (int v = 0; v < 4; ++v) val[v] += m * double(u[sidx(i, v, e, Kp)]);
             }
             for (int v = 0; v < 4; ++v) val[v] *= det;
         } else if (kind == 3) {
             for (int k = 0; k < np * 4; ++k) {
-                double x = double(u[k * Kp + e]);
+                double x = double(u[sidx(k >> 2, k & 3, e, Kp)]);
                 if (!isfinite(x)) val[0] += 1.0;
             }
         } else {
             double ws = 0.0;
             for (int q = 0; q < nq; ++q) {
-                double U[4] = {0, 0, 0, 0};
+                double U[4] = {uref.v[0], uref.v[1], uref.v[2], uref.v[3]};
                 for (int i = 0; i < np; ++i) {
                     double ph = tab64[phiV_off + q * np + i];
-                    for (int v = 0; v < 4; ++v) U[v] += ph * double(u[(i * 4 + v) * Kp + e]);
+                    for (int v = 0; v < 4; ++v) U[v] += ph * double(u[sidx(i, v, e, Kp)]);
                 }
                 if (kind == 1) {
                     val[0] += qw[q] * det * 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
@@ -147,40 +155,40 @@ inline dim3 grid_for(int K, int b) { return dim3((K + b - 1) / b); }
 
 }  // namespace
 
-hipError_t launch_pack(int precision, int np, const double* h, void* u, int K, int Kp, hipStream_t s) {
+hipError_t launch_pack(int precision, int np, const double* h, void* u, int K, int Kp, Off4 off, hipStream_t s) {
     if (precision == DGX_FP64)
-        hipLaunchKernelGGL(pack_kernel<double>, grid_for(K, 256), dim3(256), 0, s, h, (double*)u, np, K, Kp);
+        hipLaunchKernelGGL(pack_kernel<double>, grid_for(K, 256), dim3(256), 0, s, h, (double*)u, np, K, Kp, off);
     else
-        hipLaunchKernelGGL(pack_kernel<float>, grid_for(K, 256), dim3(256), 0, s, h, (float*)u, np, K, Kp);
+        hipLaunchKernelGGL(pack_kernel<float>, grid_for(K, 256), dim3(256), 0, s, h, (float*)u, np, K, Kp, off);
     return hipGetLastError();
 }
 
-hipError_t launch_unpack(int precision, int np, const void* u, double* h, int K, int Kp, hipStream_t s) {
+hipError_t launch_unpack(int precision, int np, const void* u, double* h, int K, int Kp, Off4 off, hipStream_t s) {
     if (precision == DGX_FP64)
-        hipLaunchKernelGGL(unpack_kernel<double>, grid_for(K, 256), dim3(256), 0, s, (const double*)u, h, np, K, Kp);
+        hipLaunchKernelGGL(unpack_kernel<double>, grid_for(K, 256), dim3(256), 0, s, (const double*)u, h, np, K, Kp, off);
     else
-        hipLaunchKernelGGL(unpack_kernel<float>, grid_for(K, 256), dim3(256), 0, s, (const float*)u, h, np, K, Kp);
+        hipLaunchKernelGGL(unpack_kernel<float>, grid_for(K, 256), dim3(256), 0, s, (const float*)u, h, np, K, Kp, off);
     return hipGetLastError();
 }
 
 hipError_t launch_project(int precision, int p, int rule, const double* Uq, void* u, const void*,
-                          int K, int Kp, const double* Pm, hipStream_t s) {
+                          int K, int Kp, const double* Pm, Off4 off, hipStream_t s) {
     int np = np_of(p), nq = nq_of(p, rule);
     if (precision == DGX_FP64)
-        hipLaunchKernelGGL(project_kernel<double>, grid_for(K, 256), dim3(256), 0, s, Uq, (double*)u, Pm, np, nq, K, Kp);
+        hipLaunchKernelGGL(project_kernel<double>, grid_for(K, 256), dim3(256), 0, s, Uq, (double*)u, Pm, np, nq, K, Kp, off);
     else
-        hipLaunchKernelGGL(project_kernel<float>, grid_for(K, 256), dim3(256), 0, s, Uq, (float*)u, Pm, np, nq, K, Kp);
+        hipLaunchKernelGGL(project_kernel<float>, grid_for(K, 256), dim3(256), 0, s, Uq, (float*)u, Pm, np, nq, K, Kp, off);
     return hipGetLastError();
 }
 
 hipError_t launch_eval(int precision, int p, int rule, const void* u, const void* phiV64, int K,
-                       int Kp, double* Uq, hipStream_t s) {
+                       int Kp, double* Uq, Off4 uref, hipStream_t s) {
     int np = np_of(p), nq = nq_of(p, rule);
     const double* ph = static_cast<const double*>(phiV64);
     if (precision == DGX_FP64)
-        hipLaunchKernelGGL(eval_kernel<double>, grid_for(K, 256), dim3(256), 0, s, (const double*)u, ph, Uq, np, nq, K, Kp);
+        hipLaunchKernelGGL(eval_kernel<double>, grid_for(K, 256), dim3(256), 0, s, (const double*)u, ph, Uq, np, nq, K, Kp, uref);
     else
-        hipLaunchKernelGGL(eval_kernel<float>, grid_for(K, 256), dim3(256), 0, s, (const float*)u, ph, Uq, np, nq, K, Kp);
+        hipLaunchKernelGGL(eval_kernel<float>, grid_for(K, 256), dim3(256), 0, s, (const float*)u, ph, Uq, np, nq, K, Kp, uref);
     return hipGetLastError();
 }
 
@@ -192,17 +200,17 @@ hipError_t launch_apply_minv(int, int np, const double* minv, const double* geo,
 
 hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u, const void* tab64,
                          const double* geo, const double* hmin, const double* qw, int K, int Kp,
-                         double gamma, double* partial, int nblocks, hipStream_t s) {
+                         double gamma, Off4 uref, Off4 off, double* partial, int nblocks, hipStream_t s) {
     int np = np_of(p), nq = nq_of(p, rule);
     TabLayout L(p, rule);
     const double* t = static_cast<const double*>(tab64);
     if (nblocks != (K + kRed - 1) / kRed) return hipErrorInvalidValue;
     if (precision == DGX_FP64)
         hipLaunchKernelGGL(reduce_kernel<double>, dim3(nblocks), dim3(kRed), 0, s, kind, (const double*)u, t, geo, hmin, qw,
-                           np, nq, L.phiV, L.mass, K, Kp, gamma, partial);
+                           np, nq, L.phiV, L.mass, K, Kp, gamma, uref, off, partial);
     else
         hipLaunchKernelGGL(reduce_kernel<float>, dim3(nblocks), dim3(kRed), 0, s, kind, (const float*)u, t, geo, hmin, qw,
-                           np, nq, L.phiV, L.mass, K, Kp, gamma, partial);
+                           np, nq, L.phiV, L.mass, K, Kp, gamma, uref, off, partial);
     return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ struct dgx_batch {
         sp.sigma0 = c->sigma0;
         sp.mode = MODE_RK;
         sp.ns_face_quirk = 0;
+        for (int v = 0; v < 4; ++v) sp.uref[v] = c->uref.v[v];
         return sp;
     }
 

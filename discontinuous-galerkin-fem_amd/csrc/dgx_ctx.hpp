@@ -1,6 +1,7 @@
 // Private: the context object behind the dgx C ABI, device buffers and error plumbing shared by
 // the host translation units (dgx_api.cpp, dgx_batch_api.cpp).
 #pragma once
+#include <cmath>
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
@@ -91,18 +92,33 @@ struct dgx_ctx {
     int n_bfaces = 0;
     bool has_slip = false;  // a boundary face uses a slip-wall condition
     double dt = 0.0;        // time step of this instance inside a batch
+    double phi0 = 1.0;      // value of the constant basis function
+    Off4 uref{}, off{};     // FP32 storage offset: point values uref, constant-mode coefficient off
+    static constexpr Off4 kNoOff{};
     size_t rsz = 8;
     hipStream_t stream = nullptr;
     DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
+    DevBuf tabML, tabMR;  // MFMA stage kernel lane tables (RK/RHS and residual variants)
     std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][3][2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~dgx_ctx() {
-        for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage, &partial})
+        for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage,
+                          &partial, &tabML, &tabMR})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Choose the storage offset from a representative state U (FP32 only): rounded to FP32 so
+    // that the kernels add back exactly what pack subtracted (phi0 is a power of two).
+    void set_offset(const double U[4]) {
+        if (precision == DGX_FP64) return;
+        for (int v = 0; v < 4; ++v) {
+            uref.v[v] = std::isfinite(U[v]) ? double(float(U[v])) : 0.0;
+            off.v[v] = uref.v[v] / phi0;
+        }
     }
 
     size_t state_bytes() const { return (size_t)np * 4 * Kp * rsz; }
@@ -127,6 +143,7 @@ struct dgx_ctx {
         sp.nbr = nbr.as<int32_t>();
         sp.code = code.as<uint32_t>();
         sp.tab = (mode == MODE_RESIDUAL) ? tabR.p : tabL.p;
+        sp.tabm = (mode == MODE_RESIDUAL) ? tabMR.p : tabML.p;
         sp.bc = bc.as<double>();
         sp.K = K;
         sp.Kp = Kp;
@@ -139,6 +156,7 @@ struct dgx_ctx {
         sp.sigma0 = sigma0;
         sp.mode = mode;
         sp.ns_face_quirk = (quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) ? 1 : 0;
+        for (int v = 0; v < 4; ++v) sp.uref[v] = uref.v[v];
         hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
     }
 
@@ -154,7 +172,7 @@ struct dgx_ctx {
     std::vector<double> reduce(int kind) {
         int nb = red_blocks();
         hip_check(launch_reduce(kind, precision, p, rule, u.p, tab64.p, geo.as<double>(), hmin.as<double>(),
-                                qw.as<double>(), K, Kp, gamma, partial.as<double>(), nb, stream),
+                                qw.as<double>(), K, Kp, gamma, uref, off, partial.as<double>(), nb, stream),
                   "reduce kernel launch");
         std::vector<double> h((size_t)nb * 4);
         hip_check(hipMemcpyAsync(h.data(), partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "copy");

@@ -19,6 +19,11 @@ constexpr int nq_of(int p, int rule) {
 }
 constexpr int np_of(int p) { return (p + 1) * (p + 2) / 2; }
 
+// Device state layout: coefficient i of variable v of element e at u[(i * Kp + e) * 4 + v] --
+// mode-major rows of Kp elements, the four conserved variables of an element contiguous (one
+// 16-byte access per element and mode in FP32).
+__host__ __device__ constexpr int sidx(int i, int v, int e, int Kp) { return (i * Kp + e) * 4 + v; }
+
 // Geometry rows of the per-element FP64 array geo[kGeoRows][Kp].
 enum GeoRow : int {
     G_A00 = 0, G_A01, G_A10, G_A11,  // dxi_dx = inverse of J^T (row-major)
@@ -60,6 +65,13 @@ struct TabLayout {
 // out = L(uk) (mode RHS) or out = M dU/dt (mode RESIDUAL).
 enum StageMode : int { MODE_RK = 0, MODE_RHS = 1, MODE_RESIDUAL = 2 };
 
+// Storage offset of the FP32 state: the device holds u - off on the constant mode, so that
+// point values are U = sum_i phi_i u_i + uref with uref = phi_0 off.  Rounding of the stored
+// coefficients then scales with the deviation from uref instead of with |U|.  Zero in FP64.
+struct Off4 {
+    double v[4];
+};
+
 struct StageParams {
     const void* uk;
     const void* un;
@@ -68,31 +80,34 @@ struct StageParams {
     const int32_t* nbr;
     const uint32_t* code;
     const void* tab;        // tables in the state precision
+    const void* tabm;       // lane-major tables of the MFMA kernel (FP32 Euler), or null
     const double* bc;       // [n_bcs][5]: type, 4 values
     int K, Kp;
     double dt, ca, cb;
     double gamma, mu, prandtl, sigma0;
     int mode;
     int ns_face_quirk;
+    double uref[4];  // point-value storage offset (see Off4)
 };
 
 // Kernel launchers (dgx_kernels.hip / dgx_kernels_ns.hip).  Each returns hipSuccess or the
 // launch error.
 hipError_t launch_stage(int precision, int physics, int p, int rule, const StageParams& sp,
                         hipStream_t s);
+// pack/project subtract `off` from the constant mode, unpack adds it, eval adds `uref`.
 hipError_t launch_project(int precision, int p, int rule, const double* Uq, void* u,
-                          const void* tabL, int K, int Kp, const double* w_minv, hipStream_t s);
+                          const void* tabL, int K, int Kp, const double* w_minv, Off4 off, hipStream_t s);
 hipError_t launch_eval(int precision, int p, int rule, const void* u, const void* tab, int K,
-                       int Kp, double* Uq, hipStream_t s);
+                       int Kp, double* Uq, Off4 uref, hipStream_t s);
 hipError_t launch_pack(int precision, int np, const double* host_layout_dev, void* u, int K,
-                       int Kp, hipStream_t s);
+                       int Kp, Off4 off, hipStream_t s);
 hipError_t launch_unpack(int precision, int np, const void* u, double* host_layout_dev, int K,
-                         int Kp, hipStream_t s);
+                         int Kp, Off4 off, hipStream_t s);
 // Per-block FP64 partial reductions: 0 conservation sums (4), 1 kinetic energy (1),
 // 2 CFL/density/finite (max wave/h, rho min, rho max, bad count)
 hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u, const void* tab,
                          const double* geo, const double* hmin, const double* qw, int K, int Kp,
-                         double gamma, double* partial, int nblocks, hipStream_t s);
+                         double gamma, Off4 uref, Off4 off, double* partial, int nblocks, hipStream_t s);
 hipError_t launch_apply_minv(int precision, int np, const double* minv_ref, const double* geo,
                              double* r_dev, int K, int Kp, hipStream_t s);
 // Euler stage adjoint (dgx_vjp.hip): out = cb (lam + dt (dL/du)^T lam) + add,
@@ -101,7 +116,7 @@ hipError_t launch_apply_minv(int precision, int np, const double* minv_ref, cons
 hipError_t launch_vjp(int precision, int p, int rule, const void* uk, const void* lam, const void* add,
                       const void* acc_in, void* acc_out, void* out, const double* geo, const int32_t* nbr,
                       const uint32_t* code, const void* tab, const double* bc, int K, int Kp, double dt,
-                      double ca, double cb, double gamma, int mode, hipStream_t s);
+                      double ca, double cb, double gamma, int mode, Off4 uref, hipStream_t s);
 // Batched persistent stage (dgx_batch.hip): items of (instance, first element, count, order);
 // params[instance] holds that instance's stage parameters; tabs[p-1] the order-p tables.
 struct BatchItem {
@@ -113,6 +128,6 @@ hipError_t launch_batch_stage_order(int precision, int p, const StageParams* par
                                     int n_items, const void* tab, int grid, hipStream_t s);
 // gradient of the kinetic-energy functional with respect to the coefficients (device layout)
 hipError_t launch_ke_grad(int precision, int p, int rule, const void* u, const double* tab64, const double* qw,
-                          const double* geo, void* grad, int K, int Kp, hipStream_t s);
+                          const double* geo, void* grad, int K, int Kp, Off4 uref, hipStream_t s);
 
 }  // namespace dgx

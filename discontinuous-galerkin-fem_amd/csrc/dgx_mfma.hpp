@@ -1,0 +1,61 @@
+// Lane layout of the MFMA stage kernel (dgx_stage_mfma.hip) and of its lane-major tables,
+// shared by the kernel and the host-side table builder.
+//
+// One wave processes 16 elements with v_mfma_f32_16x16x4f32 (D = A B + C, A 16x4, B 4x16):
+//   A[i][k] comes from lane i + 16k, B[k][j] from lane j + 16k, and lane j + 16g holds
+//   D[4g + r][j] in its register r.
+// Column j of every product is element j of the tile and lane group g = lane / 16 plays the
+// role of k (operands) and of the row block (results), so a lane (g, j) only ever holds data
+// of element j.  The three index sets are dealt over the (g, r) "slots" of a lane:
+//   modes          slot (g, r) -> mode R g + r        (r < R = ceil(Np / 4))
+//   volume points  slot (g, r) -> point RQ g + r      (r < RQ = ceil(Nq / 4))
+//   face points    slot (g, r), r < 3 -> face r, point g; r = 3 -> face g, point 4 (p = 4)
+// With the contraction index k dealt the same way, the result registers of one product are
+// the operand registers of the next: the coefficients loaded for the volume evaluation are the
+// ones the RK update needs, the fluxes at the volume points are the projection's operands and
+// the face fluxes are the lift's.
+#pragma once
+#include "dgx_internal.hpp"
+
+namespace dgx {
+
+template <int P>
+struct MfmaShape {
+    static constexpr int NP = np_of(P);
+    static constexpr int NQ = nq_of(P, 0);
+    static constexpr int NFQ = P + 1;
+    static constexpr int R = (NP + 3) / 4;
+    static constexpr int RQ = (NQ + 3) / 4;
+    static constexpr int RF = NFQ > 4 ? 4 : 3;
+    static_assert(R <= 4 && RQ <= 4 && NFQ <= 5, "one 16x16 tile per operand");
+    // lane tables, 64 floats each: EV[R] EF[R] PX[RQ] PY[RQ] LF[RF], then the neighbour-trace
+    // table NT[fn*2 + flip][i][q]: mode i of the neighbour at point q of the reading element's
+    // face (the neighbour's local face fn, its points reversed when flip is set)
+    static constexpr int T_EV = 0;
+    static constexpr int T_EF = T_EV + R;
+    static constexpr int T_PX = T_EF + R;
+    static constexpr int T_PY = T_PX + RQ;
+    static constexpr int T_LF = T_PY + RQ;
+    static constexpr int T_N = T_LF + RF;
+    static constexpr int NT = 6 * NP * NFQ;
+    static constexpr int TOTAL = T_N * 64 + NT;
+};
+
+// slot -> mode / volume point / face point (f * nfq + q); -1 for a padding slot
+__host__ __device__ constexpr int mf_mode(int R, int np, int g, int r) {
+    return (r < R && R * g + r < np) ? R * g + r : -1;
+}
+__host__ __device__ constexpr int mf_vpoint(int RQ, int nq, int g, int r) {
+    return (r < RQ && RQ * g + r < nq) ? RQ * g + r : -1;
+}
+__host__ __device__ constexpr int mf_fpoint(int nfq, int g, int r) {
+    return r < 3 ? (g < nfq ? r * nfq + g : -1) : ((nfq == 5 && g < 3) ? g * nfq + 4 : -1);
+}
+
+// Orders with an MFMA stage kernel (FP32 Euler, volume rule exact to degree 2p).
+constexpr bool mfma_order(int p) { return p == 3 || p == 4; }
+int mfma_table_size(int p);  // floats
+// Build the lane-major tables of order p from the (L or R) tables in TabLayout order.
+void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out);
+
+}  // namespace dgx

@@ -1,0 +1,76 @@
+// Host-side builder of the MFMA stage kernel's lane-major tables (layout in dgx_mfma.hpp).
+#include "dgx_mfma.hpp"
+
+#include <stdexcept>
+
+namespace dgx {
+
+namespace {
+
+template <int P>
+void build(const std::vector<double>& t, std::vector<float>& out) {
+    using S = MfmaShape<P>;
+    const TabLayout L(P, 0);
+    const int np = S::NP, nq = S::NQ, nfq = S::NFQ;
+    if ((int)t.size() != L.total) throw std::runtime_error("table size mismatch");
+    out.assign(S::TOTAL, 0.0f);
+    auto at = [&](int table, int lane) -> float& { return out[(size_t)table * 64 + lane]; };
+    for (int lane = 0; lane < 64; ++lane) {
+        const int i = lane % 16, k = lane / 16;
+        // rows of an evaluation product are point slots; padding rows repeat point 0 so that
+        // their (unused) fluxes stay finite
+        int vp = mf_vpoint(S::RQ, nq, i / 4, i % 4);
+        if (vp < 0) vp = 0;
+        int fp = mf_fpoint(nfq, i / 4, i % 4);
+        if (fp < 0) fp = 0;
+        for (int s = 0; s < S::R; ++s) {
+            const int m = mf_mode(S::R, np, k, s);
+            at(S::T_EV + s, lane) = m < 0 ? 0.0f : float(t[L.phiV + vp * np + m]);
+            at(S::T_EF + s, lane) = m < 0 ? 0.0f : float(t[L.phiF + fp * np + m]);
+        }
+        // rows of a projection / lift product are mode slots, k runs over point slots
+        const int mrow = mf_mode(S::R, np, i / 4, i % 4);
+        for (int r = 0; r < S::RQ; ++r) {
+            const int q = mf_vpoint(S::RQ, nq, k, r);
+            const bool ok = mrow >= 0 && q >= 0;
+            at(S::T_PX + r, lane) = ok ? float(t[L.PX + q * np + mrow]) : 0.0f;
+            at(S::T_PY + r, lane) = ok ? float(t[L.PY + q * np + mrow]) : 0.0f;
+        }
+        for (int r = 0; r < S::RF; ++r) {
+            const int f = mf_fpoint(nfq, k, r);
+            at(S::T_LF + r, lane) = (mrow >= 0 && f >= 0) ? float(t[L.LF + f * np + mrow]) : 0.0f;
+        }
+    }
+    // neighbour traces: NT[(fn*2 + flip)][i][q] = phi_{fn, q'}(mode i), q' = nfq - 1 - q for a
+    // reversed face
+    float* nt = &out[(size_t)S::T_N * 64];
+    for (int fn = 0; fn < 3; ++fn)
+        for (int flip = 0; flip < 2; ++flip)
+            for (int m = 0; m < np; ++m)
+                for (int q = 0; q < nfq; ++q) {
+                    const int qq = flip ? nfq - 1 - q : q;
+                    nt[((fn * 2 + flip) * np + m) * nfq + q] = float(t[L.phiF + (fn * nfq + qq) * np + m]);
+                }
+}
+
+}  // namespace
+
+int mfma_table_size(int p) {
+    switch (p) {
+    case 2: return MfmaShape<2>::TOTAL;
+    case 3: return MfmaShape<3>::TOTAL;
+    case 4: return MfmaShape<4>::TOTAL;
+    default: return 0;
+    }
+}
+
+void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out) {
+    switch (p) {
+    case 2: build<2>(tab, out); break;
+    case 3: build<3>(tab, out); break;
+    case 4: build<4>(tab, out); break;
+    default: throw std::runtime_error("no MFMA tables for this order");
+    }
+}
+
+}  // namespace dgx

@@ -10,8 +10,9 @@
 //   assembly    S/solver/assembler.cpp:137-279
 //   M^-1, RK3   S/solver/compressible_solver_base.cpp:54-61,106-112, I/solver/time_stepping.hpp:39-55
 //
-// Layout: state u[(i*4+v)*Kp + e] (mode-major, element fastest) so a wave's 64 elements read
-// 256 contiguous bytes per coefficient; FP64 geometry geo[row*Kp + e]; tables are wave-uniform
+// Layout: state u[(i*Kp + e)*4 + v] (mode-major rows, the 4 variables of an element contiguous),
+// so a wave's 64 elements read 1 KiB contiguous per mode; FP64 geometry geo[row*Kp + e]; tables
+// are wave-uniform
 // (scalar loads) except the neighbour-face tables, staged in LDS because the neighbour's
 // local face differs between lanes.
 //
@@ -193,12 +194,14 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
     const Real mu = Real(sp.mu);
     const Real kappa = Real(sp.mu * sp.gamma / (sp.prandtl * (sp.gamma - 1.0)));
     const bool residual_mode = sp.mode == MODE_RESIDUAL;
+    // point values start from the storage offset (zero in FP64)
+    const Real ur[4] = {Real(sp.uref[0]), Real(sp.uref[1]), Real(sp.uref[2]), Real(sp.uref[3])};
 
     Real u[NP][4];
 #pragma unroll
     for (int i = 0; i < NP; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) u[i][v] = uk[(i * 4 + v) * Kp + e];
+        for (int v = 0; v < 4; ++v) u[i][v] = uk[sidx(i, v, e, Kp)];
 
     double A[4];
 #pragma unroll
@@ -218,7 +221,7 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
     // ---------------- volume: U_q, flux, contravariant projection -------------------------
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        Real U[4] = {0, 0, 0, 0};
+        Real U[4] = {ur[0], ur[1], ur[2], ur[3]};
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
             Real ph = tab[O_phiV + q * NP + i];
@@ -288,7 +291,7 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
 #pragma unroll
         for (int q = 0; q < NFQ; ++q) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Uo[q][v] = Real(0);
+            for (int v = 0; v < 4; ++v) Uo[q][v] = ur[v];
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
                 Real ph = tab[O_phiF + (f * NFQ + q) * NP + i];
@@ -307,12 +310,12 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
 #pragma unroll
                 for (int q = 0; q < NFQ; ++q)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) Un[q][v] = Real(0);
+                    for (int v = 0; v < 4; ++v) Un[q][v] = ur[v];
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
                     Real cf[4];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) cf[v] = uk[(i * 4 + v) * Kp + nb];
+                    for (int v = 0; v < 4; ++v) cf[v] = uk[sidx(i, v, nb, Kp)];
                     const Real* col = &s_phiFT[(fn * NP + i) * NFQ];
                     Real ph[NFQ];
 #pragma unroll
@@ -328,13 +331,13 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
 #pragma unroll
                 for (int i = 0; i < NP; ++i)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) un_c[i][v] = uk[(i * 4 + v) * Kp + nb];
+                    for (int v = 0; v < 4; ++v) un_c[i][v] = uk[sidx(i, v, nb, Kp)];
 #pragma unroll
                 for (int q = 0; q < NFQ; ++q) {
                     const int qq = flip ? (NFQ - 1 - q) : q;
                     const Real* row = &s_phiF[(fn * NFQ + qq) * NP];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) Un[q][v] = Real(0);
+                    for (int v = 0; v < 4; ++v) Un[q][v] = ur[v];
 #pragma unroll
                     for (int i = 0; i < NP; ++i) {
                         Real ph = row[i];
@@ -464,21 +467,21 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
 #pragma unroll
             for (int i = 0; i < NP; ++i)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) out[(i * 4 + v) * Kp + e] = u[i][v] + dt * acc[i][v];
+                for (int v = 0; v < 4; ++v) out[sidx(i, v, e, Kp)] = u[i][v] + dt * acc[i][v];
         } else {
 #pragma unroll
             for (int i = 0; i < NP; ++i)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    Real unv = un[(i * 4 + v) * Kp + e];
-                    out[(i * 4 + v) * Kp + e] = ca * unv + cb * (u[i][v] + dt * acc[i][v]);
+                    Real unv = un[sidx(i, v, e, Kp)];
+                    out[sidx(i, v, e, Kp)] = ca * unv + cb * (u[i][v] + dt * acc[i][v]);
                 }
         }
     } else {
 #pragma unroll
         for (int i = 0; i < NP; ++i)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) out[(i * 4 + v) * Kp + e] = acc[i][v];
+            for (int v = 0; v < 4; ++v) out[sidx(i, v, e, Kp)] = acc[i][v];
     }
 }
 

@@ -1,0 +1,366 @@
+// FP32 Euler stage kernel on the matrix cores (gfx950 v_mfma_f32_16x16x4f32), orders 3 and 4,
+// volume rule exact to degree 2p.  Same operator and output modes as the per-lane stage kernel
+// (dgx_stage_impl.hpp): volume residual + Rusanov faces + inverse mass + SSP-RK3 update.
+//
+// Reference behaviour (paths relative to the reference root):
+//   volume      S/weak_forms/euler_weak_formulation.cpp:72-115
+//   faces       S/weak_forms/euler_weak_formulation.cpp:117-250
+//   assembly    S/solver/assembler.cpp:137-279
+//   M^-1, RK3   S/solver/compressible_solver_base.cpp:54-61,106-112, I/solver/time_stepping.hpp:39-55
+//
+// A wave owns 16 consecutive elements (lane layout in dgx_mfma.hpp).  The four dense
+// contractions of an element run as 16x16x4 products with the elements as columns, one product
+// chain per conserved variable:
+//   U at the volume points     = Phi_V (points x modes)   * u (modes x elements)
+//   own traces at face points  = Phi_F (points x modes)   * u
+//   projection                 = [PX PY]^T (modes x 2 Nq) * [F^; G^] (2 Nq x elements)
+//   lift                       = LF^T (modes x 3 Nfq)     * H (3 Nfq x elements)
+// The pointwise physics (contravariant fluxes, Rusanov) runs on the VALU in the slots the
+// products leave in each lane.  The neighbour's trace on a face is a per-element gather with a
+// per-element table (the neighbour's local face and point order), so it stays on the VALU: each
+// lane group forms the partial sum over its own mode slots of the neighbour's coefficients and
+// the four partials of a face point are summed through LDS.
+#include "dgx_mfma.hpp"
+#include "dgx_stage_impl.hpp"
+
+namespace dgx {
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Ghost state of a boundary face point (boundary_face_residual, euler_weak_formulation.cpp:
+// 208-233); a face without a condition (type < 0) mirrors the interior state.
+__device__ __forceinline__ void ghost_state(const float* Uo, float nx, float ny, const double* bcr, float gam,
+                                            float* Un) {
+    const int type = int(bcr[0]);
+    if (type < 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Un[v] = Uo[v];
+    } else if (type == DGX_BC_FAR_FIELD) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Un[v] = float(bcr[1 + v]);
+    } else if (type == DGX_BC_SLIP_WALL) {
+        const float ri = rcp(Uo[0]);
+        const float w1 = Uo[1] * ri, w2 = Uo[2] * ri;
+        const float W[4] = {Uo[0], w1, w2, (gam - 1.0f) * (Uo[3] - 0.5f * Uo[0] * (w1 * w1 + w2 * w2))};
+        const float vn = W[1] * nx + W[2] * ny;
+        const float Wg[4] = {W[0], W[1] - 2.0f * vn * nx, W[2] - 2.0f * vn * ny, W[3]};
+        prim2cons(Wg, gam, Un);
+    } else {
+        const float Wb[4] = {float(bcr[1]), float(bcr[2]), float(bcr[3]), float(bcr[4])};
+        prim2cons(Wb, gam, Un);
+    }
+}
+
+// Flux of one face point in the canonical (L) orientation, scaled for this element's lift.
+__device__ __forceinline__ void face_point_flux(const float* Uo, float* Un, uint32_t c, int nb, float nx, float ny,
+                                                float sf, const StageParams& sp, float gam, float* Hs) {
+    const bool boundary = (c & 3u) == kFaceBoundary;
+    const bool isL = (c >> 3) & 1u;
+    if (boundary) ghost_state(Uo, nx, ny, sp.bc + 5 * nb, gam, Un);
+    // a boundary face without a boundary condition contributes nothing
+    // (S/solver/assembler.cpp:233-243)
+    const float sfe = (boundary && sp.bc[5 * nb] < 0.0) ? 0.0f : sf;
+    const bool own_is_a = isL || boundary;
+    const float sgn = own_is_a ? 1.0f : -1.0f;
+    float Aq[4], Bq[4], H[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        Aq[v] = own_is_a ? Uo[v] : Un[v];
+        Bq[v] = own_is_a ? Un[v] : Uo[v];
+    }
+    rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Hs[v] = sgn * sfe * H[v];
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    // raw buffer (stride 0), range-checked against `bytes`; dword 3 as for every gfx9 target
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), 0x00020000);
+}
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4u load_x4(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+}
+__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+template <int P>
+__device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
+    using S = MfmaShape<P>;
+    constexpr int NP = S::NP, NFQ = S::NFQ, R = S::R, RQ = S::RQ, RF = S::RF;
+    constexpr int EB = kBlock / 4;  // elements per block: 4 waves x 16
+    __shared__ float s_nt[S::NT];
+    __shared__ float4 s_tr[3 * NFQ * EB];  // neighbour traces [face][point][element]
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int eb0 = xcd_tile(blockIdx.x, gridDim.x) * EB;
+    const int e = eb0 + 16 * w + j;  // element of this lane in the product layout
+    const bool valid = e < sp.K;
+    const int ec = valid ? e : sp.K - 1;  // lanes past the end compute on a valid element
+    // neighbour-trace role: wave f < 3 evaluates the neighbour's trace on face f of block
+    // element `lane` (one element per lane, so its gathers are as coalesced as the mesh allows)
+    const int et = min(eb0 + lane, sp.K - 1);
+    const unsigned uKp = unsigned(sp.Kp);
+    const bool residual_mode = sp.mode == MODE_RESIDUAL;
+    const float gam = float(sp.gamma);
+    const float ur[4] = {float(sp.uref[0]), float(sp.uref[1]), float(sp.uref[2]), float(sp.uref[3])};
+    const float* __restrict__ tm = static_cast<const float*>(sp.tabm);
+
+    // All global traffic goes through buffer resources (the per-lane part of an address is one
+    // 32-bit offset, the variable or geometry row a scalar offset) and every load is issued up
+    // front in dependency order, so a wave waits for memory once.  A padding mode slot gets a
+    // per-lane offset past the end of its resource, which the range check turns into a zero
+    // read or a dropped write.
+    constexpr int kNtPerThread = (S::NT + kBlock - 1) / kBlock;
+    float ntv[kNtPerThread];
+#pragma unroll
+    for (int k = 0; k < kNtPerThread; ++k) {
+        const int t = int(threadIdx.x) + k * kBlock;
+        ntv[k] = t < S::NT ? tm[S::T_N * 64 + t] : 0.0f;
+    }
+    const unsigned state_bytes = unsigned(NP) * 16u * uKp;
+    const auto r_uk = make_rsrc(sp.uk, state_bytes);
+    const auto r_geo = make_rsrc(sp.geo, unsigned(kGeoRows) * 8u * uKp);
+    const auto r_nbr = make_rsrc(sp.nbr, 12u * uKp);
+    const auto r_code = make_rsrc(sp.code, 4u * uKp);
+    const uint32_t code = __builtin_amdgcn_raw_buffer_load_b32(r_code, unsigned(ec) * 4u, 0, 0);
+    int nbf[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+        nbf[f] = int(__builtin_amdgcn_raw_buffer_load_b32(r_nbr, unsigned(ec) * 4u, unsigned(f) * 4u * uKp, 0));
+    const int ft = w < 3 ? w : 0;
+    const uint32_t ct = (__builtin_amdgcn_raw_buffer_load_b32(r_code, unsigned(et) * 4u, 0, 0) >> (4 * ft)) & 15u;
+    const int nbt = int(__builtin_amdgcn_raw_buffer_load_b32(r_nbr, unsigned(et) * 4u, unsigned(ft) * 4u * uKp, 0));
+
+    // byte offset of the four variables of mode slot (g, s) of element `el` (one 16-byte
+    // access); past the end for a padding slot
+    auto slot_off = [&](int s, unsigned el) -> unsigned {
+        const int m = mf_mode(R, NP, g, s);
+        return m >= 0 ? (unsigned(m) * uKp + el) * 16u : state_bytes;
+    };
+    // coefficients of this lane's mode slots (mode R g + s): evaluation operands and RK input
+    float b[4][R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const v4u x = load_x4(r_uk, slot_off(s, unsigned(ec)));
+#pragma unroll
+        for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
+    }
+    // this lane's entries of the product tables (A operands)
+    float tEV[R], tEF[R], tPX[RQ], tPY[RQ], tLF[RF];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        tEV[s] = tm[(S::T_EV + s) * 64 + lane];
+        tEF[s] = tm[(S::T_EF + s) * 64 + lane];
+    }
+#pragma unroll
+    for (int t = 0; t < RQ; ++t) {
+        tPX[t] = tm[(S::T_PX + t) * 64 + lane];
+        tPY[t] = tm[(S::T_PY + t) * 64 + lane];
+    }
+#pragma unroll
+    for (int t = 0; t < RF; ++t) tLF[t] = tm[(S::T_LF + t) * 64 + lane];
+    const unsigned go = unsigned(ec) * 8u;
+    double gd[kGeoRows];  // raw FP64 geometry of the element
+#pragma unroll
+    for (int k = 0; k < kGeoRows; ++k) gd[k] = load_f64(r_geo, go, unsigned(k) * 8u * uKp);
+    // u^n for the second and third RK stages
+    const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
+    float unv[4][R];
+    if (rk_blend) {
+        const auto r_un = make_rsrc(sp.un, state_bytes);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const v4u x = load_x4(r_un, slot_off(r, unsigned(ec)));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) unv[v][r] = __uint_as_float(x[v]);
+        }
+    }
+    // neighbour coefficients for the trace role (all modes of one neighbour per lane)
+    const bool t_interior = w < 3 && (ct & 3u) != kFaceBoundary;
+    float cn[NP][4];
+    if (t_interior) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const v4u x = load_x4(r_uk, (unsigned(i) * uKp + unsigned(nbt)) * 16u);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(x[v]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kNtPerThread; ++k) {
+        const int t = int(threadIdx.x) + k * kBlock;
+        if (t < S::NT) s_nt[t] = ntv[k];
+    }
+    __syncthreads();
+    // keep the scheduler from sinking the loads above into the arithmetic below
+    __builtin_amdgcn_sched_barrier(0);
+
+    // ---------------- neighbour traces (waves 0-2) ------------------------------------------
+    if (t_interior) {
+        const int fn = int(ct & 3u), flip = int((ct >> 2) & 1u);
+        const float* T = &s_nt[(fn * 2 + flip) * NP * NFQ];
+        float Un[NFQ][4];
+#pragma unroll
+        for (int q = 0; q < NFQ; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Un[q][v] = ur[v];
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int q = 0; q < NFQ; ++q) {
+                const float t = T[i * NFQ + q];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Un[q][v] = fma(t, cn[i][v], Un[q][v]);
+            }
+#pragma unroll
+        for (int q = 0; q < NFQ; ++q)
+            s_tr[(w * NFQ + q) * EB + lane] = make_float4(Un[q][0], Un[q][1], Un[q][2], Un[q][3]);
+    }
+
+    const double det = gd[G_DET];
+    const double rdet = 1.0 / det;
+    // volume terms carry |J| only for M dU/dt (with M^-1 = M_ref^-1/|J| folded they cancel);
+    // the contravariant fluxes are linear in dxi_dx, so the factor goes into its entries
+    const double sv = residual_mode ? det : 1.0;
+    const float a00 = float(sv * gd[G_A00]), a01 = float(sv * gd[G_A01]);
+    const float a10 = float(sv * gd[G_A10]), a11 = float(sv * gd[G_A11]);
+    float fnx[3], fny[3], fsf[3];  // outward normal and lift scale of each face
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        fnx[f] = float(gd[G_N0X + 2 * f]);
+        fny[f] = float(gd[G_N0X + 2 * f + 1]);
+        fsf[f] = residual_mode ? float(gd[G_L0 + f]) : float(gd[G_L0 + f] * rdet);
+    }
+
+    // ---------------- volume: evaluation, contravariant fluxes, projection -------------------
+    v4f acc[4];
+    {
+        v4f U[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < R; ++s) c = mfma16x16x4(tEV[s], b[v][s], c);
+            U[v] = c;
+        }
+        v4f Fh[4], Gh[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (r >= RQ) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) { Fh[v][r] = 0.0f; Gh[v][r] = 0.0f; }
+                continue;
+            }
+            const float U0 = U[0][r] + ur[0], U1 = U[1][r] + ur[1], U2 = U[2][r] + ur[2], U3 = U[3][r] + ur[3];
+            const float ri = rcp(U0);
+            const float uu = U1 * ri, vv = U2 * ri;
+            const float p = (gam - 1.0f) * (U3 - 0.5f * U0 * (uu * uu + vv * vv));
+            // contravariant velocities along the rows of dxi_dx^T
+            const float c1 = uu * a00 + vv * a10, c2 = uu * a01 + vv * a11;
+            Fh[0][r] = U0 * c1;            Gh[0][r] = U0 * c2;
+            Fh[1][r] = U1 * c1 + p * a00;  Gh[1][r] = U1 * c2 + p * a01;
+            Fh[2][r] = U2 * c1 + p * a10;  Gh[2][r] = U2 * c2 + p * a11;
+            Fh[3][r] = (U3 + p) * c1;      Gh[3][r] = (U3 + p) * c2;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < RQ; ++t) {
+                c = mfma16x16x4(tPX[t], Fh[v][t], c);
+                c = mfma16x16x4(tPY[t], Gh[v][t], c);
+            }
+            acc[v] = c;
+        }
+    }
+
+    // ---------------- faces ----------------------------------------------------------------
+    v4f Uf[4];  // own traces at the face slots
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < R; ++s) c = mfma16x16x4(tEF[s], b[v][s], c);
+        Uf[v] = c;
+    }
+    __syncthreads();  // neighbour traces complete
+    v4f Hs[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Hs[v] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const int eloc = 16 * w + j;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        // slot (g, f): face f, point g for f < 3; face g, point 4 for f = 3 (p = 4)
+        const int face = f < 3 ? f : g;
+        const int q = f < 3 ? g : 4;
+        if (f == 3 && (NFQ < 5 || g >= 3)) continue;
+        if (f < 3 && g >= NFQ) continue;
+        const uint32_t c = (code >> (4 * face)) & 15u;
+        const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
+        const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
+        const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
+        const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
+        const float4 tn = s_tr[(face * NFQ + q) * EB + eloc];
+        float Un[4] = {tn.x, tn.y, tn.z, tn.w};
+        const float Uo[4] = {Uf[0][f] + ur[0], Uf[1][f] + ur[1], Uf[2][f] + ur[2], Uf[3][f] + ur[3]};
+        float H[4];
+        face_point_flux(Uo, Un, c, nb, nx, ny, sf, sp, gam, H);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Hs[v][f] = H[v];
+    }
+
+    // lift: acc += LF^T H (LF carries -w_q/2 and, for the RK/RHS tables, M_ref^-1)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int t = 0; t < RF; ++t) acc[v] = mfma16x16x4(tLF[t], Hs[v][t], acc[v]);
+
+    // ---------------- output ---------------------------------------------------------------
+    if (!valid) return;
+    const auto r_out = make_rsrc(sp.out, state_bytes);
+    const float dt = float(sp.dt), ca = float(sp.ca), cb = float(sp.cb);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        v4u x;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            float o;
+            if (sp.mode != MODE_RK) o = acc[v][r];
+            else if (!rk_blend) o = b[v][r] + dt * acc[v][r];
+            else o = ca * unv[v][r] + cb * (b[v][r] + dt * acc[v][r]);
+            x[v] = __float_as_uint(o);
+        }
+        // padding slots address past the end and are dropped
+        __builtin_amdgcn_raw_buffer_store_b128(x, r_out, slot_off(r, unsigned(e)), 0, 0);
+    }
+}
+
+// p = 3 runs fastest at three waves per SIMD (the register budget that allows), p = 4 at the
+// compiler's own choice (measured on gfx950)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void stage_mfma_kernel_p3(StageParams sp) {
+    stage_mfma_body<3>(sp);
+}
+__global__ __launch_bounds__(kBlock) void stage_mfma_kernel_p4(StageParams sp) { stage_mfma_body<4>(sp); }
+
+}  // namespace
+
+hipError_t launch_stage_mfma(int p, const StageParams& sp, hipStream_t s) {
+    if (sp.K <= 0) return hipSuccess;
+    dim3 grid((sp.K + 63) / 64), block(kBlock);
+    switch (p) {
+    case 3: hipLaunchKernelGGL(stage_mfma_kernel_p3, grid, block, 0, s, sp); break;
+    case 4: hipLaunchKernelGGL(stage_mfma_kernel_p4, grid, block, 0, s, sp); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dgx

@@ -109,6 +109,7 @@ struct VjpParams {
     int K, Kp;
     double dt, ca, cb, gamma;
     int mode;
+    double uref[4];  // point-value storage offset of the state (zero in FP64)
 };
 
 template <int P, int RULE, typename Real>
@@ -147,19 +148,20 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
     for (int i = 0; i < NP; ++i)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            u[i][v] = uk[(i * 4 + v) * Kp + e];
-            lm[i][v] = lam[(i * 4 + v) * Kp + e];
+            u[i][v] = uk[sidx(i, v, e, Kp)];
+            lm[i][v] = lam[sidx(i, v, e, Kp)];
             ub[i][v] = Real(0);
         }
     const double det = geo[G_DET * Kp + e];
     const Real a00 = Real(geo[G_A00 * Kp + e]), a01 = Real(geo[G_A01 * Kp + e]);
     const Real a10 = Real(geo[G_A10 * Kp + e]), a11 = Real(geo[G_A11 * Kp + e]);
     const Real sv = residual_mode ? Real(det) : Real(1);
+    const Real ur[4] = {Real(sp.uref[0]), Real(sp.uref[1]), Real(sp.uref[2]), Real(sp.uref[3])};
 
     // ---------------- volume ----------------
 #pragma unroll 1
     for (int q = 0; q < NQ; ++q) {
-        Real U[4] = {0, 0, 0, 0}, a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+        Real U[4] = {ur[0], ur[1], ur[2], ur[3]}, a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
             Real ph = tab[O_phiV + q * NP + i];
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
 #pragma unroll
         for (int q = 0; q < NFQ; ++q)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) { Uo[q][v] = Real(0); psi[q][v] = Real(0); }
+            for (int v = 0; v < 4; ++v) { Uo[q][v] = ur[v]; psi[q][v] = Real(0); }
 #pragma unroll
         for (int q = 0; q < NFQ; ++q)
 #pragma unroll
@@ -226,14 +228,14 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
 #pragma unroll
             for (int q = 0; q < NFQ; ++q)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) Un[q][v] = Real(0);
+                for (int v = 0; v < 4; ++v) Un[q][v] = ur[v];
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
                 Real cu[4], cl[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    cu[v] = uk[(i * 4 + v) * Kp + nb];
-                    cl[v] = lam[(i * 4 + v) * Kp + nb];
+                    cu[v] = uk[sidx(i, v, nb, Kp)];
+                    cl[v] = lam[sidx(i, v, nb, Kp)];
                 }
                 const Real* colp = &s_phiFT[(fn * NP + i) * NFQ];
                 const Real* coll = &s_LFT[(fn * NP + i) * NFQ];
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
         for (int i = 0; i < NP; ++i)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int k = (i * 4 + v) * Kp + e;
+                const int k = sidx(i, v, e, Kp);
                 Real o = cb * (lm[i][v] + dt * ub[i][v]);
                 if (add) o += add[k];
                 out[k] = o;
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
 #pragma unroll
         for (int i = 0; i < NP; ++i)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) out[(i * 4 + v) * Kp + e] = ub[i][v];
+            for (int v = 0; v < 4; ++v) out[sidx(i, v, e, Kp)] = ub[i][v];
     }
 }
 
@@ -308,16 +310,17 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
 template <typename Real>
 __global__ void ke_grad_kernel(const Real* __restrict__ u, const double* __restrict__ tab64,
                                const double* __restrict__ qw, const double* __restrict__ geo,
-                               Real* __restrict__ grad, int np, int nq, int phiV_off, int K, int Kp) {
+                               Real* __restrict__ grad, int np, int nq, int phiV_off, int K, int Kp,
+                               Off4 uref) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= K) return;
     double det = geo[G_DET * Kp + e];
     double gacc[15 * 4];
     for (int k = 0; k < np * 4; ++k) gacc[k] = 0.0;
     for (int q = 0; q < nq; ++q) {
-        double U[4] = {0, 0, 0, 0};
+        double U[4] = {uref.v[0], uref.v[1], uref.v[2], uref.v[3]};
         for (int i = 0; i < np; ++i)
-            for (int v = 0; v < 4; ++v) U[v] += tab64[phiV_off + q * np + i] * double(u[(i * 4 + v) * Kp + e]);
+            for (int v = 0; v < 4; ++v) U[v] += tab64[phiV_off + q * np + i] * double(u[sidx(i, v, e, Kp)]);
         double ri = 1.0 / U[0];
         double dk[4] = {-0.5 * (U[1] * U[1] + U[2] * U[2]) * ri * ri, U[1] * ri, U[2] * ri, 0.0};
         double w = qw[q] * det;
@@ -326,7 +329,7 @@ __global__ void ke_grad_kernel(const Real* __restrict__ u, const double* __restr
             for (int v = 0; v < 4; ++v) gacc[i * 4 + v] += ph * dk[v];
         }
     }
-    for (int k = 0; k < np * 4; ++k) grad[k * Kp + e] = Real(gacc[k]);
+    for (int k = 0; k < np * 4; ++k) grad[sidx(k >> 2, k & 3, e, Kp)] = Real(gacc[k]);
 }
 
 template <typename Real>
@@ -351,22 +354,23 @@ hipError_t launch_vjp_real(int p, int rule, const VjpParams& vp, hipStream_t s) 
 hipError_t launch_vjp(int precision, int p, int rule, const void* uk, const void* lam, const void* add,
                       const void* acc_in, void* acc_out, void* out, const double* geo, const int32_t* nbr,
                       const uint32_t* code, const void* tab, const double* bc, int K, int Kp, double dt,
-                      double ca, double cb, double gamma, int mode, hipStream_t s) {
+                      double ca, double cb, double gamma, int mode, Off4 uref, hipStream_t s) {
     if (K <= 0) return hipSuccess;
-    VjpParams vp{uk, lam, add, acc_in, acc_out, out, geo, nbr, code, tab, bc, K, Kp, dt, ca, cb, gamma, mode};
+    VjpParams vp{uk, lam, add, acc_in, acc_out, out, geo, nbr, code, tab, bc, K, Kp, dt, ca, cb, gamma, mode,
+                 {uref.v[0], uref.v[1], uref.v[2], uref.v[3]}};
     return precision == DGX_FP64 ? launch_vjp_real<double>(p, rule, vp, s) : launch_vjp_real<float>(p, rule, vp, s);
 }
 
 hipError_t launch_ke_grad(int precision, int p, int rule, const void* u, const double* tab64, const double* qw,
-                          const double* geo, void* grad, int K, int Kp, hipStream_t s) {
+                          const double* geo, void* grad, int K, int Kp, Off4 uref, hipStream_t s) {
     TabLayout L(p, rule);
     dim3 grid((K + 255) / 256), block(256);
     if (precision == DGX_FP64)
         hipLaunchKernelGGL(ke_grad_kernel<double>, grid, block, 0, s, (const double*)u, tab64, qw, geo, (double*)grad,
-                           L.np, L.nq, L.phiV, K, Kp);
+                           L.np, L.nq, L.phiV, K, Kp, uref);
     else
         hipLaunchKernelGGL(ke_grad_kernel<float>, grid, block, 0, s, (const float*)u, tab64, qw, geo, (float*)grad,
-                           L.np, L.nq, L.phiV, K, Kp);
+                           L.np, L.nq, L.phiV, K, Kp, uref);
     return hipGetLastError();
 }
 

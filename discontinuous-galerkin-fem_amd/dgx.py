@@ -78,7 +78,7 @@ class Info(C.Structure):
     _fields_ = [("n_elements", C.c_int32), ("n_modes", C.c_int32), ("n_volume_points", C.c_int32),
                 ("n_face_points", C.c_int32), ("order", C.c_int32), ("basis", C.c_int32),
                 ("precision", C.c_int32), ("physics", C.c_int32), ("n_boundary_faces", C.c_int32),
-                ("device", C.c_int32), ("bytes_per_stage_alg", C.c_double)]
+                ("device", C.c_int32), ("bytes_per_stage_alg", C.c_double), ("stage_kernel", C.c_int32)]
 
 
 _lib = None

@@ -96,6 +96,9 @@ typedef struct {
     int32_t n_boundary_faces;
     int32_t device;
     double bytes_per_stage_alg; /* algorithmic HBM bytes of one fused stage launch */
+    int32_t stage_kernel;       /* 0: one element per lane; 1: 16-element matrix-core tiles
+                                   (FP32 Euler, p = 3, 4); DGX_STAGE_KERNEL=lane at creation
+                                   selects 0 */
 } dgx_info_t;
 
 /* ---- mesh helpers (host) ------------------------------------------------------------------ */
@@ -156,8 +159,9 @@ dgx_status dgx_rk3_steps(dgx_ctx* ctx, double dt, int32_t nsteps);
 dgx_status dgx_rk3_steps_async(dgx_ctx* ctx, double dt, int32_t nsteps);
 dgx_status dgx_synchronize(dgx_ctx* ctx);
 
-/* Runs nsteps steps and reports the mean device duration of one fused stage launch measured
- * with HIP events on the context stream (ms) and the wall time of the whole loop (ms). */
+/* Runs one warm-up step and nsteps timed steps and reports the mean duration of one fused
+ * stage launch (HIP events around the 3 nsteps back-to-back launches on the context stream,
+ * divided by their number; ms) and the event time of the whole timed loop (ms). */
 dgx_status dgx_rk3_profile(dgx_ctx* ctx, double dt, int32_t nsteps, double* stage_ms,
                            double* total_ms);
 

@@ -91,6 +91,44 @@ def test_rk3_fp32_field_error():
         assert err < 1e-5, (v, err)
 
 
+def test_fp32_state_round_trip_and_diagnostics():
+    # the FP32 state is stored as a deviation from its mean on the constant mode: set/get
+    # round-trips to FP32 rounding of that deviation, and every diagnostic adds the offset back
+    ora, ctx, u0 = _vortex_setup(12, 3, "dubiner", "fp32", seed=9)
+    ctx.set_state(u0)
+    back = ctx.get_state()
+    dev = u0.copy()
+    dev[:, 0, :] -= u0[:, 0, :].mean(axis=0)
+    # the offset itself is the mean rounded to FP32
+    slack = 2 ** -22 * np.abs(u0[:, 0, :].mean(axis=0))
+    assert np.all(np.abs(back - u0) <= 2 ** -23 * np.abs(dev) + slack)
+    s_ref = ora.conservation_sums(u0)
+    assert np.allclose(ctx.conservation_sums(), s_ref, rtol=1e-6, atol=1e-6)
+    assert abs(ctx.kinetic_energy() - ora.kinetic_energy(u0)) < 1e-6 * ora.kinetic_energy(u0)
+    assert np.allclose(ctx.density_range(), ora.density_range(u0), rtol=1e-6)
+    assert abs(ctx.max_cfl(0.01) - ora.max_cfl(u0, 0.01)) < 1e-6 * ora.max_cfl(u0, 0.01)
+    xy = ctx.volume_points()
+    perm = point_permutation(xy, ora.volume_points())
+    assert _rel(ctx.eval_volume(), ora.eval_volume(u0)[:, perm]) < 1e-6
+    # a projection sets its own offset (the mean of the point values)
+    ctx.project(R.vortex_state(xy))
+    assert _rel(ctx.get_state(), ora.project(R.vortex_state(ora.volume_points()))) < 1e-6
+
+
+def test_rk3_fp32_error_scales_with_deviation():
+    # 100 steps at the headline CFL: with the offset the density error stays below 1e-6
+    ora, ctx, u0 = _vortex_setup(16, 3, "dubiner", "fp32", seed=5)
+    c64 = dgx.Context(*R.split_quad_mesh(16), 3, basis="dubiner", precision="fp64")
+    ctx.set_state(u0)
+    c64.set_state(u0)
+    dt = dgx.stable_dt(c64, 0.3)
+    ctx.rk3_steps(dt, 100)
+    c64.rk3_steps(dt, 100)
+    a, b = ctx.get_state(), c64.get_state()
+    err = np.linalg.norm(a[..., 0] - b[..., 0]) / np.linalg.norm(b[..., 0])
+    assert err < 1e-6, err
+
+
 def test_free_stream_preserved_fp32():
     verts, tris = R.split_quad_mesh(8)
     ctx = dgx.Context(verts, tris, 3, precision="fp32")

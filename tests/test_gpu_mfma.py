@@ -1,0 +1,106 @@
+"""The matrix-core stage kernel (FP32 Euler, p = 3 and 4) against the one-element-per-lane
+stage kernel and the FP64 oracle, on periodic and bounded meshes whose element counts are not
+multiples of the 16-element wave tile or the 64-element block.
+
+Tolerances: the residual is a small difference of volume and face terms that are each larger
+by about the inverse mesh width, so FP32 rounding shows up at 1e-6 to 1e-5 of its largest entry
+and differently in the two kernels' summation orders.  Each kernel is therefore held against the
+FP64 oracle (2e-5 on meshes of a few dozen elements, 1e-4 at 4050 elements) and the
+matrix-core kernel's error may not exceed twice the per-lane kernel's (plus 1e-6).  After three
+RK3 steps the two kernels' states agree to relative 1e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import dgx
+import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _context(lane, *args, **kw):
+    # DGX_STAGE_KERNEL is read when the context is created
+    if lane:
+        os.environ["DGX_STAGE_KERNEL"] = "lane"
+    try:
+        return dgx.Context(*args, **kw)
+    finally:
+        os.environ.pop("DGX_STAGE_KERNEL", None)
+
+
+FAR = [1.0, 1.0, 0.0, 1 / 0.4 + 0.5]
+MIXED = {"Bottom": ("far_field", FAR), "Top": ("slip_wall", [0, 0, 0, 0]),
+         "Left": ("no_slip_wall", [1.0, 0.0, 0.0, 1.0])}  # Right: no condition
+
+
+def _setup(p, case, n=5):
+    length = 10.0 if case == "periodic" else 1.0
+    verts, tris = R.split_quad_mesh(n, length)
+    if case == "periodic":
+        ora = R.Oracle(verts, tris, p, basis="dubiner")
+        kw = {}
+        xy = ora.volume_points()
+        pert = np.repeat(0.01 * R.uniform_pm1(3, np.arange(ora.K))[:, None], ora.Nq, axis=1)
+        U = R.vortex_state(xy, rho_pert=pert)
+    else:
+        ora = R.Oracle(verts, tris, p, basis="dubiner", periodic=0)
+        bcs = MIXED if case == "mixed" else {t: (case, FAR if case == "far_field" else
+                                                 [1.0, 0.0, 0.0, 1.0] if case == "no_slip_wall" else [0, 0, 0, 0])
+                                             for t in ("Bottom", "Right", "Top", "Left")}
+        for tag, (typ, val) in bcs.items():
+            ora.set_bc(tag, typ, val)
+        kw = dict(periodic=0, bcs=bcs)
+        xy = ora.volume_points()
+        U = R.vortex_state(xy, xc=0.5, yc=0.5, beta=1.0, length=1.0)
+    u0 = ora.project(U)
+    mf = _context(False, verts, tris, p, basis="dubiner", precision="fp32", **kw)
+    ln = _context(True, verts, tris, p, basis="dubiner", precision="fp32", **kw)
+    return ora, mf, ln, u0
+
+
+@pytest.mark.parametrize("p", [3, 4])
+@pytest.mark.parametrize("case", ["periodic", "far_field", "slip_wall", "no_slip_wall", "mixed"])
+def test_mfma_residual_matches_lane_kernel_and_oracle(p, case):
+    ora, mf, ln, u0 = _setup(p, case)
+    assert mf.K % 16 != 0
+    r_mf, r_ln, r_ref = mf.residual(u0), ln.residual(u0), ora.residual(u0)
+    e_mf, e_ln = _rel(r_mf, r_ref), _rel(r_ln, r_ref)
+    assert e_mf < 2e-5 and e_ln < 2e-5, (e_mf, e_ln)
+    assert e_mf < 2 * e_ln + 1e-6, (e_mf, e_ln)
+
+
+@pytest.mark.parametrize("p", [3, 4])
+@pytest.mark.parametrize("case", ["periodic", "mixed"])
+def test_mfma_rk3_steps_match_lane_kernel(p, case):
+    ora, mf, ln, u0 = _setup(p, case, n=7)
+    dt = 0.2 * dgx.stable_dt(ln, 0.3)
+    for c in (mf, ln):
+        c.set_state(u0)
+        c.rk3_steps(dt, 3)
+    a, b = mf.get_state(), ln.get_state()
+    assert _rel(a, b) < 1e-6
+    u_ref, _ = ora.rk3(u0, dt, 3)
+    assert _rel(a, u_ref) < 2e-5
+
+
+def test_mfma_large_mesh_tail_block():
+    # 2 * 45^2 = 4050 elements: 63 full blocks of 64 and a partial one
+    verts, tris = R.split_quad_mesh(45)
+    ora = R.Oracle(verts, tris, 3, basis="dubiner")
+    xy = ora.volume_points()
+    u0 = ora.project(R.vortex_state(xy))
+    mf = _context(False, verts, tris, 3, basis="dubiner", precision="fp32")
+    ln = _context(True, verts, tris, 3, basis="dubiner", precision="fp32")
+    r_ref = ora.residual(u0)
+    e_mf, e_ln = _rel(mf.residual(u0), r_ref), _rel(ln.residual(u0), r_ref)
+    assert e_mf < 1e-4 and e_ln < 1e-4, (e_mf, e_ln)
+    assert e_mf < 2 * e_ln + 1e-6, (e_mf, e_ln)
+    # the tail block's elements see the same operator: compare the last 50 elements alone
+    tail = slice(mf.K - 50, mf.K)
+    assert _rel(mf.residual(u0)[tail], r_ref[tail]) < 1e-4
