@@ -329,6 +329,10 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         c.upload(c.tabML, m.data(), m.size() * sizeof(float));
         build_mfma_tables(c.p, tR, m);
         c.upload(c.tabMR, m.data(), m.size() * sizeof(float));
+        build_mfma_vjp_tables(c.p, tL, m);
+        c.upload(c.tabVL, m.data(), m.size() * sizeof(float));
+        build_mfma_vjp_tables(c.p, tR, m);
+        c.upload(c.tabVR, m.data(), m.size() * sizeof(float));
     }
     c.u.alloc(c.state_bytes());
     c.u1.alloc(c.state_bytes());
@@ -676,7 +680,8 @@ void vjp(dgx_ctx* c, const void* uk, const void* lam, const void* add, const voi
          int mode, double dt, double ca, double cb) {
     hip_check(launch_vjp(c->precision, c->p, c->rule, uk, lam, add, acc_in, acc_out, out, c->geo.as<double>(),
                          c->nbr.as<int32_t>(), c->code.as<uint32_t>(), mode == MODE_RESIDUAL ? c->tabR.p : c->tabL.p,
-                         c->bc.as<double>(), c->K, c->Kp, dt, ca, cb, c->gamma, mode, c->uref, c->stream),
+                         c->bc.as<double>(), c->K, c->Kp, dt, ca, cb, c->gamma, mode, c->uref,
+                         mode == MODE_RESIDUAL ? c->tabVR.p : c->tabVL.p, c->stream),
               "vjp kernel launch");
 }
 

@@ -99,12 +99,13 @@ struct dgx_ctx {
     hipStream_t stream = nullptr;
     DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
     DevBuf tabML, tabMR;  // MFMA stage kernel lane tables (RK/RHS and residual variants)
+    DevBuf tabVL, tabVR;  // MFMA stage adjoint lane tables (same two variants)
     std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][3][2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~dgx_ctx() {
         for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage,
-                          &partial, &tabML, &tabMR})
+                          &partial, &tabML, &tabMR, &tabVL, &tabVR})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);

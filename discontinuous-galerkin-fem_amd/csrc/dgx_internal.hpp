@@ -110,13 +110,34 @@ hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u
                          double gamma, Off4 uref, Off4 off, double* partial, int nblocks, hipStream_t s);
 hipError_t launch_apply_minv(int precision, int np, const double* minv_ref, const double* geo,
                              double* r_dev, int K, int Kp, hipStream_t s);
+// Parameters of one stage-adjoint launch (dgx_vjp.hip, dgx_vjp_mfma.hip).
+struct VjpParams {
+    const void* uk;      // stage input state
+    const void* lam;     // adjoint of the stage output (or of L / R in RHS / RESIDUAL mode)
+    const void* add;     // optional addend for the output (may be null)
+    const void* acc_in;  // optional accumulator input (may be null)
+    void* acc_out;       // optional accumulator output: acc_in + ca lam (may be null)
+    void* out;           // lam_uk
+    const double* geo;
+    const int32_t* nbr;
+    const uint32_t* code;
+    const void* tab;
+    const double* bc;
+    int K, Kp;
+    double dt, ca, cb, gamma;
+    int mode;
+    double uref[4];    // point-value storage offset of the state (zero in FP64)
+    const void* tabm;  // lane tables of the matrix-core VJP kernel (FP32 Euler), or null
+};
+
 // Euler stage adjoint (dgx_vjp.hip): out = cb (lam + dt (dL/du)^T lam) + add,
 // acc_out = acc_in + ca lam (mode RK); out = (dL/du)^T lam (mode RHS) or (dR/du)^T lam
 // (mode RESIDUAL, unfolded tables).
 hipError_t launch_vjp(int precision, int p, int rule, const void* uk, const void* lam, const void* add,
                       const void* acc_in, void* acc_out, void* out, const double* geo, const int32_t* nbr,
                       const uint32_t* code, const void* tab, const double* bc, int K, int Kp, double dt,
-                      double ca, double cb, double gamma, int mode, Off4 uref, hipStream_t s);
+                      double ca, double cb, double gamma, int mode, Off4 uref, const void* tabm,
+                      hipStream_t s);
 // Batched persistent stage (dgx_batch.hip): items of (instance, first element, count, order);
 // params[instance] holds that instance's stage parameters; tabs[p-1] the order-p tables.
 struct BatchItem {

@@ -41,6 +41,33 @@ struct MfmaShape {
     static constexpr int TOTAL = T_N * 64 + NT;
 };
 
+// Lane tables of the matrix-core stage adjoint (dgx_vjp_mfma.hip), 64 floats each:
+//   EV[R]  Phi_V  (rows: volume slots, k: modes)     U at the volume points
+//   XV[R]  PX     (rows: volume slots, k: modes)     lambda projected for the x-flux adjoint
+//   YV[R]  PY     (rows: volume slots, k: modes)     lambda projected for the y-flux adjoint
+//   VT[RQ] Phi_V  (rows: modes, k: volume slots)     transposed evaluation
+//   EF[R]  Phi_F  (rows: face slots, k: modes)       own traces
+//   LE[R]  LF     (rows: face slots, k: modes)       own lift weights of lambda
+//   FT[RF] Phi_F  (rows: modes, k: face slots)       transposed trace evaluation
+// then NT[fn*2 + flip][i][q] (Phi_F of the neighbour, as for the forward kernel) and
+// NL[fn*2 + flip][i][q] (its LF).  Padding rows of EV / EF repeat point 0 (finite states);
+// every other padding entry is zero.
+template <int P>
+struct MfmaVjpShape {
+    using F = MfmaShape<P>;
+    static constexpr int R = F::R, RQ = F::RQ, RF = F::RF;
+    static constexpr int T_EV = 0;
+    static constexpr int T_XV = T_EV + R;
+    static constexpr int T_YV = T_XV + R;
+    static constexpr int T_VT = T_YV + R;
+    static constexpr int T_EF = T_VT + RQ;
+    static constexpr int T_LE = T_EF + R;
+    static constexpr int T_FT = T_LE + R;
+    static constexpr int T_N = T_FT + RF;
+    static constexpr int NT = F::NT;
+    static constexpr int TOTAL = T_N * 64 + 2 * NT;
+};
+
 // slot -> mode / volume point / face point (f * nfq + q); -1 for a padding slot
 __host__ __device__ constexpr int mf_mode(int R, int np, int g, int r) {
     return (r < R && R * g + r < np) ? R * g + r : -1;
@@ -57,5 +84,6 @@ constexpr bool mfma_order(int p) { return p == 3 || p == 4; }
 int mfma_table_size(int p);  // floats
 // Build the lane-major tables of order p from the (L or R) tables in TabLayout order.
 void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out);
+void build_mfma_vjp_tables(int p, const std::vector<double>& tab, std::vector<float>& out);
 
 }  // namespace dgx

@@ -53,7 +53,60 @@ void build(const std::vector<double>& t, std::vector<float>& out) {
                 }
 }
 
+template <int P>
+void build_vjp(const std::vector<double>& t, std::vector<float>& out) {
+    using S = MfmaVjpShape<P>;
+    using F = MfmaShape<P>;
+    const TabLayout L(P, 0);
+    const int np = F::NP, nq = F::NQ, nfq = F::NFQ;
+    if ((int)t.size() != L.total) throw std::runtime_error("table size mismatch");
+    out.assign(S::TOTAL, 0.0f);
+    auto at = [&](int table, int lane) -> float& { return out[(size_t)table * 64 + lane]; };
+    for (int lane = 0; lane < 64; ++lane) {
+        const int i = lane % 16, k = lane / 16;
+        const int vq = mf_vpoint(S::RQ, nq, i / 4, i % 4), fq = mf_fpoint(nfq, i / 4, i % 4);
+        const int vp = vq < 0 ? 0 : vq, fp = fq < 0 ? 0 : fq;
+        for (int s = 0; s < S::R; ++s) {
+            const int m = mf_mode(S::R, np, k, s);
+            if (m < 0) continue;
+            at(S::T_EV + s, lane) = float(t[L.phiV + vp * np + m]);
+            at(S::T_XV + s, lane) = vq < 0 ? 0.0f : float(t[L.PX + vq * np + m]);
+            at(S::T_YV + s, lane) = vq < 0 ? 0.0f : float(t[L.PY + vq * np + m]);
+            at(S::T_EF + s, lane) = float(t[L.phiF + fp * np + m]);
+            at(S::T_LE + s, lane) = fq < 0 ? 0.0f : float(t[L.LF + fq * np + m]);
+        }
+        const int mrow = mf_mode(S::R, np, i / 4, i % 4);
+        for (int r = 0; r < S::RQ; ++r) {
+            const int q = mf_vpoint(S::RQ, nq, k, r);
+            at(S::T_VT + r, lane) = (mrow >= 0 && q >= 0) ? float(t[L.phiV + q * np + mrow]) : 0.0f;
+        }
+        for (int r = 0; r < S::RF; ++r) {
+            const int f = mf_fpoint(nfq, k, r);
+            at(S::T_FT + r, lane) = (mrow >= 0 && f >= 0) ? float(t[L.phiF + f * np + mrow]) : 0.0f;
+        }
+    }
+    float* nt = &out[(size_t)S::T_N * 64];
+    float* nl = nt + S::NT;
+    for (int fn = 0; fn < 3; ++fn)
+        for (int flip = 0; flip < 2; ++flip)
+            for (int m = 0; m < np; ++m)
+                for (int q = 0; q < nfq; ++q) {
+                    const int qq = flip ? nfq - 1 - q : q;
+                    const size_t o = ((size_t)(fn * 2 + flip) * np + m) * nfq + q;
+                    nt[o] = float(t[L.phiF + (fn * nfq + qq) * np + m]);
+                    nl[o] = float(t[L.LF + (fn * nfq + qq) * np + m]);
+                }
+}
+
 }  // namespace
+
+void build_mfma_vjp_tables(int p, const std::vector<double>& tab, std::vector<float>& out) {
+    switch (p) {
+    case 3: build_vjp<3>(tab, out); break;
+    case 4: build_vjp<4>(tab, out); break;
+    default: throw std::runtime_error("no MFMA adjoint tables for this order");
+    }
+}
 
 int mfma_table_size(int p) {
     switch (p) {

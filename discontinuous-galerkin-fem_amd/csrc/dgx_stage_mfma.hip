@@ -20,75 +20,10 @@
 // per-element table (the neighbour's local face and point order), so it stays on the VALU: each
 // lane group forms the partial sum over its own mode slots of the neighbour's coefficients and
 // the four partials of a face point are summed through LDS.
-#include "dgx_mfma.hpp"
-#include "dgx_stage_impl.hpp"
+#include "dgx_mfma_dev.hpp"
 
 namespace dgx {
 namespace {
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-
-// Ghost state of a boundary face point (boundary_face_residual, euler_weak_formulation.cpp:
-// 208-233); a face without a condition (type < 0) mirrors the interior state.
-__device__ __forceinline__ void ghost_state(const float* Uo, float nx, float ny, const double* bcr, float gam,
-                                            float* Un) {
-    const int type = int(bcr[0]);
-    if (type < 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Un[v] = Uo[v];
-    } else if (type == DGX_BC_FAR_FIELD) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Un[v] = float(bcr[1 + v]);
-    } else if (type == DGX_BC_SLIP_WALL) {
-        const float ri = rcp(Uo[0]);
-        const float w1 = Uo[1] * ri, w2 = Uo[2] * ri;
-        const float W[4] = {Uo[0], w1, w2, (gam - 1.0f) * (Uo[3] - 0.5f * Uo[0] * (w1 * w1 + w2 * w2))};
-        const float vn = W[1] * nx + W[2] * ny;
-        const float Wg[4] = {W[0], W[1] - 2.0f * vn * nx, W[2] - 2.0f * vn * ny, W[3]};
-        prim2cons(Wg, gam, Un);
-    } else {
-        const float Wb[4] = {float(bcr[1]), float(bcr[2]), float(bcr[3]), float(bcr[4])};
-        prim2cons(Wb, gam, Un);
-    }
-}
-
-// Flux of one face point in the canonical (L) orientation, scaled for this element's lift.
-__device__ __forceinline__ void face_point_flux(const float* Uo, float* Un, uint32_t c, int nb, float nx, float ny,
-                                                float sf, const StageParams& sp, float gam, float* Hs) {
-    const bool boundary = (c & 3u) == kFaceBoundary;
-    const bool isL = (c >> 3) & 1u;
-    if (boundary) ghost_state(Uo, nx, ny, sp.bc + 5 * nb, gam, Un);
-    // a boundary face without a boundary condition contributes nothing
-    // (S/solver/assembler.cpp:233-243)
-    const float sfe = (boundary && sp.bc[5 * nb] < 0.0) ? 0.0f : sf;
-    const bool own_is_a = isL || boundary;
-    const float sgn = own_is_a ? 1.0f : -1.0f;
-    float Aq[4], Bq[4], H[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        Aq[v] = own_is_a ? Uo[v] : Un[v];
-        Bq[v] = own_is_a ? Un[v] : Uo[v];
-    }
-    rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Hs[v] = sgn * sfe * H[v];
-}
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
-    // raw buffer (stride 0), range-checked against `bytes`; dword 3 as for every gfx9 target
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), 0x00020000);
-}
-typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ v4u load_x4(__amdgpu_buffer_rsrc_t r, unsigned voff) {
-    return __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
-}
-__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
 
 template <int P>
 __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {

@@ -17,100 +17,15 @@
 //   stage  out = ca un + cb (uk + dt L(uk))
 //   =>     lam_uk = cb (lam + dt (dL/du)^T lam) + add,   acc_out = acc_in + ca lam
 // Mode RESIDUAL uses the unfolded tables (|J| and len scaling) and returns (dR/du)^T lam.
-#include "dgx_internal.hpp"
+#include "dgx_mfma.hpp"
+#include "dgx_vjp_math.hpp"
 
 namespace dgx {
+
+hipError_t launch_vjp_mfma(int p, const VjpParams& vp, hipStream_t s);
+
 namespace {
 
-template <typename Real>
-__device__ __forceinline__ Real rcp_(Real x) { return Real(1) / x; }
-template <>
-__device__ __forceinline__ float rcp_<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-
-__device__ __forceinline__ int xcd_tile_(int bid, int nblk) {
-    if ((nblk & 7) != 0) return bid;
-    return (bid & 7) * (nblk >> 3) + (bid >> 3);
-}
-
-// Ubar += (dFn/dU)^T psi + sbar * d(|un| + c)/dU, Fn = (r un, mx un + p nx, my un + p ny,
-// (E+p) un), un = (u nx + v ny) for the (not necessarily unit) direction n.
-template <typename Real>
-__device__ __forceinline__ void flux_dir_adjoint(const Real* U, Real nx, Real ny, const Real* psi,
-                                                 Real sbar, Real g, Real* Ubar) {
-    const Real gm1 = g - Real(1);
-    Real r = U[0], mx = U[1], my = U[2], E = U[3];
-    Real ri = rcp_(r);
-    Real u = mx * ri, v = my * ri;
-    Real k = Real(0.5) * (u * u + v * v);
-    Real p = gm1 * (E - r * k);
-    Real un = u * nx + v * ny;
-    Real unbar = psi[0] * r + psi[1] * mx + psi[2] * my + psi[3] * (E + p);
-    Real pbar = psi[1] * nx + psi[2] * ny + psi[3] * un;
-    Real rbar = psi[0] * un, mxbar = psi[1] * un, mybar = psi[2] * un, Ebar = psi[3] * un;
-    if (sbar != Real(0)) {
-        // s = |un| + c, c = sqrt(g p / r)
-        Real c = sqrt(g * p * ri);
-        unbar += sbar * (un >= Real(0) ? Real(1) : Real(-1));
-        pbar += sbar * g / (Real(2) * c * r);
-        rbar -= sbar * c / (Real(2) * r);
-    }
-    Real ubar = unbar * nx, vbar = unbar * ny;
-    Ebar += gm1 * pbar;
-    rbar -= gm1 * k * pbar;
-    ubar -= gm1 * r * u * pbar;
-    vbar -= gm1 * r * v * pbar;
-    mxbar += ubar * ri;
-    mybar += vbar * ri;
-    rbar -= (ubar * u + vbar * v) * ri;
-    Ubar[0] += rbar;
-    Ubar[1] += mxbar;
-    Ubar[2] += mybar;
-    Ubar[3] += Ebar;
-}
-
-template <typename Real>
-__device__ __forceinline__ Real wave_speed(const Real* U, Real nx, Real ny, Real g) {
-    Real ri = rcp_(U[0]);
-    Real u = U[1] * ri, v = U[2] * ri;
-    Real p = (g - Real(1)) * (U[3] - Real(0.5) * U[0] * (u * u + v * v));
-    return fabs(u * nx + v * ny) + sqrt(g * p * ri);
-}
-
-// Ubar_own = (dH/dU_own)^T psi for H(A, B, n) = 1/2 (Fn(A) + Fn(B)) - 1/2 s (B - A),
-// s = max(sA, sB) (A branch at ties).  own_is_A selects the side.
-template <typename Real>
-__device__ __forceinline__ void rusanov_adjoint(const Real* A, const Real* B, Real nx, Real ny,
-                                                const Real* psi, bool own_is_A, Real g, Real* Ubar) {
-    Real sA = wave_speed(A, nx, ny, g), sB = wave_speed(B, nx, ny, g);
-    bool from_A = sA >= sB;
-    Real s = from_A ? sA : sB;
-    Real jump = psi[0] * (B[0] - A[0]) + psi[1] * (B[1] - A[1]) + psi[2] * (B[2] - A[2]) + psi[3] * (B[3] - A[3]);
-    Real half_psi[4] = {Real(0.5) * psi[0], Real(0.5) * psi[1], Real(0.5) * psi[2], Real(0.5) * psi[3]};
-    const Real* own = own_is_A ? A : B;
-    Real sbar = (own_is_A == from_A) ? Real(-0.5) * jump : Real(0);
-    flux_dir_adjoint(own, nx, ny, half_psi, sbar, g, Ubar);
-    Real sgn = own_is_A ? Real(0.5) : Real(-0.5);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Ubar[v] += sgn * s * psi[v];
-}
-
-struct VjpParams {
-    const void* uk;      // stage input state
-    const void* lam;     // adjoint of the stage output (or of L / R in RHS / RESIDUAL mode)
-    const void* add;     // optional addend for the output (may be null)
-    const void* acc_in;  // optional accumulator input (may be null)
-    void* acc_out;       // optional accumulator output: acc_in + ca lam (may be null)
-    void* out;           // lam_uk
-    const double* geo;
-    const int32_t* nbr;
-    const uint32_t* code;
-    const void* tab;
-    const double* bc;
-    int K, Kp;
-    double dt, ca, cb, gamma;
-    int mode;
-    double uref[4];  // point-value storage offset of the state (zero in FP64)
-};
 
 template <int P, int RULE, typename Real>
 __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
@@ -251,13 +166,23 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
                 }
             }
         } else {
+            // constant ghost states: FAR_FIELD conserved, NO_SLIP_WALL from primitives (the
+            // host rejects slip walls, whose ghost depends on the interior state)
             const double* bcr = sp.bc + 5 * nb;
             const int type = int(bcr[0]);
             skip = type < 0;
+            const Real gm1 = g - Real(1);
+            Real Ug[4] = {Real(bcr[1]), Real(bcr[2]), Real(bcr[3]), Real(bcr[4])};
+            if (type == DGX_BC_NO_SLIP_WALL) {
+                const Real r = Ug[0], uu = Ug[1], vv = Ug[2], pp = Ug[3];
+                Ug[1] = r * uu;
+                Ug[2] = r * vv;
+                Ug[3] = pp / gm1 + Real(0.5) * r * (uu * uu + vv * vv);
+            }
 #pragma unroll
             for (int q = 0; q < NFQ; ++q)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) Un[q][v] = skip ? Uo[q][v] : Real(bcr[1 + v]);
+                for (int v = 0; v < 4; ++v) Un[q][v] = skip ? Uo[q][v] : Ug[v];
         }
         if (skip) continue;
 
@@ -354,10 +279,12 @@ hipError_t launch_vjp_real(int p, int rule, const VjpParams& vp, hipStream_t s) 
 hipError_t launch_vjp(int precision, int p, int rule, const void* uk, const void* lam, const void* add,
                       const void* acc_in, void* acc_out, void* out, const double* geo, const int32_t* nbr,
                       const uint32_t* code, const void* tab, const double* bc, int K, int Kp, double dt,
-                      double ca, double cb, double gamma, int mode, Off4 uref, hipStream_t s) {
+                      double ca, double cb, double gamma, int mode, Off4 uref, const void* tabm,
+                      hipStream_t s) {
     if (K <= 0) return hipSuccess;
     VjpParams vp{uk, lam, add, acc_in, acc_out, out, geo, nbr, code, tab, bc, K, Kp, dt, ca, cb, gamma, mode,
-                 {uref.v[0], uref.v[1], uref.v[2], uref.v[3]}};
+                 {uref.v[0], uref.v[1], uref.v[2], uref.v[3]}, tabm};
+    if (tabm && precision == DGX_FP32 && rule == 0 && mfma_order(p)) return launch_vjp_mfma(p, vp, s);
     return precision == DGX_FP64 ? launch_vjp_real<double>(p, rule, vp, s) : launch_vjp_real<float>(p, rule, vp, s);
 }
 

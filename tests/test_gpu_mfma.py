@@ -104,3 +104,69 @@ def test_mfma_large_mesh_tail_block():
     # the partial last block (its residual is free-stream small) against the global scale
     tail = slice(mf.K - 50, mf.K)
     assert np.abs(mf.residual(u0)[tail] - r_ref[tail]).max() < 1e-4 * np.abs(r_ref).max()
+
+
+# ---------------- stage adjoint ------------------------------------------------------------
+
+def _adj_setup(p, case, n=5):
+    ora, mf, ln, u0 = _setup(p, "periodic" if case == "periodic" else case, n)
+    verts, tris = R.split_quad_mesh(n, 10.0 if case == "periodic" else 1.0)
+    kw = {} if case == "periodic" else dict(periodic=0, bcs=MIXED if case == "mixed" else {
+        t: (case, FAR if case == "far_field" else [1.0, 0.0, 0.0, 1.0]) for t in ("Bottom", "Right", "Top", "Left")})
+    c64 = dgx.Context(verts, tris, p, basis="dubiner", precision="fp64", **kw)
+    return ora, mf, ln, c64, u0
+
+
+ADJ_MIXED = {"Bottom": ("far_field", FAR), "Left": ("no_slip_wall", [1.0, 0.0, 0.0, 1.0])}
+
+
+@pytest.mark.parametrize("p", [3, 4])
+@pytest.mark.parametrize("case", ["periodic", "far_field", "no_slip_wall"])
+def test_mfma_vjp_matches_lane_and_fp64(p, case):
+    # the matrix-core adjoint against the per-lane FP32 adjoint and the FP64 adjoint, for the
+    # residual (unfolded tables) and the right-hand side (folded tables)
+    ora, mf, ln, c64, u0 = _adj_setup(p, case)
+    lam = np.random.default_rng(p).standard_normal(u0.shape)
+    for f in ("residual_vjp", "rhs_vjp"):
+        g64 = getattr(c64, f)(lam, u0)
+        e_mf = _rel(getattr(mf, f)(lam, u0), g64)
+        e_ln = _rel(getattr(ln, f)(lam, u0), g64)
+        assert e_mf < 5e-5 and e_ln < 5e-5, (f, e_mf, e_ln)
+        assert e_mf < 2 * e_ln + 1e-6, (f, e_mf, e_ln)
+
+
+@pytest.mark.parametrize("p", [3, 4])
+def test_mfma_rk3_vjp_matches_lane(p):
+    ora, mf, ln, u0 = _setup(p, "periodic", n=7)
+    dt = dgx.stable_dt(ln, 0.3)
+    lam_T = np.random.default_rng(3).standard_normal(u0.shape)
+    g = []
+    for c in (mf, ln):
+        c.set_state(u0)
+        g.append(c.rk3_vjp(dt, 6, 3, lam_T))
+    assert np.linalg.norm(g[0] - g[1]) / np.linalg.norm(g[1]) < 1e-5
+
+
+def test_vjp_no_slip_ghost_against_oracle_fd():
+    # FP64 adjoint on a bounded mesh with far-field and no-slip faces, by central differences
+    # of the oracle residual: the no-slip ghost state is built from primitive values
+    verts, tris = R.split_quad_mesh(4, 1.0)
+    ora = R.Oracle(verts, tris, 2, basis="dubiner", periodic=0)
+    for tag, (typ, val) in ADJ_MIXED.items():
+        ora.set_bc(tag, typ, val)
+    ctx = dgx.Context(verts, tris, 2, basis="dubiner", precision="fp64", periodic=0, bcs=ADJ_MIXED)
+    xy = ora.volume_points()
+    U = R.vortex_state(xy, xc=0.43, yc=0.57, beta=1.0, length=1.0)
+    rho, u, v = U[..., 0], U[..., 1] / U[..., 0], U[..., 2] / U[..., 0] + 0.3
+    pr = 0.4 * (U[..., 3] - 0.5 * (U[..., 1] ** 2 + U[..., 2] ** 2) / rho)
+    u0 = ora.project(np.stack([rho, rho * u, rho * v, pr / 0.4 + 0.5 * rho * (u * u + v * v)], axis=-1))
+    lam = np.random.default_rng(11).standard_normal(u0.shape)
+    g = ctx.residual_vjp(lam, u0)
+    ok = 0
+    for d in [np.random.default_rng(20 + k).standard_normal(u0.shape) for k in range(4)]:
+        errs = []
+        for h in (1e-6, 1e-7):
+            fd = np.sum((ora.residual(u0 + h * d) - ora.residual(u0 - h * d)) / (2 * h) * lam)
+            errs.append(abs(np.sum(g * d) - fd) / abs(fd))
+        ok += min(errs) < 1e-6
+    assert ok >= 3
