@@ -692,7 +692,7 @@ struct AdjointBuffers {
 
 // Reverse sweep of nsteps SSP-RK3 steps from the state in c->u with checkpoints every
 // `every` steps; `lam` holds lambda_T on entry (device, in b.l0) and lambda_0 on exit.
-void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every) {
+void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every, hipEvent_t mid = nullptr) {
     const size_t sb = c->state_bytes();
     const int nck = (nsteps + every - 1) / every;
     if (b.ckpt.size() < (size_t)nck) b.ckpt.resize(nck);
@@ -705,6 +705,7 @@ void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every
             hip_check(hipMemcpyAsync(b.ckpt[s / every].p, c->u.p, sb, hipMemcpyDeviceToDevice, c->stream), "ckpt");
         c->rk3_step(dt);
     }
+    if (mid) hip_check(hipEventRecord(mid, c->stream), "record");
     // reverse sweep, segment by segment
     for (int j = nck - 1; j >= 0; --j) {
         const int s0 = j * every, len = std::min(every, nsteps - s0);
@@ -803,7 +804,8 @@ dgx_status dgx_rk3_vjp(dgx_ctx* c, double dt, int32_t nsteps, int32_t every, con
     });
 }
 
-dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t every, double* fwd_ms, double* adj_ms) {
+dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t every, double* fwd_ms,
+                               double* ckpt_fwd_ms, double* rev_ms) {
     return guarded([&] {
         adjoint_supported(c);
         if (nsteps <= 0 || every <= 0) throw std::invalid_argument("bad adjoint arguments");
@@ -828,16 +830,21 @@ dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t ev
         hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
                                  c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->uref, c->stream), "ke grad");
         hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        hipEvent_t mid = nullptr;
+        hip_check(hipEventCreate(&mid), "hipEventCreate");
         hip_check(hipEventRecord(c->ev0, c->stream), "record");
-        rk3_adjoint(c, b, dt, nsteps, every);
+        rk3_adjoint(c, b, dt, nsteps, every, mid);
         hip_check(hipEventRecord(c->ev1, c->stream), "record");
         hip_check(hipStreamSynchronize(c->stream), "sync");
-        float a = 0;
-        hip_check(hipEventElapsedTime(&a, c->ev0, c->ev1), "elapsed");
+        float a = 0, r = 0;
+        hip_check(hipEventElapsedTime(&a, c->ev0, mid), "elapsed");
+        hip_check(hipEventElapsedTime(&r, mid, c->ev1), "elapsed");
+        (void)hipEventDestroy(mid);
         hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
         if (fwd_ms) *fwd_ms = f;
-        if (adj_ms) *adj_ms = a;
+        if (ckpt_fwd_ms) *ckpt_fwd_ms = a;
+        if (rev_ms) *rev_ms = r;
     });
 }
 

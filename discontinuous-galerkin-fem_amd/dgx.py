@@ -115,7 +115,7 @@ def load():
     L.dgx_rhs_vjp.argtypes = [vp, dp, dp, dp]
     L.dgx_kinetic_energy_grad.argtypes = [vp, dp, dp]
     L.dgx_rk3_vjp.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, dp, dp]
-    L.dgx_rk3_vjp_profile.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, dp, dp]
+    L.dgx_rk3_vjp_profile.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, dp, dp, dp]
     L.dgx_batch_create.argtypes = [C.c_int32, C.POINTER(MeshDesc), C.POINTER(SpaceDesc), C.POINTER(Physics),
                                    C.c_int32, C.POINTER(vp)]
     L.dgx_batch_destroy.argtypes = [vp]
@@ -358,9 +358,10 @@ class Context:
         return out
 
     def rk3_vjp_profile(self, dt, nsteps, ckpt_every):
-        f, a = C.c_double(), C.c_double()
-        _check(load().dgx_rk3_vjp_profile(self._h, dt, nsteps, ckpt_every, C.byref(f), C.byref(a)))
-        return f.value, a.value
+        """(forward ms, checkpointed forward sweep ms, reverse sweep ms) of an nsteps run."""
+        f, a, r = C.c_double(), C.c_double(), C.c_double()
+        _check(load().dgx_rk3_vjp_profile(self._h, dt, nsteps, ckpt_every, C.byref(f), C.byref(a), C.byref(r)))
+        return f.value, a.value, r.value
 
 
 class _Borrowed(Context):

@@ -194,11 +194,12 @@ dgx_status dgx_kinetic_energy_grad(dgx_ctx* ctx, double* J, double* grad);
  * return the context holds u_T.  lambda_T NULL seeds with dJ/du_T of the kinetic energy. */
 dgx_status dgx_rk3_vjp(dgx_ctx* ctx, double dt, int32_t nsteps, int32_t ckpt_every,
                        const double* lambda_T, double* lambda_0);
-/* Times a plain forward run of nsteps (fwd_ms) and dgx_rk3_vjp of the same run seeded with the
- * kinetic-energy gradient (adj_ms, recompute included), from the current state, on the context
- * stream.  The state is restored to the starting state afterwards. */
+/* Times, from the current state on the context stream, a plain forward run of nsteps (fwd_ms)
+ * and the two phases of dgx_rk3_vjp of the same run seeded with the kinetic-energy gradient:
+ * the forward sweep that stores the checkpoints (ckpt_fwd_ms) and the reverse sweep, i.e. the
+ * per-segment recompute and the stage adjoints (rev_ms).  The state is restored afterwards. */
 dgx_status dgx_rk3_vjp_profile(dgx_ctx* ctx, double dt, int32_t nsteps, int32_t ckpt_every,
-                               double* fwd_ms, double* adj_ms);
+                               double* fwd_ms, double* ckpt_fwd_ms, double* rev_ms);
 
 /* ---- batched instances ---------------------------------------------------------------------
  * No counterpart in the reference (one solver per process there).  Many independent Euler

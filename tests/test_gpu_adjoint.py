@@ -181,8 +181,8 @@ def test_adjoint_profile_runs_and_restores_state():
     ctx.set_state(u0)
     before = ctx.get_state()
     dt = dgx.stable_dt(ctx, 0.3)
-    f, a = ctx.rk3_vjp_profile(dt, 20, 5)
-    assert f > 0 and a > 0
+    f, a, r = ctx.rk3_vjp_profile(dt, 20, 5)
+    assert f > 0 and a > 0 and r > 0
     assert np.array_equal(ctx.get_state(), before)
 
 
