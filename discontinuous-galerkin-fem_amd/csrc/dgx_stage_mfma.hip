@@ -30,8 +30,9 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     using S = MfmaShape<P>;
     constexpr int NP = S::NP, NFQ = S::NFQ, R = S::R, RQ = S::RQ, RF = S::RF;
     constexpr int EB = kBlock / 4;  // elements per block: 4 waves x 16
-    __shared__ float s_nt[S::NT];
-    __shared__ float4 s_tr[3 * NFQ * EB];  // neighbour traces [face][point][element]
+    __shared__ float s_tab[S::TOTAL];      // lane tables, then the neighbour-trace table
+    __shared__ double s_geo[kGeoRows][EB];  // FP64 geometry rows of the block's elements
+    __shared__ float4 s_tr[3 * NFQ * EB];   // neighbour traces [face][point][element]
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
@@ -52,13 +53,15 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     // 32-bit offset, the variable or geometry row a scalar offset) and every load is issued up
     // front in dependency order, so a wave waits for memory once.  A padding mode slot gets a
     // per-lane offset past the end of its resource, which the range check turns into a zero
-    // read or a dropped write.
-    constexpr int kNtPerThread = (S::NT + kBlock - 1) / kBlock;
-    float ntv[kNtPerThread];
+    // read or a dropped write.  The tables and the block's geometry are loaded cooperatively
+    // (one contiguous geometry row of 64 elements per wave instruction) and staged in LDS, so
+    // no lane holds them in registers while the loads are in flight.
+    constexpr int kTabPerThread = (S::TOTAL + kBlock - 1) / kBlock;
+    float tabv[kTabPerThread];
 #pragma unroll
-    for (int k = 0; k < kNtPerThread; ++k) {
+    for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        ntv[k] = t < S::NT ? tm[S::T_N * 64 + t] : 0.0f;
+        tabv[k] = t < S::TOTAL ? tm[t] : 0.0f;
     }
     const unsigned state_bytes = unsigned(NP) * 16u * uKp;
     const auto r_uk = make_rsrc(sp.uk, state_bytes);
@@ -88,35 +91,14 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
     }
-    // this lane's entries of the product tables (A operands)
-    float tEV[R], tEF[R], tPX[RQ], tPY[RQ], tLF[RF];
+    // geometry rows of the block: thread t loads row t / 64 + 4k of element t % 64
+    constexpr int kGeoPerThread = (kGeoRows * EB + kBlock - 1) / kBlock;
+    double geov[kGeoPerThread];
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-        tEV[s] = tm[(S::T_EV + s) * 64 + lane];
-        tEF[s] = tm[(S::T_EF + s) * 64 + lane];
-    }
-#pragma unroll
-    for (int t = 0; t < RQ; ++t) {
-        tPX[t] = tm[(S::T_PX + t) * 64 + lane];
-        tPY[t] = tm[(S::T_PY + t) * 64 + lane];
-    }
-#pragma unroll
-    for (int t = 0; t < RF; ++t) tLF[t] = tm[(S::T_LF + t) * 64 + lane];
-    const unsigned go = unsigned(ec) * 8u;
-    double gd[kGeoRows];  // raw FP64 geometry of the element
-#pragma unroll
-    for (int k = 0; k < kGeoRows; ++k) gd[k] = load_f64(r_geo, go, unsigned(k) * 8u * uKp);
-    // u^n for the second and third RK stages
-    const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
-    float unv[4][R];
-    if (rk_blend) {
-        const auto r_un = make_rsrc(sp.un, state_bytes);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const v4u x = load_x4(r_un, slot_off(r, unsigned(ec)));
-#pragma unroll
-            for (int v = 0; v < 4; ++v) unv[v][r] = __uint_as_float(x[v]);
-        }
+    for (int k = 0; k < kGeoPerThread; ++k) {
+        const int t = int(threadIdx.x) + k * kBlock;
+        const int row = t / EB, el = min(eb0 + t % EB, sp.K - 1);
+        geov[k] = t < kGeoRows * EB ? load_f64(r_geo, unsigned(el) * 8u, unsigned(row) * 8u * uKp) : 0.0;
     }
     // neighbour coefficients for the trace role (all modes of one neighbour per lane)
     const bool t_interior = w < 3 && (ct & 3u) != kFaceBoundary;
@@ -130,9 +112,14 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
         }
     }
 #pragma unroll
-    for (int k = 0; k < kNtPerThread; ++k) {
+    for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        if (t < S::NT) s_nt[t] = ntv[k];
+        if (t < S::TOTAL) s_tab[t] = tabv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kGeoPerThread; ++k) {
+        const int t = int(threadIdx.x) + k * kBlock;
+        if (t < kGeoRows * EB) s_geo[t / EB][t % EB] = geov[k];
     }
     __syncthreads();
     // keep the scheduler from sinking the loads above into the arithmetic below
@@ -141,7 +128,7 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     // ---------------- neighbour traces (waves 0-2) ------------------------------------------
     if (t_interior) {
         const int fn = int(ct & 3u), flip = int((ct >> 2) & 1u);
-        const float* T = &s_nt[(fn * 2 + flip) * NP * NFQ];
+        const float* T = &s_tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
         float Un[NFQ][4];
 #pragma unroll
         for (int q = 0; q < NFQ; ++q)
@@ -160,6 +147,26 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
             s_tr[(w * NFQ + q) * EB + lane] = make_float4(Un[q][0], Un[q][1], Un[q][2], Un[q][3]);
     }
 
+    const int eloc = 16 * w + j;
+    double gd[kGeoRows];  // FP64 geometry of this lane's element
+#pragma unroll
+    for (int k = 0; k < kGeoRows; ++k) gd[k] = s_geo[k][eloc];
+    // this lane's entries of the product tables (A operands)
+    const float* lt = &s_tab[lane];
+    float tEV[R], tEF[R], tPX[RQ], tPY[RQ], tLF[RF];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        tEV[s] = lt[(S::T_EV + s) * 64];
+        tEF[s] = lt[(S::T_EF + s) * 64];
+    }
+#pragma unroll
+    for (int t = 0; t < RQ; ++t) {
+        tPX[t] = lt[(S::T_PX + t) * 64];
+        tPY[t] = lt[(S::T_PY + t) * 64];
+    }
+#pragma unroll
+    for (int t = 0; t < RF; ++t) tLF[t] = lt[(S::T_LF + t) * 64];
+
     const double det = gd[G_DET];
     const double rdet = 1.0 / det;
     // volume terms carry |J| only for M dU/dt (with M^-1 = M_ref^-1/|J| folded they cancel);
@@ -175,81 +182,111 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
         fsf[f] = residual_mode ? float(gd[G_L0 + f]) : float(gd[G_L0 + f] * rdet);
     }
 
-    // ---------------- volume: evaluation, contravariant fluxes, projection -------------------
-    v4f acc[4];
-    {
-        v4f U[4];
+    // ---------------- evaluations at the volume and face slots (matrix cores) ---------------
+    v4f U[4], Uf[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int v = 0; v < 4; ++v) {
+        v4f c = {0.0f, 0.0f, 0.0f, 0.0f}, d = c;
 #pragma unroll
-            for (int s = 0; s < R; ++s) c = mfma16x16x4(tEV[s], b[v][s], c);
-            U[v] = c;
+        for (int s = 0; s < R; ++s) {
+            c = mfma16x16x4(tEV[s], b[v][s], c);
+            d = mfma16x16x4(tEF[s], b[v][s], d);
         }
-        v4f Fh[4], Gh[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            if (r >= RQ) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) { Fh[v][r] = 0.0f; Gh[v][r] = 0.0f; }
-                continue;
-            }
-            const float U0 = U[0][r] + ur[0], U1 = U[1][r] + ur[1], U2 = U[2][r] + ur[2], U3 = U[3][r] + ur[3];
-            const float ri = rcp(U0);
-            const float uu = U1 * ri, vv = U2 * ri;
-            const float p = (gam - 1.0f) * (U3 - 0.5f * U0 * (uu * uu + vv * vv));
-            // contravariant velocities along the rows of dxi_dx^T
-            const float c1 = uu * a00 + vv * a10, c2 = uu * a01 + vv * a11;
-            Fh[0][r] = U0 * c1;            Gh[0][r] = U0 * c2;
-            Fh[1][r] = U1 * c1 + p * a00;  Gh[1][r] = U1 * c2 + p * a01;
-            Fh[2][r] = U2 * c1 + p * a10;  Gh[2][r] = U2 * c2 + p * a11;
-            Fh[3][r] = (U3 + p) * c1;      Gh[3][r] = (U3 + p) * c2;
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int t = 0; t < RQ; ++t) {
-                c = mfma16x16x4(tPX[t], Fh[v][t], c);
-                c = mfma16x16x4(tPY[t], Gh[v][t], c);
-            }
-            acc[v] = c;
-        }
+        U[v] = c;
+        Uf[v] = d;
     }
 
-    // ---------------- faces ----------------------------------------------------------------
-    v4f Uf[4];  // own traces at the face slots
+    // ---------------- contravariant volume fluxes -----------------------------------------
+    v4f Fh[4], Gh[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (r >= RQ) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { Fh[v][r] = 0.0f; Gh[v][r] = 0.0f; }
+            continue;
+        }
+        const float U0 = U[0][r] + ur[0], U1 = U[1][r] + ur[1], U2 = U[2][r] + ur[2], U3 = U[3][r] + ur[3];
+        const float ri = rcp(U0);
+        const float uu = U1 * ri, vv = U2 * ri;
+        const float p = (gam - 1.0f) * (U3 - 0.5f * U0 * (uu * uu + vv * vv));
+        // contravariant velocities along the rows of dxi_dx^T
+        const float c1 = uu * a00 + vv * a10, c2 = uu * a01 + vv * a11;
+        Fh[0][r] = U0 * c1;            Gh[0][r] = U0 * c2;
+        Fh[1][r] = U1 * c1 + p * a00;  Gh[1][r] = U1 * c2 + p * a01;
+        Fh[2][r] = U2 * c1 + p * a10;  Gh[2][r] = U2 * c2 + p * a11;
+        Fh[3][r] = (U3 + p) * c1;      Gh[3][r] = (U3 + p) * c2;
+    }
+    __syncthreads();  // neighbour traces complete
+
+    // The projection below and the face fluxes after it are independent, and both sit in one
+    // basic block, so the scheduler can issue the face arithmetic while the matrix cores work.
+    v4f acc[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < R; ++s) c = mfma16x16x4(tEF[s], b[v][s], c);
-        Uf[v] = c;
+        for (int t = 0; t < RQ; ++t) {
+            c = mfma16x16x4(tPX[t], Fh[v][t], c);
+            c = mfma16x16x4(tPY[t], Gh[v][t], c);
+        }
+        acc[v] = c;
     }
-    __syncthreads();  // neighbour traces complete
+
+    // ---------------- faces: slot (g, f) = face f, point g (f < 3); face g, point 4 (f = 3) ---
+    // Interior faces first, without branches (a boundary slot computes a value from stale LDS
+    // that the pass after it replaces).
     v4f Hs[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Hs[v] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    const int eloc = 16 * w + j;
+    bool any_boundary = false;
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-        // slot (g, f): face f, point g for f < 3; face g, point 4 for f = 3 (p = 4)
-        const int face = f < 3 ? f : g;
-        const int q = f < 3 ? g : 4;
-        if (f == 3 && (NFQ < 5 || g >= 3)) continue;
-        if (f < 3 && g >= NFQ) continue;
+        const bool slot_valid = f < 3 ? g < NFQ : (NFQ == 5 && g < 3);
+        const int face = f < 3 ? f : (g < 3 ? g : 0);
+        const int q = f < 3 ? (g < NFQ ? g : 0) : 4;
+        if (f == 3 && NFQ < 5) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Hs[v][f] = 0.0f;
+            continue;
+        }
         const uint32_t c = (code >> (4 * face)) & 15u;
-        const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
+        const bool isL = (c >> 3) & 1u;
+        any_boundary |= slot_valid && (c & 3u) == kFaceBoundary;
         const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
         const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
         const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
         const float4 tn = s_tr[(face * NFQ + q) * EB + eloc];
-        float Un[4] = {tn.x, tn.y, tn.z, tn.w};
+        const float Un[4] = {tn.x, tn.y, tn.z, tn.w};
         const float Uo[4] = {Uf[0][f] + ur[0], Uf[1][f] + ur[1], Uf[2][f] + ur[2], Uf[3][f] + ur[3]};
-        float H[4];
-        face_point_flux(Uo, Un, c, nb, nx, ny, sf, sp, gam, H);
+        // canonical orientation: A side = the L element, normal = the L-side outward normal
+        const float sgn = isL ? 1.0f : -1.0f;
+        float Aq[4], Bq[4], H[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Hs[v][f] = H[v];
+        for (int v = 0; v < 4; ++v) {
+            Aq[v] = isL ? Uo[v] : Un[v];
+            Bq[v] = isL ? Un[v] : Uo[v];
+        }
+        rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Hs[v][f] = slot_valid ? sgn * sf * H[v] : 0.0f;
+    }
+    // boundary face slots: ghost state from the boundary condition (only waves that have one)
+    if (__builtin_amdgcn_ballot_w64(any_boundary) != 0) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            if (f == 3 && NFQ < 5) continue;
+            const bool slot_valid = f < 3 ? g < NFQ : (NFQ == 5 && g < 3);
+            const int face = f < 3 ? f : (g < 3 ? g : 0);
+            const uint32_t c = (code >> (4 * face)) & 15u;
+            if (!slot_valid || (c & 3u) != kFaceBoundary) continue;
+            const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
+            const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
+            const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
+            const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
+            const float Uo[4] = {Uf[0][f] + ur[0], Uf[1][f] + ur[1], Uf[2][f] + ur[2], Uf[3][f] + ur[3]};
+            float Un[4], H[4];
+            face_point_flux(Uo, Un, c, nb, nx, ny, sf, sp, gam, H);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Hs[v][f] = H[v];
+        }
     }
 
     // lift: acc += LF^T H (LF carries -w_q/2 and, for the RK/RHS tables, M_ref^-1)
@@ -262,6 +299,18 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     if (!valid) return;
     const auto r_out = make_rsrc(sp.out, state_bytes);
     const float dt = float(sp.dt), ca = float(sp.ca), cb = float(sp.cb);
+    // u^n for the second and third RK stages
+    const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
+    float unv[4][R];
+    if (rk_blend) {
+        const auto r_un = make_rsrc(sp.un, state_bytes);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const v4u x = load_x4(r_un, slot_off(r, unsigned(e)));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) unv[v][r] = __uint_as_float(x[v]);
+        }
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         v4u x;
