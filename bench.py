@@ -18,6 +18,8 @@ Output: one JSON line (rank 0).  Besides the contract keys it carries
                   a bounded sample of the same mesh
   fp64          : the same workload in FP64 (DOF-stage/s)
   l2_err_fp32_vs_fp64 : per-field relative L2 at volume points after the timed steps
+  bf16_volume_variant : the same run with bfloat16 operands in the volume products (FP32
+                  accumulation and state): DOF-stage/s and per-field L2 vs FP64
 """
 import argparse
 import json
@@ -163,6 +165,18 @@ def main():
         result["fp64"] = {"value": dof_stage / secs64, "ms_per_step": secs64 / args.steps * 1e3}
         err = l2_rel(Uq32, Uq64)
         result["l2_err_fp32_vs_fp64"] = dict(zip(["rho", "rhou", "rhov", "E"], err))
+        # variant: bfloat16 operands in the volume products, FP32 accumulation and state
+        if p in (3, 4):
+            ctxb, _, _, _ = make_ctx(n, p, "fp32_bf16vol", SEED)
+            secsb = timed_steps(ctxb, dt, args.steps, args.warmup)
+            Uqb = ctxb.eval_volume()
+            ctxb.set_state(u0)
+            stage_b, _ = ctxb.rk3_profile(dt, min(args.steps, 50))
+            result["bf16_volume_variant"] = {
+                "value": dof_stage / secsb, "ms_per_step": secsb / args.steps * 1e3, "launch_ms": stage_b,
+                "kernel": "stage_mfma_bf16_kernel_p%d" % p,
+                "l2_err_vs_fp64": dict(zip(["rho", "rhou", "rhov", "E"], l2_rel(Uqb, Uq64)))}
+            ctxb.close()
         ctx64.close()
 
     # analytic oracle on the unperturbed twin (periodic-truncation floor ~1e-5)

@@ -56,10 +56,12 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
     if (s.order < 1 || s.order > 4) throw std::invalid_argument("Element order must be between 1 and 4");
     if (s.basis != DGX_BASIS_MONOMIAL && s.basis != DGX_BASIS_DUBINER) throw std::invalid_argument("unknown basis");
     if (s.rule != DGX_RULE_2P && s.rule != DGX_RULE_2P1) throw std::invalid_argument("unknown volume rule");
-    if (precision != DGX_FP64 && precision != DGX_FP32) {
-        if (precision == DGX_FP32_BF16VOL) throw Fail(DGX_E_UNSUPPORTED, "the BF16-input volume variant is not built");
+    if (precision != DGX_FP64 && precision != DGX_FP32 && precision != DGX_FP32_BF16VOL)
         throw std::invalid_argument("unknown precision");
-    }
+    // the bfloat16 volume products exist in the matrix-core Euler kernel only
+    if (precision == DGX_FP32_BF16VOL &&
+        (ph.kind != DGX_EULER || s.rule != DGX_RULE_2P || !mfma_order(s.order)))
+        throw Fail(DGX_E_UNSUPPORTED, "the BF16-input volume variant covers Euler, p = 3 or 4, volume rule 2p");
     if (ph.kind != DGX_EULER && ph.kind != DGX_NAVIER_STOKES) throw std::invalid_argument("unknown physics kind");
     if (!(ph.gamma > 1.0)) throw std::invalid_argument("gamma must exceed 1");
     if (ph.kind == DGX_NAVIER_STOKES) {  // NavierStokesWeakFormulation ctor (:18-34)
@@ -319,15 +321,16 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         c.upload_real<float>(c.tabL, tL);
         c.upload_real<float>(c.tabR, tR);
     }
-    // matrix-core stage kernel: FP32 Euler, degree-2p volume rule, p = 2..4
-    // (DGX_STAGE_KERNEL=lane keeps the one-element-per-lane kernel)
+    // matrix-core stage kernel: FP32 Euler, degree-2p volume rule, p = 3, 4
+    // (DGX_STAGE_KERNEL=lane keeps the one-element-per-lane kernel for FP32)
     const char* kv = std::getenv("DGX_STAGE_KERNEL");
-    const bool lane_only = kv && std::string(kv) == "lane";
-    if (precision == DGX_FP32 && c.physics == DGX_EULER && c.rule == 0 && mfma_order(c.p) && !lane_only) {
+    const bool lane_only = kv && std::string(kv) == "lane" && precision == DGX_FP32;
+    if (precision != DGX_FP64 && c.physics == DGX_EULER && c.rule == 0 && mfma_order(c.p) && !lane_only) {
+        const bool bf16 = precision == DGX_FP32_BF16VOL;
         std::vector<float> m;
-        build_mfma_tables(c.p, tL, m);
+        build_mfma_tables(c.p, tL, m, bf16);
         c.upload(c.tabML, m.data(), m.size() * sizeof(float));
-        build_mfma_tables(c.p, tR, m);
+        build_mfma_tables(c.p, tR, m, bf16);
         c.upload(c.tabMR, m.data(), m.size() * sizeof(float));
         build_mfma_vjp_tables(c.p, tL, m);
         c.upload(c.tabVL, m.data(), m.size() * sizeof(float));
@@ -659,6 +662,8 @@ void adjoint_supported(const dgx_ctx* c) {
     if (!c) throw std::invalid_argument("null context");
     if (c->physics != DGX_EULER) throw Fail(DGX_E_UNSUPPORTED, "the adjoint is built for the Euler residual only");
     if (c->has_slip) throw Fail(DGX_E_UNSUPPORTED, "the adjoint does not differentiate slip-wall ghost states");
+    if (c->precision == DGX_FP32_BF16VOL)
+        throw Fail(DGX_E_UNSUPPORTED, "the adjoint is built for the FP32 and FP64 operators");
 }
 
 // host [K][Np][4] -> device SoA in `dst`; states carry the storage offset, adjoints do not

@@ -191,6 +191,8 @@ dgx_status dgx_batch_create(int32_t n, const dgx_mesh_desc* meshes, const dgx_sp
     dgx_status st = guarded([&] {
         if (n <= 0 || !meshes || !spaces || !physics) throw std::invalid_argument("bad batch description");
         if (physics->kind != DGX_EULER) throw Fail(DGX_E_UNSUPPORTED, "batched instances run the Euler residual");
+        if (precision == DGX_FP32_BF16VOL)
+            throw Fail(DGX_E_UNSUPPORTED, "batched instances run the FP32 and FP64 operators");
         for (int i = 0; i < n; ++i) {
             if (spaces[i].rule != DGX_RULE_2P) throw Fail(DGX_E_UNSUPPORTED, "batched instances use the degree-2p rule");
             dgx_ctx* c = nullptr;

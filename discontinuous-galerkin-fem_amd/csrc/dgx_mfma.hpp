@@ -39,6 +39,14 @@ struct MfmaShape {
     static constexpr int T_N = T_LF + RF;
     static constexpr int NT = 6 * NP * NFQ;
     static constexpr int TOTAL = T_N * 64 + NT;
+    // bfloat16 operands of the volume products (v_mfma_f32_16x16x16_bf16: lane i + 16 g holds
+    // A[i][4g .. 4g+3]), two 32-bit words of packed pairs per lane and table, placed after the
+    // FP32 tables: EB (Phi_V, k = mode slot (g, t)), XB / YB (PX / PY, rows = mode slots,
+    // k = volume slot (g, t))
+    static constexpr int T_EB = TOTAL;
+    static constexpr int T_XB = T_EB + 2 * 64;
+    static constexpr int T_YB = T_XB + 2 * 64;
+    static constexpr int TOTAL_BF = T_YB + 2 * 64;
 };
 
 // Lane tables of the matrix-core stage adjoint (dgx_vjp_mfma.hip), 64 floats each:
@@ -83,7 +91,8 @@ __host__ __device__ constexpr int mf_fpoint(int nfq, int g, int r) {
 constexpr bool mfma_order(int p) { return p == 3 || p == 4; }
 int mfma_table_size(int p);  // floats
 // Build the lane-major tables of order p from the (L or R) tables in TabLayout order.
-void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out);
+// With bf16 set, the bfloat16 volume tables follow (TOTAL_BF floats in all).
+void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out, bool bf16 = false);
 void build_mfma_vjp_tables(int p, const std::vector<double>& tab, std::vector<float>& out);
 
 }  // namespace dgx

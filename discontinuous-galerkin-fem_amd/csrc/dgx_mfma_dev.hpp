@@ -13,6 +13,20 @@ __device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// bfloat16 product with FP32 accumulation, K = 16: lane i + 16 g supplies A[i][4g .. 4g+3] and
+// B[4g .. 4g+3][i] as two 32-bit words of packed pairs (low half first).
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v4f mfma16x16x16bf16(unsigned a0, unsigned a1, unsigned b0, unsigned b1, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(v4s, v2u{a0, a1}),
+                                                      __builtin_bit_cast(v4s, v2u{b0, b1}), c, 0, 0, 0);
+}
+// two floats rounded to bfloat16 (nearest even, v_cvt_pk_bf16_f32) and packed
+__device__ __forceinline__ unsigned pack_bf16x2(float lo, float hi) {
+    return unsigned(__builtin_bit_cast(unsigned short, (__bf16)lo)) |
+           (unsigned(__builtin_bit_cast(unsigned short, (__bf16)hi)) << 16);
+}
+
 // Ghost state of a boundary face point (boundary_face_residual, euler_weak_formulation.cpp:
 // 208-233); a face without a condition (type < 0) mirrors the interior state.
 __device__ __forceinline__ void ghost_state(const float* Uo, float nx, float ny, const double* bcr, float gam,

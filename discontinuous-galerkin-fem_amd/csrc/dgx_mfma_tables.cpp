@@ -1,20 +1,57 @@
 // Host-side builder of the MFMA stage kernel's lane-major tables (layout in dgx_mfma.hpp).
 #include "dgx_mfma.hpp"
 
+#include <cstdint>
+#include <cstring>
 #include <stdexcept>
 
 namespace dgx {
 
 namespace {
 
+// Round to bfloat16, nearest-even (finite inputs).
+uint16_t to_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+// Two bfloat16 values packed in one 32-bit word (low half first), stored in a float slot.
+float pack_bf16(float lo, float hi) {
+    const uint32_t w = uint32_t(to_bf16(lo)) | (uint32_t(to_bf16(hi)) << 16);
+    float f;
+    std::memcpy(&f, &w, 4);
+    return f;
+}
+
 template <int P>
-void build(const std::vector<double>& t, std::vector<float>& out) {
+void build(const std::vector<double>& t, std::vector<float>& out, bool bf16) {
     using S = MfmaShape<P>;
     const TabLayout L(P, 0);
     const int np = S::NP, nq = S::NQ, nfq = S::NFQ;
     if ((int)t.size() != L.total) throw std::runtime_error("table size mismatch");
-    out.assign(S::TOTAL, 0.0f);
+    out.assign(bf16 ? S::TOTAL_BF : S::TOTAL, 0.0f);
     auto at = [&](int table, int lane) -> float& { return out[(size_t)table * 64 + lane]; };
+    for (int lane = 0; bf16 && lane < 64; ++lane) {
+        const int i = lane % 16, g = lane / 16;
+        int vp = mf_vpoint(S::RQ, nq, i / 4, i % 4);
+        if (vp < 0) vp = 0;
+        const int mrow = mf_mode(S::R, np, i / 4, i % 4);
+        float e4[4], x4[4], y4[4];
+        for (int tt = 0; tt < 4; ++tt) {
+            const int m = mf_mode(S::R, np, g, tt);     // k = 4g + tt: mode slot (g, tt)
+            const int q = mf_vpoint(S::RQ, nq, g, tt);  // or volume slot (g, tt)
+            e4[tt] = m < 0 ? 0.0f : float(t[L.phiV + vp * np + m]);
+            x4[tt] = (mrow < 0 || q < 0) ? 0.0f : float(t[L.PX + q * np + mrow]);
+            y4[tt] = (mrow < 0 || q < 0) ? 0.0f : float(t[L.PY + q * np + mrow]);
+        }
+        for (int h = 0; h < 2; ++h) {
+            out[(size_t)S::T_EB + h * 64 + lane] = pack_bf16(e4[2 * h], e4[2 * h + 1]);
+            out[(size_t)S::T_XB + h * 64 + lane] = pack_bf16(x4[2 * h], x4[2 * h + 1]);
+            out[(size_t)S::T_YB + h * 64 + lane] = pack_bf16(y4[2 * h], y4[2 * h + 1]);
+        }
+    }
     for (int lane = 0; lane < 64; ++lane) {
         const int i = lane % 16, k = lane / 16;
         // rows of an evaluation product are point slots; padding rows repeat point 0 so that
@@ -117,11 +154,11 @@ int mfma_table_size(int p) {
     }
 }
 
-void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out) {
+void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out, bool bf16) {
     switch (p) {
-    case 2: build<2>(tab, out); break;
-    case 3: build<3>(tab, out); break;
-    case 4: build<4>(tab, out); break;
+    case 2: build<2>(tab, out, bf16); break;
+    case 3: build<3>(tab, out, bf16); break;
+    case 4: build<4>(tab, out, bf16); break;
     default: throw std::runtime_error("no MFMA tables for this order");
     }
 }

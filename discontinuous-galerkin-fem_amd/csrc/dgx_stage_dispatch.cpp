@@ -8,14 +8,14 @@ hipError_t launch_stage_f64_euler(int p, int rule, bool quirk, const StageParams
 hipError_t launch_stage_f32_ns(int p, int rule, bool quirk, const StageParams& sp, hipStream_t s);
 hipError_t launch_stage_f64_ns(int p, int rule, bool quirk, const StageParams& sp, hipStream_t s);
 
-hipError_t launch_stage_mfma(int p, const StageParams& sp, hipStream_t s);
+hipError_t launch_stage_mfma(int p, bool bf16vol, const StageParams& sp, hipStream_t s);
 
 hipError_t launch_stage(int precision, int physics, int p, int rule, const StageParams& sp, hipStream_t s) {
     if (sp.K <= 0) return hipSuccess;
     // FP32 Euler with the degree-2p volume rule runs on the matrix cores when its lane tables
     // were built for the context
-    if (sp.tabm && precision == DGX_FP32 && physics == DGX_EULER && rule == 0 && mfma_order(p))
-        return launch_stage_mfma(p, sp, s);
+    if (sp.tabm && precision != DGX_FP64 && physics == DGX_EULER && rule == 0 && mfma_order(p))
+        return launch_stage_mfma(p, precision == DGX_FP32_BF16VOL, sp, s);
     const bool quirk = sp.ns_face_quirk != 0;
     const bool f64 = precision == DGX_FP64;
     if (physics == DGX_EULER)

@@ -25,12 +25,13 @@
 namespace dgx {
 namespace {
 
-template <int P>
+template <int P, bool BF16>
 __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     using S = MfmaShape<P>;
+    constexpr int NTAB = BF16 ? S::TOTAL_BF : S::TOTAL;
     constexpr int NP = S::NP, NFQ = S::NFQ, R = S::R, RQ = S::RQ, RF = S::RF;
     constexpr int EB = kBlock / 4;  // elements per block: 4 waves x 16
-    __shared__ float s_tab[S::TOTAL];      // lane tables, then the neighbour-trace table
+    __shared__ float s_tab[NTAB];          // lane tables, neighbour-trace table, bf16 tables
     __shared__ double s_geo[kGeoRows][EB];  // FP64 geometry rows of the block's elements
     __shared__ float4 s_tr[3 * NFQ * EB];   // neighbour traces [face][point][element]
 
@@ -56,12 +57,12 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     // read or a dropped write.  The tables and the block's geometry are loaded cooperatively
     // (one contiguous geometry row of 64 elements per wave instruction) and staged in LDS, so
     // no lane holds them in registers while the loads are in flight.
-    constexpr int kTabPerThread = (S::TOTAL + kBlock - 1) / kBlock;
+    constexpr int kTabPerThread = (NTAB + kBlock - 1) / kBlock;
     float tabv[kTabPerThread];
 #pragma unroll
     for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        tabv[k] = t < S::TOTAL ? tm[t] : 0.0f;
+        tabv[k] = t < NTAB ? tm[t] : 0.0f;
     }
     const unsigned state_bytes = unsigned(NP) * 16u * uKp;
     const auto r_uk = make_rsrc(sp.uk, state_bytes);
@@ -114,7 +115,7 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
 #pragma unroll
     for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        if (t < S::TOTAL) s_tab[t] = tabv[k];
+        if (t < NTAB) s_tab[t] = tabv[k];
     }
 #pragma unroll
     for (int k = 0; k < kGeoPerThread; ++k) {
@@ -188,9 +189,15 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     for (int v = 0; v < 4; ++v) {
         v4f c = {0.0f, 0.0f, 0.0f, 0.0f}, d = c;
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-            c = mfma16x16x4(tEV[s], b[v][s], c);
-            d = mfma16x16x4(tEF[s], b[v][s], d);
+        for (int s = 0; s < R; ++s) d = mfma16x16x4(tEF[s], b[v][s], d);
+        if constexpr (BF16) {
+            // volume evaluation from bfloat16 coefficients and basis values, FP32 accumulation
+            const unsigned* eb = reinterpret_cast<const unsigned*>(&s_tab[S::T_EB + lane]);
+            c = mfma16x16x16bf16(eb[0], eb[64], pack_bf16x2(b[v][0], R > 1 ? b[v][1] : 0.0f),
+                                 pack_bf16x2(R > 2 ? b[v][2] : 0.0f, R > 3 ? b[v][R > 3 ? 3 : 0] : 0.0f), c);
+        } else {
+#pragma unroll
+            for (int s = 0; s < R; ++s) c = mfma16x16x4(tEV[s], b[v][s], c);
         }
         U[v] = c;
         Uf[v] = d;
@@ -224,10 +231,18 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (BF16) {
+            // projection of bfloat16 fluxes on bfloat16 basis gradients, FP32 accumulation
+            const unsigned* xb = reinterpret_cast<const unsigned*>(&s_tab[S::T_XB + lane]);
+            const unsigned* yb = reinterpret_cast<const unsigned*>(&s_tab[S::T_YB + lane]);
+            c = mfma16x16x16bf16(xb[0], xb[64], pack_bf16x2(Fh[v][0], Fh[v][1]), pack_bf16x2(Fh[v][2], Fh[v][3]), c);
+            c = mfma16x16x16bf16(yb[0], yb[64], pack_bf16x2(Gh[v][0], Gh[v][1]), pack_bf16x2(Gh[v][2], Gh[v][3]), c);
+        } else {
 #pragma unroll
-        for (int t = 0; t < RQ; ++t) {
-            c = mfma16x16x4(tPX[t], Fh[v][t], c);
-            c = mfma16x16x4(tPY[t], Gh[v][t], c);
+            for (int t = 0; t < RQ; ++t) {
+                c = mfma16x16x4(tPX[t], Fh[v][t], c);
+                c = mfma16x16x4(tPY[t], Gh[v][t], c);
+            }
         }
         acc[v] = c;
     }
@@ -330,18 +345,25 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
 // p = 3 runs fastest at three waves per SIMD (the register budget that allows), p = 4 at the
 // compiler's own choice (measured on gfx950)
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void stage_mfma_kernel_p3(StageParams sp) {
-    stage_mfma_body<3>(sp);
+    stage_mfma_body<3, false>(sp);
 }
-__global__ __launch_bounds__(kBlock) void stage_mfma_kernel_p4(StageParams sp) { stage_mfma_body<4>(sp); }
+__global__ __launch_bounds__(kBlock) void stage_mfma_kernel_p4(StageParams sp) { stage_mfma_body<4, false>(sp); }
+// bfloat16 volume products (precision DGX_FP32_BF16VOL)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void stage_mfma_bf16_kernel_p3(StageParams sp) {
+    stage_mfma_body<3, true>(sp);
+}
+__global__ __launch_bounds__(kBlock) void stage_mfma_bf16_kernel_p4(StageParams sp) { stage_mfma_body<4, true>(sp); }
 
 }  // namespace
 
-hipError_t launch_stage_mfma(int p, const StageParams& sp, hipStream_t s) {
+hipError_t launch_stage_mfma(int p, bool bf16vol, const StageParams& sp, hipStream_t s) {
     if (sp.K <= 0) return hipSuccess;
     dim3 grid((sp.K + 63) / 64), block(kBlock);
-    switch (p) {
-    case 3: hipLaunchKernelGGL(stage_mfma_kernel_p3, grid, block, 0, s, sp); break;
-    case 4: hipLaunchKernelGGL(stage_mfma_kernel_p4, grid, block, 0, s, sp); break;
+    switch (p * 2 + (bf16vol ? 1 : 0)) {
+    case 6: hipLaunchKernelGGL(stage_mfma_kernel_p3, grid, block, 0, s, sp); break;
+    case 7: hipLaunchKernelGGL(stage_mfma_bf16_kernel_p3, grid, block, 0, s, sp); break;
+    case 8: hipLaunchKernelGGL(stage_mfma_kernel_p4, grid, block, 0, s, sp); break;
+    case 9: hipLaunchKernelGGL(stage_mfma_bf16_kernel_p4, grid, block, 0, s, sp); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -170,3 +170,51 @@ def test_vjp_no_slip_ghost_against_oracle_fd():
             errs.append(abs(np.sum(g * d) - fd) / abs(fd))
         ok += min(errs) < 1e-6
     assert ok >= 3
+
+
+# ---------------- bfloat16 volume products ---------------------------------------------------
+
+@pytest.mark.parametrize("p", [3, 4])
+def test_bf16_volume_variant_error_is_bf16_sized(p):
+    # bfloat16 keeps 8 significant bits (relative rounding 2^-9) of the volume operands, so the
+    # variant's residual error scales with the volume term (about 100x the residual here, where
+    # volume and face terms nearly cancel): above the FP32 kernel's, below 2^-5 of that term
+    ora, mf, ln, u0 = _setup(p, "periodic", n=6)
+    verts, tris = R.split_quad_mesh(6)
+    bf = dgx.Context(verts, tris, p, basis="dubiner", precision="fp32_bf16vol")
+    assert bf.info.stage_kernel == 1
+    r_ref = ora.residual(u0)
+    vol = max(np.abs(ora.volume_residual(e, u0[e])).max() for e in range(ora.K))
+    d_bf = np.abs(bf.residual(u0) - r_ref).max()
+    d_32 = np.abs(mf.residual(u0) - r_ref).max()
+    assert d_32 < d_bf < 2 ** -5 * vol, (d_bf, d_32, vol)
+
+
+def test_bf16_volume_variant_steps_and_conserves():
+    ora, mf, ln, u0 = _setup(3, "periodic", n=8)
+    verts, tris = R.split_quad_mesh(8)
+    bf = dgx.Context(verts, tris, 3, basis="dubiner", precision="fp32_bf16vol")
+    bf.set_state(u0)
+    mf.set_state(u0)
+    s0 = bf.conservation_sums()
+    dt = dgx.stable_dt(mf, 0.3)
+    bf.rk3_steps(dt, 20)
+    mf.rk3_steps(dt, 20)
+    a, b = bf.get_state(), mf.get_state()
+    assert np.isfinite(a).all()
+    assert np.linalg.norm(a[..., 0] - b[..., 0]) / np.linalg.norm(b[..., 0]) < 1e-2
+    # the fluxes stay a telescoping sum over faces: mass is conserved to FP32 rounding
+    assert abs(bf.conservation_sums()[0] - s0[0]) < 1e-5 * abs(s0[0])
+
+
+def test_bf16_volume_variant_limits():
+    verts, tris = R.split_quad_mesh(4)
+    for kw in (dict(order=2), dict(order=3, weak_form=dgx.NavierStokesWeakFormulation())):
+        order = kw.pop("order")
+        with pytest.raises(dgx.DGXError) as e:
+            dgx.Context(verts, tris, order, precision="fp32_bf16vol", **kw)
+        assert e.value.status == dgx.DGX_E_UNSUPPORTED
+    bf = dgx.Context(verts, tris, 3, precision="fp32_bf16vol")
+    with pytest.raises(dgx.DGXError) as e:
+        bf.rhs_vjp(np.zeros((bf.K, bf.Np, 4)), np.zeros((bf.K, bf.Np, 4)) + 1.0)
+    assert e.value.status == dgx.DGX_E_UNSUPPORTED
