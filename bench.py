@@ -14,7 +14,8 @@ One DOF-stage update = one conserved-variable modal coefficient advanced by one 
 Output: one JSON line (rank 0).  Besides the contract keys it carries
   roofline      : dominant kernel (the fused FP32 stage kernel), algorithmic bytes per launch
                   / mean launch duration from HIP events on the launch stream, vs 8 TB/s
-  cpu_baseline  : the FP64 CPU oracle (port of the reference loop), single thread, timed on
+  cpu_baseline  : the FP64 CPU oracle (port of the reference loop) on OMP_NUM_THREADS threads
+                  (and on one thread, single_thread_value), timed on
                   a bounded sample of the same mesh
   fp64          : the same workload in FP64 (DOF-stage/s)
   l2_err_fp32_vs_fp64 : per-field relative L2 at volume points after the timed steps
@@ -68,13 +69,19 @@ def l2_rel(a, b):
     return [float(np.linalg.norm(a[..., v] - b[..., v]) / np.linalg.norm(b[..., v])) for v in range(4)]
 
 
-def cpu_baseline(verts, tris, p, u0, dt, steps):
+def cpu_baseline(verts, tris, p, u0, dt, steps, threads):
+    """The FP64 CPU oracle (a port of the reference's loop structure) on the host: `steps` steps
+    on one thread, and 4x as many on `threads` OpenMP threads (the same arithmetic, bitwise),
+    each after one warm-up step.  Returns DOF-stage/s and seconds for both."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcpu as R
     ora = R.Oracle(verts, tris, p, basis="dubiner")
-    _, secs = ora.rk3(u0, dt, 1)  # warm-up step
-    _, secs = ora.rk3(u0, dt, steps)
-    return 12.0 * ora.Np * ora.K * steps / secs, secs
+    dof = 12.0 * ora.Np * ora.K
+    ora.rk3(u0, dt, 1)
+    _, secs1 = ora.rk3(u0, dt, steps)
+    ora.rk3_threads(u0, dt, 1, threads)
+    _, secsn = ora.rk3_threads(u0, dt, 4 * steps, threads)
+    return dof * steps / secs1, secs1, dof * 4 * steps / secsn, secsn
 
 
 def load_traffic(workload, kernel=None):
@@ -188,11 +195,14 @@ def main():
     ctxa.close()
 
     if not args.no_cpu:
-        v, secs = cpu_baseline(verts, tris, p, u0, dt, args.cpu_steps)
-        result["cpu_baseline"] = {"value": v, "unit": "DOF-stage updates/s", "cores": 1, "kind": "port",
-                                  "sample": f"{args.cpu_steps} FP64 SSP-RK3 steps of the same {K}-element p={p} mesh "
-                                            f"({secs:.1f} s, one thread, oracle/refcpu.cpp)",
-                                  "host_cpus": os.cpu_count()}
+        # the box's CPU share is what OMP_NUM_THREADS says (not os.cpu_count())
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        v1, s1, vn, sn = cpu_baseline(verts, tris, p, u0, dt, args.cpu_steps, threads)
+        result["cpu_baseline"] = {"value": vn, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+                                  "sample": f"{4 * args.cpu_steps} FP64 SSP-RK3 steps of the same {K}-element p={p} "
+                                            f"mesh on {threads} OpenMP threads ({sn:.1f} s, oracle/refcpu.cpp); "
+                                            f"{args.cpu_steps} steps on one thread ({s1:.1f} s)",
+                                  "single_thread_value": v1, "host_cpus": os.cpu_count()}
     ctx32.close()
     print(json.dumps(result))
 

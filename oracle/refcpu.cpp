@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <omp.h>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -384,6 +385,10 @@ struct Ctx {
     std::vector<double> minv;         // K*np*np
     NSParams ns{1.4, 1e-3, 0.72, 5.0, 1.0};
     std::vector<double> rbuf, kbuf;
+    // threaded timing path: per-face contributions and, per element, its faces in face order
+    std::vector<double> fl, fr, fvl, fvr, fb, fvb;
+    std::vector<std::vector<std::pair<int, int>>> eif;  // (interior face, side 0 = L / 1 = R)
+    std::vector<std::vector<int>> ebf;                   // boundary faces
 };
 
 // Connectivity through a map keyed by the sorted vertex pair (S/core/mesh.cpp:312-359 builds
@@ -866,6 +871,111 @@ void rhs(Ctx& c, const double* u, double* k) {
     apply_minv(c, c.rbuf.data(), k);
 }
 
+// The same residual with threads, for the multi-core CPU baseline: elements and faces are
+// processed in parallel and every face contribution is written to its own buffer, then each
+// element adds its contributions in the order of the sequential assembly (volume, viscous
+// volume, interior faces by index, viscous interior faces, boundary faces, viscous boundary
+// faces), so the result is bitwise that of assemble().
+void assemble_threaded(Ctx& c, const double* u, double* R) {
+    const int nb = c.np * 4;
+    const bool visc = c.physics == 1;
+    const int nif = (int)c.ifaces.size(), nbf = (int)c.bfaces.size();
+    if (c.eif.empty()) {
+        c.eif.assign(c.K, {});
+        c.ebf.assign(c.K, {});
+        for (int k = 0; k < nif; ++k) {
+            c.eif[c.ifaces[k].eL].push_back({k, 0});
+            c.eif[c.ifaces[k].eR].push_back({k, 1});
+        }
+        for (int k = 0; k < nbf; ++k) c.ebf[c.bfaces[k].e].push_back(k);
+        c.fl.assign((size_t)nif * nb, 0.0);
+        c.fr.assign((size_t)nif * nb, 0.0);
+        c.fb.assign((size_t)nbf * nb, 0.0);
+        if (visc) {
+            c.fvl.assign((size_t)nif * nb, 0.0);
+            c.fvr.assign((size_t)nif * nb, 0.0);
+            c.fvb.assign((size_t)nbf * nb, 0.0);
+        }
+    }
+#pragma omp parallel
+    {
+#pragma omp for schedule(static)
+        for (int k = 0; k < nif; ++k) {
+            const auto& fc = c.ifaces[k];
+            interior_face_residual(c, fc, u + (size_t)fc.eL * nb, u + (size_t)fc.eR * nb, &c.fl[(size_t)k * nb],
+                                   &c.fr[(size_t)k * nb]);
+            if (visc)
+                viscous_interior_face_residual(c, fc, u + (size_t)fc.eL * nb, u + (size_t)fc.eR * nb,
+                                               &c.fvl[(size_t)k * nb], &c.fvr[(size_t)k * nb]);
+        }
+#pragma omp for schedule(static)
+        for (int k = 0; k < nbf; ++k) {
+            const BC* bc = face_bc(c, c.bfaces[k]);
+            if (!bc) continue;
+            boundary_face_residual(c, c.bfaces[k], *bc, u + (size_t)c.bfaces[k].e * nb, &c.fb[(size_t)k * nb]);
+            if (visc) viscous_boundary_face_residual(c, c.bfaces[k], bc, u + (size_t)c.bfaces[k].e * nb,
+                                                     &c.fvb[(size_t)k * nb]);
+        }
+#pragma omp for schedule(static)
+        for (int e = 0; e < c.K; ++e) {
+            double* r = R + (size_t)e * nb;
+            std::vector<double> a(nb);
+            volume_residual(c, e, u + (size_t)e * nb, r);
+            if (visc) {
+                viscous_volume_residual(c, e, u + (size_t)e * nb, a.data());
+                for (int k = 0; k < nb; ++k) r[k] += a[k];
+            }
+            for (const auto& [k, side] : c.eif[e]) {
+                const double* x = &(side ? c.fr : c.fl)[(size_t)k * nb];
+                for (int i = 0; i < nb; ++i) r[i] += x[i];
+            }
+            if (visc)
+                for (const auto& [k, side] : c.eif[e]) {
+                    const double* x = &(side ? c.fvr : c.fvl)[(size_t)k * nb];
+                    for (int i = 0; i < nb; ++i) r[i] += x[i];
+                }
+            for (int k : c.ebf[e]) {
+                if (!face_bc(c, c.bfaces[k])) continue;
+                for (int i = 0; i < nb; ++i) r[i] += c.fb[(size_t)k * nb + i];
+            }
+            if (visc)
+                for (int k : c.ebf[e]) {
+                    if (!face_bc(c, c.bfaces[k])) continue;
+                    for (int i = 0; i < nb; ++i) r[i] += c.fvb[(size_t)k * nb + i];
+                }
+        }
+    }
+}
+
+void rk3_step_threaded(Ctx& c, double* u, double dt) {
+    const size_t n = (size_t)c.K * c.np * 4;
+    const int nb = c.np * 4;
+    std::vector<double> k(n), u1(n), u2(n), r(n);
+    auto L = [&](const double* x) {
+        assemble_threaded(c, x, r.data());
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < c.K; ++e) {
+            const int np = c.np;
+            const double* M = &c.minv[(size_t)e * np * np];
+            for (int i = 0; i < np; ++i)
+                for (int v = 0; v < 4; ++v) {
+                    double s = 0.0;
+                    for (int j = 0; j < np; ++j) s += M[i * np + j] * r[(size_t)e * nb + j * 4 + v];
+                    k[(size_t)e * nb + i * 4 + v] = s;
+                }
+        }
+    };
+    L(u);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i) u1[i] = u[i] + dt * k[i];
+    L(u1.data());
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i) u2[i] = 0.75 * u[i] + 0.25 * (u1[i] + dt * k[i]);
+    L(u2.data());
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i) u[i] = (1.0 / 3.0) * u[i] + (2.0 / 3.0) * (u2[i] + dt * k[i]);
+}
+
 // SSP_RK::step_rk3 (I/solver/time_stepping.hpp:39-55) with the reference's operation order:
 // u1 = u + dt k1; u2 = 0.75 u + 0.25 (u1 + dt k2); u' = (1/3) u + (2/3) (u2 + dt k3).
 void rk3_step(Ctx& c, double* u, double dt) {
@@ -959,6 +1069,16 @@ double ref_rk3(void* h, double* u, double dt, int nsteps) {
     Ctx& c = *static_cast<Ctx*>(h);
     auto t0 = std::chrono::steady_clock::now();
     for (int s = 0; s < nsteps; ++s) rk3_step(c, u, dt);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// nsteps with the threaded residual on `nthreads` OpenMP threads (bitwise equal to ref_rk3).
+double ref_rk3_threads(void* h, double* u, double dt, int nsteps, int nthreads) {
+    Ctx& c = *static_cast<Ctx*>(h);
+    omp_set_num_threads(nthreads > 0 ? nthreads : 1);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < nsteps; ++s) rk3_step_threaded(c, u, dt);
     auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
 }

@@ -50,6 +50,7 @@ int ref_apply_minv(void* h, double* r);
 int ref_rhs(void* h, const double* u, double* k);
 // SSP-RK3, nsteps steps in place; returns seconds spent in the loop (steady clock).
 double ref_rk3(void* h, double* u, double dt, int nsteps);
+double ref_rk3_threads(void* h, double* u, double dt, int nsteps, int nthreads);
 int ref_project(void* h, const double* Uq, double* u);  // Uq: [K][Nq][4] at volume points
 int ref_volume_points(void* h, double* xy);              // [K][Nq][2]
 int ref_eval_volume(void* h, const double* u, double* Uq);  // [K][Nq][4]

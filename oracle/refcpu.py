@@ -47,6 +47,8 @@ def lib():
         L.ref_volume_points.argtypes = [C.c_void_p, dp]
         L.ref_rk3.restype = C.c_double
         L.ref_rk3.argtypes = [C.c_void_p, dp, C.c_double, C.c_int]
+        L.ref_rk3_threads.restype = C.c_double
+        L.ref_rk3_threads.argtypes = [C.c_void_p, dp, C.c_double, C.c_int, C.c_int]
         L.ref_max_cfl.restype = C.c_double
         L.ref_max_cfl.argtypes = [C.c_void_p, dp, C.c_double]
         L.ref_density_range.argtypes = [C.c_void_p, dp, dp]
@@ -233,6 +235,12 @@ class Oracle:
         """Returns (u_new, seconds in the stepping loop)."""
         u = np.array(u, dtype=np.float64, order="C")
         secs = lib().ref_rk3(self._h, _dp(u), dt, nsteps)
+        return u, secs
+
+    def rk3_threads(self, u, dt, nsteps, nthreads):
+        """rk3 on `nthreads` OpenMP threads (same result bit for bit): (u_new, seconds)."""
+        u = np.array(u, dtype=np.float64, order="C")
+        secs = lib().ref_rk3_threads(self._h, _dp(u), dt, nsteps, nthreads)
         return u, secs
 
     def volume_points(self):

@@ -255,3 +255,18 @@ def test_connectivity_and_periodic_pairing():
             mid = 0.5 * (V[f] + V[(f + 1) % 3])
             assert abs(np.linalg.norm(n) - 1) < 1e-15 and n @ (mid - c) > 0
             assert abs(L - np.linalg.norm(V[(f + 1) % 3] - V[f])) < 1e-15
+
+
+@pytest.mark.parametrize("physics", ["euler", "ns"])
+def test_threaded_rk3_is_bitwise_the_sequential_one(physics):
+    # the multi-core CPU baseline runs the same arithmetic in the same per-element order
+    verts, tris = R.split_quad_mesh(6, 1.0)
+    kw = dict(physics="ns", mu=0.01) if physics == "ns" else {}
+    ora = R.Oracle(verts, tris, 2, basis="dubiner", periodic=1, **kw)  # Bottom/Top bounded
+    ora.set_bc("Bottom", "far_field", [1.0, 1.0, 0.0, 1 / 0.4 + 0.5])
+    ora.set_bc("Top", "no_slip_wall", [1.0, 0.0, 0.0, 1.0])
+    xy = ora.volume_points()
+    u0 = ora.project(R.vortex_state(xy, xc=0.5, yc=0.5, beta=1.0, length=1.0))
+    a, _ = ora.rk3(u0, 1e-3, 3)
+    b, _ = ora.rk3_threads(u0, 1e-3, 3, 4)
+    assert np.array_equal(a, b)
