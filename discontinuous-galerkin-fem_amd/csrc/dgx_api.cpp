@@ -321,7 +321,7 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         c.upload_real<float>(c.tabL, tL);
         c.upload_real<float>(c.tabR, tR);
     }
-    // matrix-core stage kernel: FP32 Euler, degree-2p volume rule, p = 3, 4
+    // matrix-core stage kernels: FP32 Euler and Navier-Stokes, degree-2p volume rule, p = 3, 4
     // (DGX_STAGE_KERNEL=lane keeps the one-element-per-lane kernel for FP32)
     const char* kv = std::getenv("DGX_STAGE_KERNEL");
     const bool lane_only = kv && std::string(kv) == "lane" && precision == DGX_FP32;
@@ -336,6 +336,16 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         c.upload(c.tabVL, m.data(), m.size() * sizeof(float));
         build_mfma_vjp_tables(c.p, tR, m);
         c.upload(c.tabVR, m.data(), m.size() * sizeof(float));
+    }
+    if (precision == DGX_FP32 && c.physics == DGX_NAVIER_STOKES && c.rule == 0 && mfma_order(c.p) && !lane_only) {
+        std::vector<float> m;
+        build_mfma_ns_tables(c.p, tL, false, m);
+        c.upload(c.tabML, m.data(), m.size() * sizeof(float));
+        build_mfma_ns_tables(c.p, tR, false, m);
+        c.upload(c.tabMR, m.data(), m.size() * sizeof(float));
+        build_mfma_ns_tables(c.p, tL, true, m);
+        c.upload(c.tabNT, m.data(), m.size() * sizeof(float));
+        c.rec.alloc((size_t)3 * c.nfq * c.Kp * 32);
     }
     c.u.alloc(c.state_bytes());
     c.u1.alloc(c.state_bytes());

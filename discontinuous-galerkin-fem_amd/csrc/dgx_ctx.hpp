@@ -100,12 +100,13 @@ struct dgx_ctx {
     DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
     DevBuf tabML, tabMR;  // MFMA stage kernel lane tables (RK/RHS and residual variants)
     DevBuf tabVL, tabVR;  // MFMA stage adjoint lane tables (same two variants)
+    DevBuf tabNT, rec;    // MFMA Navier-Stokes face-trace tables and face-point records
     std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][3][2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~dgx_ctx() {
         for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage,
-                          &partial, &tabML, &tabMR, &tabVL, &tabVR})
+                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -145,6 +146,8 @@ struct dgx_ctx {
         sp.code = code.as<uint32_t>();
         sp.tab = (mode == MODE_RESIDUAL) ? tabR.p : tabL.p;
         sp.tabm = (mode == MODE_RESIDUAL) ? tabMR.p : tabML.p;
+        sp.tabt = tabNT.p;
+        sp.rec = rec.p;
         sp.bc = bc.as<double>();
         sp.K = K;
         sp.Kp = Kp;

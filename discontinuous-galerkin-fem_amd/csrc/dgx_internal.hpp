@@ -80,7 +80,9 @@ struct StageParams {
     const int32_t* nbr;
     const uint32_t* code;
     const void* tab;        // tables in the state precision
-    const void* tabm;       // lane-major tables of the MFMA kernel (FP32 Euler), or null
+    const void* tabm;       // lane-major tables of the MFMA kernel (FP32 Euler / Navier-Stokes), or null
+    const void* tabt;       // face-trace tables of the MFMA Navier-Stokes kernel, or null
+    void* rec;              // face-point records [3 Nfq][Kp] (U, Fv.n) of the MFMA Navier-Stokes stage
     const double* bc;       // [n_bcs][5]: type, 4 values
     int K, Kp;
     double dt, ca, cb;

@@ -76,6 +76,27 @@ struct MfmaVjpShape {
     static constexpr int TOTAL = T_N * 64 + 2 * NT;
 };
 
+// Lane tables of the matrix-core Navier-Stokes stage (dgx_stage_mfma_ns.hip), 64 floats each.
+// Face-trace kernel: EF[R] (Phi_F) and FG[8][R], the 8 face-gradient slots (c, d, k) of the
+// reference's read (rows: face slots, k: modes).  Stage kernel: EV[R], GX[R], GY[R] (Phi_V and
+// its reference derivatives; rows: volume slots, k: modes), PX[RQ], PY[RQ] (rows: modes, k:
+// volume slots), LF[RF] (rows: modes, k: face slots).
+template <int P>
+struct MfmaNsShape {
+    using F = MfmaShape<P>;
+    static constexpr int R = F::R, RQ = F::RQ, RF = F::RF;
+    static constexpr int T_EF = 0;
+    static constexpr int T_FG = T_EF + R;
+    static constexpr int TOTAL_T = (T_FG + 8 * R) * 64;
+    static constexpr int M_EV = 0;
+    static constexpr int M_GX = M_EV + R;
+    static constexpr int M_GY = M_GX + R;
+    static constexpr int M_PX = M_GY + R;
+    static constexpr int M_PY = M_PX + RQ;
+    static constexpr int M_LF = M_PY + RQ;
+    static constexpr int TOTAL_M = (M_LF + RF) * 64;
+};
+
 // slot -> mode / volume point / face point (f * nfq + q); -1 for a padding slot
 __host__ __device__ constexpr int mf_mode(int R, int np, int g, int r) {
     return (r < R && R * g + r < np) ? R * g + r : -1;
@@ -94,5 +115,7 @@ int mfma_table_size(int p);  // floats
 // With bf16 set, the bfloat16 volume tables follow (TOTAL_BF floats in all).
 void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out, bool bf16 = false);
 void build_mfma_vjp_tables(int p, const std::vector<double>& tab, std::vector<float>& out);
+// Navier-Stokes: face-trace tables (trace = true) or stage tables from the L / R tables.
+void build_mfma_ns_tables(int p, const std::vector<double>& tab, bool trace, std::vector<float>& out);
 
 }  // namespace dgx

@@ -135,7 +135,57 @@ void build_vjp(const std::vector<double>& t, std::vector<float>& out) {
                 }
 }
 
+template <int P>
+void build_ns(const std::vector<double>& t, bool trace, std::vector<float>& out) {
+    using S = MfmaNsShape<P>;
+    using F = MfmaShape<P>;
+    const TabLayout L(P, 0);
+    const int np = F::NP, nq = F::NQ, nfq = F::NFQ;
+    if ((int)t.size() != L.total) throw std::runtime_error("table size mismatch");
+    out.assign(trace ? S::TOTAL_T : S::TOTAL_M, 0.0f);
+    auto at = [&](int table, int lane) -> float& { return out[(size_t)table * 64 + lane]; };
+    for (int lane = 0; lane < 64; ++lane) {
+        const int i = lane % 16, k = lane / 16;
+        int vp = mf_vpoint(S::RQ, nq, i / 4, i % 4), fp = mf_fpoint(nfq, i / 4, i % 4);
+        if (vp < 0) vp = 0;  // padding rows repeat a valid point (finite states)
+        if (fp < 0) fp = 0;
+        const int mrow = mf_mode(S::R, np, i / 4, i % 4);
+        for (int s = 0; s < S::R; ++s) {
+            const int m = mf_mode(S::R, np, k, s);
+            if (m < 0) continue;
+            if (trace) {
+                at(S::T_EF + s, lane) = float(t[L.phiF + fp * np + m]);
+                for (int slot = 0; slot < 8; ++slot)
+                    at(S::T_FG + slot * S::R + s, lane) = float(t[L.FG + (fp * 8 + slot) * np + m]);
+            } else {
+                at(S::M_EV + s, lane) = float(t[L.phiV + vp * np + m]);
+                at(S::M_GX + s, lane) = float(t[L.DXV + vp * np + m]);
+                at(S::M_GY + s, lane) = float(t[L.DYV + vp * np + m]);
+            }
+        }
+        if (trace) continue;
+        for (int r = 0; r < S::RQ; ++r) {
+            const int q = mf_vpoint(S::RQ, nq, k, r);
+            const bool ok = mrow >= 0 && q >= 0;
+            at(S::M_PX + r, lane) = ok ? float(t[L.PX + q * np + mrow]) : 0.0f;
+            at(S::M_PY + r, lane) = ok ? float(t[L.PY + q * np + mrow]) : 0.0f;
+        }
+        for (int r = 0; r < S::RF; ++r) {
+            const int f = mf_fpoint(nfq, k, r);
+            at(S::M_LF + r, lane) = (mrow >= 0 && f >= 0) ? float(t[L.LF + f * np + mrow]) : 0.0f;
+        }
+    }
+}
+
 }  // namespace
+
+void build_mfma_ns_tables(int p, const std::vector<double>& tab, bool trace, std::vector<float>& out) {
+    switch (p) {
+    case 3: build_ns<3>(tab, trace, out); break;
+    case 4: build_ns<4>(tab, trace, out); break;
+    default: throw std::runtime_error("no MFMA Navier-Stokes tables for this order");
+    }
+}
 
 void build_mfma_vjp_tables(int p, const std::vector<double>& tab, std::vector<float>& out) {
     switch (p) {
