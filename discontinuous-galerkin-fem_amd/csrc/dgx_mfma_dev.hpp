@@ -51,6 +51,26 @@ __device__ __forceinline__ void ghost_state(const float* Uo, float nx, float ny,
     }
 }
 
+// Twice the Rusanov flux, 2 H(A, B, n) = F(A).n + F(B).n - s (B - A): the same arithmetic as
+// rusanov() with the factor 1/2 left to the caller's face scale.
+__device__ __forceinline__ void rusanov2(const float* A, const float* B, float nx, float ny, float g, float* H) {
+    const float ria = rcp(A[0]), rib = rcp(B[0]);
+    const float ua = A[1] * ria, va = A[2] * ria, ub = B[1] * rib, vb = B[2] * rib;
+    const float pa = (g - 1.0f) * (A[3] - 0.5f * A[0] * (ua * ua + va * va));
+    const float pb = (g - 1.0f) * (B[3] - 0.5f * B[0] * (ub * ub + vb * vb));
+    const float una = ua * nx + va * ny, unb = ub * nx + vb * ny;
+    const float ca = dsqrt(g * pa * ria), cb = dsqrt(g * pb * rib);
+    const float s = fmaxf(fabsf(una) + ca, fabsf(unb) + cb);
+    const float fa0 = A[0] * una, fb0 = B[0] * unb;
+    const float fa1 = A[1] * una + pa * nx, fb1 = B[1] * unb + pb * nx;
+    const float fa2 = A[2] * una + pa * ny, fb2 = B[2] * unb + pb * ny;
+    const float fa3 = (A[3] + pa) * una, fb3 = (B[3] + pb) * unb;
+    H[0] = (fa0 + fb0) - s * (B[0] - A[0]);
+    H[1] = (fa1 + fb1) - s * (B[1] - A[1]);
+    H[2] = (fa2 + fb2) - s * (B[2] - A[2]);
+    H[3] = (fa3 + fb3) - s * (B[3] - A[3]);
+}
+
 // Flux of one face point in the canonical (L) orientation, scaled for this element's lift.
 __device__ __forceinline__ void face_point_flux(const float* Uo, float* Un, uint32_t c, int nb, float nx, float ny,
                                                 float sf, const StageParams& sp, float gam, float* Hs) {

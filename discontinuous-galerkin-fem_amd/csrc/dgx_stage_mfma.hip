@@ -74,7 +74,10 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
 #pragma unroll
     for (int f = 0; f < 3; ++f)
         nbf[f] = int(__builtin_amdgcn_raw_buffer_load_b32(r_nbr, unsigned(ec) * 4u, unsigned(f) * 4u * uKp, 0));
-    const int ft = w < 3 ? w : 0;
+    // wave-uniform values that enter scalar buffer offsets go through readfirstlane, so the
+    // loads take them from a scalar register instead of a per-lane loop
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const int ft = wu < 3 ? wu : 0;
     const uint32_t ct = (__builtin_amdgcn_raw_buffer_load_b32(r_code, unsigned(et) * 4u, 0, 0) >> (4 * ft)) & 15u;
     const int nbt = int(__builtin_amdgcn_raw_buffer_load_b32(r_nbr, unsigned(et) * 4u, unsigned(ft) * 4u * uKp, 0));
 
@@ -92,15 +95,13 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
     }
-    // geometry rows of the block: thread t loads row t / 64 + 4k of element t % 64
+    // geometry rows of the block: wave w loads rows w + 4k of the block's elements (rows past
+    // the last read past the resource: zeros, no traffic)
     constexpr int kGeoPerThread = (kGeoRows * EB + kBlock - 1) / kBlock;
     double geov[kGeoPerThread];
 #pragma unroll
-    for (int k = 0; k < kGeoPerThread; ++k) {
-        const int t = int(threadIdx.x) + k * kBlock;
-        const int row = t / EB, el = min(eb0 + t % EB, sp.K - 1);
-        geov[k] = t < kGeoRows * EB ? load_f64(r_geo, unsigned(el) * 8u, unsigned(row) * 8u * uKp) : 0.0;
-    }
+    for (int k = 0; k < kGeoPerThread; ++k)
+        geov[k] = load_f64(r_geo, unsigned(et) * 8u, unsigned(min(wu + 4 * k, kGeoRows)) * 8u * uKp);
     // neighbour coefficients for the trace role (all modes of one neighbour per lane)
     const bool t_interior = w < 3 && (ct & 3u) != kFaceBoundary;
     float cn[NP][4];
@@ -118,10 +119,8 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
         if (t < NTAB) s_tab[t] = tabv[k];
     }
 #pragma unroll
-    for (int k = 0; k < kGeoPerThread; ++k) {
-        const int t = int(threadIdx.x) + k * kBlock;
-        if (t < kGeoRows * EB) s_geo[t / EB][t % EB] = geov[k];
-    }
+    for (int k = 0; k < kGeoPerThread; ++k)
+        if (wu + 4 * k < kGeoRows) s_geo[wu + 4 * k][lane] = geov[k];
     __syncthreads();
     // keep the scheduler from sinking the loads above into the arithmetic below
     __builtin_amdgcn_sched_barrier(0);
@@ -279,9 +278,11 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
             Aq[v] = isL ? Uo[v] : Un[v];
             Bq[v] = isL ? Un[v] : Uo[v];
         }
-        rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
+        rusanov2(Aq, Bq, sgn * nx, sgn * ny, gam, H);
+        // a padding slot may hold a non-finite value (stale LDS): select, do not scale
+        const float hs = 0.5f * sgn * sf;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Hs[v][f] = slot_valid ? sgn * sf * H[v] : 0.0f;
+        for (int v = 0; v < 4; ++v) Hs[v][f] = slot_valid ? hs * H[v] : 0.0f;
     }
     // boundary face slots: ghost state from the boundary condition (only waves that have one)
     if (__builtin_amdgcn_ballot_w64(any_boundary) != 0) {
