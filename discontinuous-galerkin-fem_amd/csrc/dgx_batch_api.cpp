@@ -20,6 +20,9 @@ struct dgx_batch {
     DevBuf items_all, items_p[4];     // work lists (fused, per order)
     int n_all = 0, n_p[4] = {0, 0, 0, 0};
     const void* tabs[4] = {nullptr, nullptr, nullptr, nullptr};
+    // orders whose instances all run the matrix-core stage (FP32, p = 3, 4): their lane tables;
+    // those items always go through the per-order matrix-core launch
+    const void* tabm[4] = {nullptr, nullptr, nullptr, nullptr};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -75,7 +78,8 @@ struct dgx_batch {
             if (!params[w].p) params[w].alloc(n * sizeof(StageParams));
             hip_check(hipMemcpy(params[w].p, h.data(), n * sizeof(StageParams), hipMemcpyHostToDevice), "params");
         }
-        // work items, most expensive orders first so the fixed-stride walk spreads them
+        // work items, most expensive orders first so the fixed-stride walk spreads them; items
+        // of matrix-core orders are whole 64-element tiles and stay out of the fused list
         std::vector<BatchItem> all, per[4];
         for (int p = 4; p >= 1; --p)
             for (int i = 0; i < n; ++i) {
@@ -84,7 +88,7 @@ struct dgx_batch {
                 const int chunk = 64 * std::max(1, tune.chunk[p - 1]);
                 for (int e0 = 0; e0 < c->K; e0 += chunk) {
                     BatchItem it{i, e0, std::min(chunk, c->K - e0), p};
-                    all.push_back(it);
+                    if (!tabm[p - 1]) all.push_back(it);
                     per[p - 1].push_back(it);
                 }
             }
@@ -114,12 +118,21 @@ struct dgx_batch {
 
     void launch_window_stage(int w) {
         const StageParams* pr = static_cast<const StageParams*>(params[w].p);
+        for (int p = 4; p >= 1; --p)
+            if (tabm[p - 1] && n_p[p - 1])
+                hip_check(launch_batch_stage_mfma(p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1], tabm[p - 1], grid(),
+                                                  stream),
+                          "batch stage (matrix cores)");
         if (tune.layout == 0) {
-            hip_check(launch_batch_stage(precision, pr, items_all.as<BatchItem>(), n_all, tabs, order_mask, grid(), stream),
-                      "batch stage");
+            int mask = order_mask;
+            for (int p = 1; p <= 4; ++p)
+                if (tabm[p - 1]) mask &= ~(1 << (p - 1));
+            if (n_all)
+                hip_check(launch_batch_stage(precision, pr, items_all.as<BatchItem>(), n_all, tabs, mask, grid(), stream),
+                          "batch stage");
         } else {
             for (int p = 4; p >= 1; --p)
-                if (n_p[p - 1])
+                if (!tabm[p - 1] && n_p[p - 1])
                     hip_check(launch_batch_stage_order(precision, p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1],
                                                        tabs[p - 1], grid(), stream),
                               "batch stage (order)");
@@ -206,6 +219,19 @@ dgx_status dgx_batch_create(int32_t n, const dgx_mesh_desc* meshes, const dgx_sp
         // identical tables per order need the same basis across instances
         for (auto* c : b->inst)
             if (c->basis != b->inst[0]->basis) throw std::invalid_argument("batched instances must share one basis");
+        // matrix-core orders: every instance of the order has the lane tables (contexts of one
+        // batch are created alike, so this holds for all or none)
+        for (int p = 1; p <= 4; ++p) {
+            const void* t = nullptr;
+            bool all = true, any = false;
+            for (auto* c : b->inst)
+                if (c->p == p) {
+                    any = true;
+                    all = all && c->tabML.p;
+                    if (!t) t = c->tabML.p;
+                }
+            b->tabm[p - 1] = (any && all) ? t : nullptr;
+        }
         b->precision = precision;
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");

@@ -149,6 +149,10 @@ hipError_t launch_batch_stage(int precision, const StageParams* params, const Ba
                               const void* const tabs[4], int order_mask, int grid, hipStream_t s);
 hipError_t launch_batch_stage_order(int precision, int p, const StageParams* params, const BatchItem* items,
                                     int n_items, const void* tab, int grid, hipStream_t s);
+// Batched FP32 Euler items of order p = 3 or 4 on the matrix cores (dgx_stage_mfma.hip): block
+// work items of whole 64-element tiles, `tab` the order's lane tables.
+hipError_t launch_batch_stage_mfma(int p, const StageParams* params, const BatchItem* items, int n_items,
+                                   const void* tab, int grid, hipStream_t s);
 // gradient of the kinetic-energy functional with respect to the coefficients (device layout)
 hipError_t launch_ke_grad(int precision, int p, int rule, const void* u, const double* tab64, const double* qw,
                           const double* geo, void* grad, int K, int Kp, Off4 uref, hipStream_t s);

@@ -25,19 +25,34 @@
 namespace dgx {
 namespace {
 
+constexpr int kTileEl = kBlock / 4;  // elements per block tile: 4 waves x 16
+
+// LDS of one block: lane tables (and the neighbour-trace and bf16 tables), the tile's FP64
+// geometry rows and the neighbour traces [face][point][element].
 template <int P, bool BF16>
-__device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
+struct MfmaLds {
     using S = MfmaShape<P>;
-    constexpr int NTAB = BF16 ? S::TOTAL_BF : S::TOTAL;
+    static constexpr int NTAB = BF16 ? S::TOTAL_BF : S::TOTAL;
+    float tab[NTAB];
+    double geo[kGeoRows][kTileEl];
+    float4 tr[3 * S::NFQ * kTileEl];
+};
+
+// One 64-element tile starting at element eb0.  TABLES: the tile's prologue also stages the
+// lane tables in LDS (a single-tile block); otherwise the caller has staged them.  The caller
+// separates consecutive tiles of one block with a barrier.
+template <int P, bool BF16, bool TABLES>
+__device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int eb0, MfmaLds<P, BF16>& L) {
+    using S = MfmaShape<P>;
+    constexpr int NTAB = MfmaLds<P, BF16>::NTAB;
     constexpr int NP = S::NP, NFQ = S::NFQ, R = S::R, RQ = S::RQ, RF = S::RF;
-    constexpr int EB = kBlock / 4;  // elements per block: 4 waves x 16
-    __shared__ float s_tab[NTAB];          // lane tables, neighbour-trace table, bf16 tables
-    __shared__ double s_geo[kGeoRows][EB];  // FP64 geometry rows of the block's elements
-    __shared__ float4 s_tr[3 * NFQ * EB];   // neighbour traces [face][point][element]
+    constexpr int EB = kTileEl;
+    float* s_tab = L.tab;
+    double (*s_geo)[EB] = L.geo;
+    float4* s_tr = L.tr;
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
-    const int eb0 = xcd_tile(blockIdx.x, gridDim.x) * EB;
     const int e = eb0 + 16 * w + j;  // element of this lane in the product layout
     const bool valid = e < sp.K;
     const int ec = valid ? e : sp.K - 1;  // lanes past the end compute on a valid element
@@ -57,8 +72,8 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     // read or a dropped write.  The tables and the block's geometry are loaded cooperatively
     // (one contiguous geometry row of 64 elements per wave instruction) and staged in LDS, so
     // no lane holds them in registers while the loads are in flight.
-    constexpr int kTabPerThread = (NTAB + kBlock - 1) / kBlock;
-    float tabv[kTabPerThread];
+    constexpr int kTabPerThread = TABLES ? (NTAB + kBlock - 1) / kBlock : 0;
+    float tabv[kTabPerThread > 0 ? kTabPerThread : 1];
 #pragma unroll
     for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
@@ -312,7 +327,7 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
         for (int t = 0; t < RF; ++t) acc[v] = mfma16x16x4(tLF[t], Hs[v][t], acc[v]);
 
     // ---------------- output ---------------------------------------------------------------
-    if (!valid) return;
+    if (!valid) return;  // (no barrier follows inside the tile)
     const auto r_out = make_rsrc(sp.out, state_bytes);
     const float dt = float(sp.dt), ca = float(sp.ca), cb = float(sp.cb);
     // u^n for the second and third RK stages
@@ -343,6 +358,33 @@ __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     }
 }
 
+template <int P, bool BF16>
+__device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
+    __shared__ MfmaLds<P, BF16> L;
+    stage_mfma_tile<P, BF16, true>(sp, xcd_tile(blockIdx.x, gridDim.x) * kTileEl, L);
+}
+
+// Batched instances of one order: each block walks its share of the work items (instance,
+// first element, element count; counts are whole tiles except at an instance's end) with a
+// fixed stride, tile by tile, the order's tables staged once.
+template <int P>
+__device__ __forceinline__ void batch_stage_mfma_body(const StageParams* __restrict__ params,
+                                                      const BatchItem* __restrict__ items, int n_items,
+                                                      const float* __restrict__ tm) {
+    using S = MfmaShape<P>;
+    __shared__ MfmaLds<P, false> L;
+    for (int k = int(threadIdx.x); k < S::TOTAL; k += kBlock) L.tab[k] = tm[k];
+    __syncthreads();
+    for (int it = int(blockIdx.x); it < n_items; it += int(gridDim.x)) {
+        const BatchItem item = items[it];
+        const StageParams sp = params[item.instance];
+        for (int e0 = item.e0; e0 < item.e0 + item.count; e0 += kTileEl) {
+            stage_mfma_tile<P, false, false>(sp, e0, L);
+            __syncthreads();  // the next tile overwrites the staged geometry and traces
+        }
+    }
+}
+
 // p = 3 runs fastest at three waves per SIMD (the register budget that allows), p = 4 at the
 // compiler's own choice (measured on gfx950)
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void stage_mfma_kernel_p3(StageParams sp) {
@@ -354,6 +396,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
     stage_mfma_body<3, true>(sp);
 }
 __global__ __launch_bounds__(kBlock) void stage_mfma_bf16_kernel_p4(StageParams sp) { stage_mfma_body<4, true>(sp); }
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_stage_mfma_kernel_p3(
+    const StageParams* params, const BatchItem* items, int n_items, const float* tab) {
+    batch_stage_mfma_body<3>(params, items, n_items, tab);
+}
+__global__ __launch_bounds__(kBlock) void batch_stage_mfma_kernel_p4(const StageParams* params, const BatchItem* items,
+                                                                     int n_items, const float* tab) {
+    batch_stage_mfma_body<4>(params, items, n_items, tab);
+}
 
 }  // namespace
 
@@ -365,6 +415,18 @@ hipError_t launch_stage_mfma(int p, bool bf16vol, const StageParams& sp, hipStre
     case 7: hipLaunchKernelGGL(stage_mfma_bf16_kernel_p3, grid, block, 0, s, sp); break;
     case 8: hipLaunchKernelGGL(stage_mfma_kernel_p4, grid, block, 0, s, sp); break;
     case 9: hipLaunchKernelGGL(stage_mfma_bf16_kernel_p4, grid, block, 0, s, sp); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_stage_mfma(int p, const StageParams* params, const BatchItem* items, int n_items,
+                                   const void* tab, int grid, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    const float* t = static_cast<const float*>(tab);
+    switch (p) {
+    case 3: hipLaunchKernelGGL(batch_stage_mfma_kernel_p3, dim3(grid), dim3(kBlock), 0, s, params, items, n_items, t); break;
+    case 4: hipLaunchKernelGGL(batch_stage_mfma_kernel_p4, dim3(grid), dim3(kBlock), 0, s, params, items, n_items, t); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

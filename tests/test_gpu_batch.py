@@ -1,6 +1,6 @@
 """Batched instances: the persistent graph-captured batched step must reproduce the
-single-context step of every instance (the same element code; FP64 to 1e-13 relative), for
-any tuning the autotuner may pick, and odd step counts."""
+single-context step of every instance (the same element code; FP64 to 1e-13 relative, FP32 with
+the matrix-core orders to 1e-6), for any tuning the autotuner may pick, and odd step counts."""
 import numpy as np
 import pytest
 
@@ -44,6 +44,26 @@ def test_batched_steps_match_single_context(layout, nsteps):
         ctx.rk3_steps(dt, nsteps)
         ref = ctx.get_state()
         assert np.abs(g - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("layout", ["fused", "per_order"])
+@pytest.mark.parametrize("nsteps", [4, 5])
+def test_batched_fp32_matches_single_context(layout, nsteps):
+    # FP32: orders 3 and 4 run the matrix-core stage in both the batch (tile work items) and
+    # the single context; 1 and 2 the per-lane stage.  The single context chooses its own
+    # storage offset, so the comparison is to FP32 rounding (1e-6 relative after the steps).
+    b, meshes, dts = _make(SPECS, "fp32")
+    b.set_tuning(layout, chunk=(1, 2, 4, 1), blocks_per_cu=1)
+    u0 = [c.get_state() for c in b.instances]
+    b.rk3_steps(nsteps)
+    got = [c.get_state() for c in b.instances]
+    for (v, t), s, u, g, dt in zip(meshes, SPECS, u0, got, dts):
+        ctx = dgx.Context(v, t, s["p"], precision="fp32")
+        assert ctx.info.stage_kernel == (1 if s["p"] >= 3 else 0)
+        ctx.set_state(u)
+        ctx.rk3_steps(dt, nsteps)
+        ref = ctx.get_state()
+        assert np.abs(g - ref).max() < 1e-6 * np.abs(ref).max(), s
 
 
 def test_batched_instance_matches_oracle():
