@@ -14,6 +14,9 @@ One DOF-stage update = one conserved-variable modal coefficient advanced by one 
 Output: one JSON line (rank 0).  Besides the contract keys it carries
   roofline      : dominant kernel (the fused FP32 stage kernel), algorithmic bytes per launch
                   / mean launch duration from HIP events on the launch stream, vs 8 TB/s
+  compute       : the compute side for the same kernel: SIMD issue cycles (VALU + FP32 MFMA,
+                  which serialise on gfx950) from the committed SQ counters over the live
+                  launch time, and the FP32 matrix-product rate vs 157.3 TFLOP/s
   cpu_baseline  : the FP64 CPU oracle (port of the reference loop) on OMP_NUM_THREADS threads
                   (and on one thread, single_thread_value), timed on
                   a bounded sample of the same mesh
@@ -84,9 +87,9 @@ def cpu_baseline(verts, tris, p, u0, dt, steps, threads):
     return dof * steps / secs1, secs1, dof * 4 * steps / secsn, secsn
 
 
-def load_traffic(workload, kernel=None):
-    """HBM bytes per launch of `kernel` from the committed counter summary, or None when the
-    summary is missing or was measured on another kernel."""
+def load_counters(workload, kernel=None):
+    """Per-launch counter summary (HBM bytes, instruction counts) of `kernel` committed under
+    profiles/, or None when it is missing or was measured on another kernel."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
@@ -94,7 +97,35 @@ def load_traffic(workload, kernel=None):
         t = json.load(f)
     if kernel is not None and t.get("kernel") != kernel:
         return None
-    return t.get("bytes_per_launch")
+    return t
+
+
+def load_traffic(workload, kernel=None):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    t = load_counters(workload, kernel)
+    return t.get("bytes_per_launch") if t else None
+
+
+CLOCK_HZ = 2.4e9           # MI355X peak engine clock
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def compute_fraction(workload, kernel, launch_ms):
+    """Compute side of the roofline from the committed SQ counters and the live launch time.
+    On gfx950 the FP32 matrix products and the VALU share one issue path (they serialise), so
+    the compute bound is the SIMD issue time: VALU instructions x 4 cycles plus the measured
+    FP32-MFMA busy cycles, per SIMD, over the launch time at the peak clock."""
+    t = load_counters(workload, kernel)
+    if not t or "valu_insts" not in t:
+        return None
+    simds = t["simds"]
+    cycles = (4.0 * t["valu_insts"] + t["mfma_busy_cycles"]) / simds
+    launch_s = launch_ms * 1e-3
+    mfma_tflops = t["mfma_insts"] * 2048.0 / launch_s / 1e12
+    return {"bound": "simd issue (VALU + FP32 MFMA)", "issue_cycles_per_simd": cycles,
+            "issue_floor_ms": cycles / CLOCK_HZ * 1e3, "frac": cycles / (launch_s * CLOCK_HZ),
+            "mfma_tflops": mfma_tflops, "mfma_peak_tflops": FP32_MFMA_PEAK_TFLOPS,
+            "mfma_frac": mfma_tflops / FP32_MFMA_PEAK_TFLOPS}
 
 
 def main():
@@ -163,6 +194,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload, kname),
                      "kernel": kname, "launch_ms": stage_ms,
                      "alg_bytes_per_launch": bytes_launch},
+        "compute": compute_fraction(workload, kname, stage_ms),
     }
 
     if not args.no_fp64:
@@ -231,12 +263,18 @@ def bench_ns(args):
     stage_ms, _ = ctx.rk3_profile(dt, 20)
     K, Np = ctx.K, ctx.Np
     bytes_launch = ctx.info.bytes_per_stage_alg
+    # one RK stage = the face-trace launch + the stage launch on the matrix cores
+    kname = ("ns_trace_quirk_kernel_p%d+ns_stage_mfma_kernel_p%d" % (p, p) if ctx.info.stage_kernel == 1
+             else "stage_kernel<%d,float,NS>" % p)
     _line("NS RHS DOF-stage updates/s (FP32)", 12.0 * Np * K * steps / secs, "DOF-stage updates/s", steps,
           args.warmup, secs / steps * 1e3, "f32",
           {"workload": f"ns_taylor_green_n{n}_p{p}", "elements": K, "order": p, "mu": 1 / 1600, "dt": dt},
           roofline={"bound": "hbm", "achieved": bytes_launch / (stage_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": bytes_launch / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                    "traffic": load_traffic(f"ns_taylor_green_n{n}_p{p}"), "launch_ms": stage_ms})
+                    "traffic": load_traffic(f"ns_taylor_green_n{n}_p{p}", kname), "kernel": kname,
+                    "launch_ms": stage_ms, "alg_bytes_per_launch": bytes_launch},
+          # NS p = 4 sits near the ridge: the compute side beside the bandwidth side
+          compute=compute_fraction(f"ns_taylor_green_n{n}_p{p}", kname, stage_ms))
 
 
 def bench_adjoint(args):
