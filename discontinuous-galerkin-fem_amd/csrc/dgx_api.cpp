@@ -325,7 +325,15 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
     // (DGX_STAGE_KERNEL=lane keeps the one-element-per-lane kernel for FP32)
     const char* kv = std::getenv("DGX_STAGE_KERNEL");
     const bool lane_only = kv && std::string(kv) == "lane" && precision == DGX_FP32;
-    if (precision != DGX_FP64 && c.physics == DGX_EULER && c.rule == 0 && mfma_order(c.p) && !lane_only) {
+    // the matrix-core kernels address every buffer through 32-bit byte offsets: the largest one
+    // (state, geometry, or the NS face-point records) must stay below 2 GiB
+    const double big = std::max({16.0 * c.np, 8.0 * kGeoRows, c.physics == DGX_NAVIER_STOKES ? 96.0 * c.nfq : 0.0}) * c.Kp;
+    const bool fits32 = big < 2147483648.0;
+    if (4.0 * c.np * c.Kp >= 2147483648.0)
+        throw Fail(DGX_E_UNSUPPORTED, "mesh too large: state indices exceed 32 bits");
+    if (precision == DGX_FP32_BF16VOL && !fits32)
+        throw Fail(DGX_E_UNSUPPORTED, "mesh too large for the BF16-input volume variant (2 GiB buffers)");
+    if (precision != DGX_FP64 && c.physics == DGX_EULER && c.rule == 0 && mfma_order(c.p) && !lane_only && fits32) {
         const bool bf16 = precision == DGX_FP32_BF16VOL;
         std::vector<float> m;
         build_mfma_tables(c.p, tL, m, bf16);
@@ -337,7 +345,8 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         build_mfma_vjp_tables(c.p, tR, m);
         c.upload(c.tabVR, m.data(), m.size() * sizeof(float));
     }
-    if (precision == DGX_FP32 && c.physics == DGX_NAVIER_STOKES && c.rule == 0 && mfma_order(c.p) && !lane_only) {
+    if (precision == DGX_FP32 && c.physics == DGX_NAVIER_STOKES && c.rule == 0 && mfma_order(c.p) && !lane_only &&
+        fits32) {
         std::vector<float> m;
         build_mfma_ns_tables(c.p, tL, false, m);
         c.upload(c.tabML, m.data(), m.size() * sizeof(float));

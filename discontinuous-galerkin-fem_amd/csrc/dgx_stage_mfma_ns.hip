@@ -95,26 +95,34 @@ __device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
         for (int s = 0; s < R; ++s) c = mfma16x16x4(lt[(S::T_EF + s) * 64], b[v][s], c);
         Uf[v] = c;
     }
-    // face gradients: g[v][c] = sum_{d,k} A[d][k] (FG[(c*2+d)*2+k] u)_v; without the reference's
-    // read only the d = c slots are populated
+    // face gradients: g[v][c] = sum_{d,k} FG[(c*2+d)*2+k] (A[d][k] u)_v -- the element's
+    // dxi_dx entry scales the operand (one column per element), so each (variable, component)
+    // accumulates in a single product chain; without the reference's read only the d = c slots
+    // are populated
     v4f gr[4][2];
 #pragma unroll
     for (int v = 0; v < 4; ++v) gr[v][0] = gr[v][1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int slot = 0; slot < 8; ++slot) {
-        const int c = slot >> 2, d = (slot >> 1) & 1, k = slot & 1;
-        if (!QUIRK && d != c) continue;
-        const float wA = float(gA[d * 2 + k]);
+    for (int dk = 0; dk < 4; ++dk) {
+        const int d = dk >> 1, k = dk & 1;
+        const float wA = float(gA[dk]);
+        float bs[4][R];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            v4f x = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int s = 0; s < R; ++s) x = mfma16x16x4(lt[(S::T_FG + slot * R + s) * 64], b[v][s], x);
-            gr[v][c] += wA * x;
+            for (int s = 0; s < R; ++s) bs[v][s] = wA * b[v][s];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (!QUIRK && d != c) continue;
+            const int slot = (c * 2 + d) * 2 + k;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int s = 0; s < R; ++s) gr[v][c] = mfma16x16x4(lt[(S::T_FG + slot * R + s) * 64], bs[v][s], gr[v][c]);
         }
     }
     if (!valid) return;
-    FaceRecord* __restrict__ rec = static_cast<FaceRecord*>(sp.rec);
+    const auto r_rec = make_rsrc(sp.rec, unsigned(3 * NFQ) * 32u * uKp);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int fp = mf_fpoint(NFQ, g, r);
@@ -131,10 +139,12 @@ __device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
         viscous_flux(U, gq, gam, mu, kappa, Fv, Gv);
         const float nx = float(face == 0 ? gN[0] : face == 1 ? gN[2] : gN[4]);
         const float ny = float(face == 0 ? gN[1] : face == 1 ? gN[3] : gN[5]);
-        FaceRecord o;
-        o.u = make_float4(U[0], U[1], U[2], U[3]);
-        o.v = make_float4(0.0f, Fv[1] * nx + Gv[1] * ny, Fv[2] * nx + Gv[2] * ny, Fv[3] * nx + Gv[3] * ny);
-        rec[size_t(fp) * uKp + unsigned(e)] = o;
+        const unsigned ro = (unsigned(fp) * uKp + unsigned(e)) * 32u;
+        const v4u xu = {__float_as_uint(U[0]), __float_as_uint(U[1]), __float_as_uint(U[2]), __float_as_uint(U[3])};
+        const v4u xv = {0u, __float_as_uint(Fv[1] * nx + Gv[1] * ny), __float_as_uint(Fv[2] * nx + Gv[2] * ny),
+                        __float_as_uint(Fv[3] * nx + Gv[3] * ny)};
+        __builtin_amdgcn_raw_buffer_store_b128(xu, r_rec, ro, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(xv, r_rec, ro + 16u, 0, 0);
     }
 }
 
@@ -159,7 +169,14 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
     const float kappa = float(sp.mu * sp.gamma / (sp.prandtl * (sp.gamma - 1.0)));
     const float ur[4] = {float(sp.uref[0]), float(sp.uref[1]), float(sp.uref[2]), float(sp.uref[3])};
     const float* __restrict__ tm = static_cast<const float*>(sp.tabm);
-    const FaceRecord* __restrict__ rec = static_cast<const FaceRecord*>(sp.rec);
+    const auto r_rec = make_rsrc(sp.rec, unsigned(3 * NFQ) * 32u * uKp);
+    auto load_record = [&](unsigned idx) {
+        FaceRecord r;
+        const v4u a = load_x4(r_rec, idx * 32u), b = load_x4(r_rec, idx * 32u + 16u);
+        r.u = make_float4(__uint_as_float(a[0]), __uint_as_float(a[1]), __uint_as_float(a[2]), __uint_as_float(a[3]));
+        r.v = make_float4(__uint_as_float(b[0]), __uint_as_float(b[1]), __uint_as_float(b[2]), __uint_as_float(b[3]));
+        return r;
+    };
 
     // ---------------- loads, issued up front ----------------------------------------------
     constexpr int kTabPerThread = (S::TOTAL_M + kBlock - 1) / kBlock;
@@ -198,21 +215,6 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
         const int row = t / EB, el = min(eb0 + t % EB, sp.K - 1);
         geov[k] = t < kGeoRows * EB ? load_f64(r_geo, unsigned(el) * 8u, unsigned(row) * 8u * uKp) : 0.0;
     }
-    // face records of this lane's slots: own, and the neighbour's at the matching point
-    FaceRecord ro[4], rn[4];
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        const int fp = mf_fpoint(NFQ, g, f);
-        const int fpc = fp < 0 ? 0 : fp;
-        const int face = fpc / NFQ, q = fpc % NFQ;
-        ro[f] = rec[size_t(fpc) * uKp + unsigned(ec)];
-        const uint32_t c = (code >> (4 * face)) & 15u;
-        const int fn = int(c & 3u);
-        const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
-        const bool interior = fn != int(kFaceBoundary) && fp >= 0;
-        const int qq = ((c >> 2) & 1u) ? NFQ - 1 - q : q;
-        rn[f] = rec[size_t(interior ? fn * NFQ + qq : 0) * uKp + unsigned(interior ? nb : ec)];
-    }
 #pragma unroll
     for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
@@ -239,8 +241,25 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
     const float a00 = float(sv * gd[G_A00]), a01 = float(sv * gd[G_A01]);
     const float a10 = float(sv * gd[G_A10]), a11 = float(sv * gd[G_A11]);
 
+    // per face: outward normal, lift scale, and the penalty sigma0 mu (p+1)^2 / max(h, 1e-12)
+    // (navier_stokes_weak_formulation.cpp:275-278), the latter as an FP32 reciprocal
+    const float sig0 = float(sp.sigma0 * sp.mu * (P + 1) * (P + 1));
+    float fnx[3], fny[3], fsf[3], fsg[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        fnx[f] = float(gd[G_N0X + 2 * f]);
+        fny[f] = float(gd[G_N0X + 2 * f + 1]);
+        fsf[f] = residual_mode ? float(gd[G_L0 + f]) : float(gd[G_L0 + f] * rdet);
+        fsg[f] = sig0 * rcp(fmaxf(float(gd[G_L0 + f]), 1e-12f));
+    }
+
     // ---------------- volume ----------------------------------------------------------------
+    // The projection contracts one register slot per product, so slot r's products follow its
+    // fluxes directly: one slot of fluxes is live at a time (same accumulation order per
+    // variable as slot-by-slot after all fluxes).
     v4f acc[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     {
         v4f U[4], GX[4], GY[4];
 #pragma unroll
@@ -256,14 +275,8 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             GX[v] = cx;
             GY[v] = cy;
         }
-        v4f Fh[4], Gh[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            if (r >= RQ) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) { Fh[v][r] = 0.0f; Gh[v][r] = 0.0f; }
-                continue;
-            }
+        for (int r = 0; r < RQ; ++r) {
             const float Uq[4] = {U[0][r] + ur[0], U[1][r] + ur[1], U[2][r] + ur[2], U[3][r] + ur[3]};
             const float ri = rcp(Uq[0]);
             const float uu = Uq[1] * ri, vv = Uq[2] * ri;
@@ -277,25 +290,36 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             }
             float Fv[4], Gv[4];
             viscous_flux(Uq, gq, gam, mu, kappa, Fv, Gv);
-            Fh[0][r] = Uq[0] * c1 - (a00 * Fv[0] + a10 * Gv[0]);
-            Gh[0][r] = Uq[0] * c2 - (a01 * Fv[0] + a11 * Gv[0]);
-            Fh[1][r] = Uq[1] * c1 + p * a00 - (a00 * Fv[1] + a10 * Gv[1]);
-            Gh[1][r] = Uq[1] * c2 + p * a01 - (a01 * Fv[1] + a11 * Gv[1]);
-            Fh[2][r] = Uq[2] * c1 + p * a10 - (a00 * Fv[2] + a10 * Gv[2]);
-            Gh[2][r] = Uq[2] * c2 + p * a11 - (a01 * Fv[2] + a11 * Gv[2]);
-            Fh[3][r] = (Uq[3] + p) * c1 - (a00 * Fv[3] + a10 * Gv[3]);
-            Gh[3][r] = (Uq[3] + p) * c2 - (a01 * Fv[3] + a11 * Gv[3]);
-        }
+            float fh[4], gh[4];
+            fh[0] = Uq[0] * c1 - (a00 * Fv[0] + a10 * Gv[0]);
+            gh[0] = Uq[0] * c2 - (a01 * Fv[0] + a11 * Gv[0]);
+            fh[1] = Uq[1] * c1 + p * a00 - (a00 * Fv[1] + a10 * Gv[1]);
+            gh[1] = Uq[1] * c2 + p * a01 - (a01 * Fv[1] + a11 * Gv[1]);
+            fh[2] = Uq[2] * c1 + p * a10 - (a00 * Fv[2] + a10 * Gv[2]);
+            gh[2] = Uq[2] * c2 + p * a11 - (a01 * Fv[2] + a11 * Gv[2]);
+            fh[3] = (Uq[3] + p) * c1 - (a00 * Fv[3] + a10 * Gv[3]);
+            gh[3] = (Uq[3] + p) * c2 - (a01 * Fv[3] + a11 * Gv[3]);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int t = 0; t < RQ; ++t) {
-                c = mfma16x16x4(lt[(S::M_PX + t) * 64], Fh[v][t], c);
-                c = mfma16x16x4(lt[(S::M_PY + t) * 64], Gh[v][t], c);
+            for (int v = 0; v < 4; ++v) {
+                acc[v] = mfma16x16x4(lt[(S::M_PX + r) * 64], fh[v], acc[v]);
+                acc[v] = mfma16x16x4(lt[(S::M_PY + r) * 64], gh[v], acc[v]);
             }
-            acc[v] = c;
         }
+    }
+    // face records of this lane's slots: its own, and the neighbour's at the matching point
+    FaceRecord ro[4], rn[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int fp = mf_fpoint(NFQ, g, f);
+        const int fpc = fp < 0 ? 0 : fp;
+        const int face = fpc / NFQ, q = fpc % NFQ;
+        ro[f] = load_record(unsigned(fpc) * uKp + unsigned(ec));
+        const uint32_t c = (code >> (4 * face)) & 15u;
+        const int fn = int(c & 3u);
+        const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
+        const bool interior = fn != int(kFaceBoundary) && fp >= 0;
+        const int qq = ((c >> 2) & 1u) ? NFQ - 1 - q : q;
+        rn[f] = load_record(unsigned(interior ? fn * NFQ + qq : 0) * uKp + unsigned(interior ? nb : ec));
     }
 
     // ---------------- faces --------------------------------------------------------------
@@ -311,13 +335,12 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
         const bool boundary = (c & 3u) == kFaceBoundary;
         const bool isL = (c >> 3) & 1u;
         const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
-        const float nx = float(gd[G_N0X + 2 * face]), ny = float(gd[G_N0X + 2 * face + 1]);
-        const double len = gd[G_L0 + face];
-        const float sf = residual_mode ? float(len) : float(len * rdet);
+        const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
+        const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
+        const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
         const float Uo[4] = {ro[f].u.x, ro[f].u.y, ro[f].u.z, ro[f].u.w};
         const float vo[4] = {ro[f].v.x, ro[f].v.y, ro[f].v.z, ro[f].v.w};
-        // penalty sigma0 mu (p+1)^2 / max(h, 1e-12) (navier_stokes_weak_formulation.cpp:275-278)
-        const float sigma = float(sp.sigma0 * sp.mu * (P + 1) * (P + 1) / fmax(len, 1e-12));
+        const float sigma = face == 0 ? fsg[0] : face == 1 ? fsg[1] : fsg[2];
         float Un[4], Fn[4];
         float sfe = sf, sgn = 1.0f;
         if (boundary) {
@@ -395,12 +418,24 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void ns_trace_kernel_p3(StageParams sp) { ns_trace_body<3, false>(sp); }
-__global__ __launch_bounds__(kBlock) void ns_trace_kernel_p4(StageParams sp) { ns_trace_body<4, false>(sp); }
-__global__ __launch_bounds__(kBlock) void ns_trace_quirk_kernel_p3(StageParams sp) { ns_trace_body<3, true>(sp); }
-__global__ __launch_bounds__(kBlock) void ns_trace_quirk_kernel_p4(StageParams sp) { ns_trace_body<4, true>(sp); }
-__global__ __launch_bounds__(kBlock) void ns_stage_mfma_kernel_p3(StageParams sp) { ns_stage_body<3>(sp); }
-__global__ __launch_bounds__(kBlock) void ns_stage_mfma_kernel_p4(StageParams sp) { ns_stage_body<4>(sp); }
+// four waves per SIMD fit the face-trace kernels without spilling
+#define DGX_TRACE_KERNEL(NAME, P, QUIRK)                                                                  \
+    __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void NAME(StageParams sp) { \
+        ns_trace_body<P, QUIRK>(sp);                                                                        \
+    }
+DGX_TRACE_KERNEL(ns_trace_kernel_p3, 3, false)
+DGX_TRACE_KERNEL(ns_trace_kernel_p4, 4, false)
+DGX_TRACE_KERNEL(ns_trace_quirk_kernel_p3, 3, true)
+DGX_TRACE_KERNEL(ns_trace_quirk_kernel_p4, 4, true)
+#undef DGX_TRACE_KERNEL
+// the stage kernels fit three waves per SIMD without spilling when asked to (the default
+// schedule keeps more registers and runs two)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void ns_stage_mfma_kernel_p3(StageParams sp) {
+    ns_stage_body<3>(sp);
+}
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void ns_stage_mfma_kernel_p4(StageParams sp) {
+    ns_stage_body<4>(sp);
+}
 
 }  // namespace
 
