@@ -217,7 +217,16 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int eb0, 
         Uf[v] = d;
     }
 
-    // ---------------- contravariant volume fluxes -----------------------------------------
+    // ---------------- contravariant volume fluxes and their projection -------------------------
+    // The projection contracts one register slot per product, so slot r's two products can follow
+    // its fluxes directly, keeping one slot of fluxes live (the accumulation order per variable is
+    // the slot order either way): p = 4 then fits four waves per SIMD instead of three; p = 3
+    // times best with the products after all slots (measured).  bfloat16: one product covers
+    // all four slots.
+    constexpr bool kSlotProjection = !BF16 && P == 4;
+    v4f acc[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     v4f Fh[4], Gh[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -236,30 +245,34 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int eb0, 
         Fh[1][r] = U1 * c1 + p * a00;  Gh[1][r] = U1 * c2 + p * a01;
         Fh[2][r] = U2 * c1 + p * a10;  Gh[2][r] = U2 * c2 + p * a11;
         Fh[3][r] = (U3 + p) * c1;      Gh[3][r] = (U3 + p) * c2;
-    }
-    __syncthreads();  // neighbour traces complete
-
-    // The projection below and the face fluxes after it are independent, and both sit in one
-    // basic block, so the scheduler can issue the face arithmetic while the matrix cores work.
-    v4f acc[4];
+        if constexpr (kSlotProjection) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
-        if constexpr (BF16) {
-            // projection of bfloat16 fluxes on bfloat16 basis gradients, FP32 accumulation
-            const unsigned* xb = reinterpret_cast<const unsigned*>(&s_tab[S::T_XB + lane]);
-            const unsigned* yb = reinterpret_cast<const unsigned*>(&s_tab[S::T_YB + lane]);
-            c = mfma16x16x16bf16(xb[0], xb[64], pack_bf16x2(Fh[v][0], Fh[v][1]), pack_bf16x2(Fh[v][2], Fh[v][3]), c);
-            c = mfma16x16x16bf16(yb[0], yb[64], pack_bf16x2(Gh[v][0], Gh[v][1]), pack_bf16x2(Gh[v][2], Gh[v][3]), c);
-        } else {
-#pragma unroll
-            for (int t = 0; t < RQ; ++t) {
-                c = mfma16x16x4(tPX[t], Fh[v][t], c);
-                c = mfma16x16x4(tPY[t], Gh[v][t], c);
+            for (int v = 0; v < 4; ++v) {
+                acc[v] = mfma16x16x4(tPX[r], Fh[v][r], acc[v]);
+                acc[v] = mfma16x16x4(tPY[r], Gh[v][r], acc[v]);
             }
         }
-        acc[v] = c;
     }
+    if constexpr (!BF16 && !kSlotProjection) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int t = 0; t < RQ; ++t) {
+                acc[v] = mfma16x16x4(tPX[t], Fh[v][t], acc[v]);
+                acc[v] = mfma16x16x4(tPY[t], Gh[v][t], acc[v]);
+            }
+    }
+    if constexpr (BF16) {
+        // projection of bfloat16 fluxes on bfloat16 basis gradients, FP32 accumulation
+        const unsigned* xb = reinterpret_cast<const unsigned*>(&s_tab[S::T_XB + lane]);
+        const unsigned* yb = reinterpret_cast<const unsigned*>(&s_tab[S::T_YB + lane]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            acc[v] = mfma16x16x16bf16(xb[0], xb[64], pack_bf16x2(Fh[v][0], Fh[v][1]), pack_bf16x2(Fh[v][2], Fh[v][3]), acc[v]);
+            acc[v] = mfma16x16x16bf16(yb[0], yb[64], pack_bf16x2(Gh[v][0], Gh[v][1]), pack_bf16x2(Gh[v][2], Gh[v][3]), acc[v]);
+        }
+    }
+    __syncthreads();  // neighbour traces complete
 
     // ---------------- faces: slot (g, f) = face f, point g (f < 3); face g, point 4 (f = 3) ---
     // Interior faces first, without branches (a boundary slot computes a value from stale LDS
@@ -385,23 +398,27 @@ __device__ __forceinline__ void batch_stage_mfma_body(const StageParams* __restr
     }
 }
 
-// p = 3 runs fastest at three waves per SIMD (the register budget that allows), p = 4 at the
-// compiler's own choice (measured on gfx950)
+// waves per SIMD as measured best on gfx950: p = 3 is asked for three (it gets four), p = 4
+// for four
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void stage_mfma_kernel_p3(StageParams sp) {
     stage_mfma_body<3, false>(sp);
 }
-__global__ __launch_bounds__(kBlock) void stage_mfma_kernel_p4(StageParams sp) { stage_mfma_body<4, false>(sp); }
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void stage_mfma_kernel_p4(StageParams sp) {
+    stage_mfma_body<4, false>(sp);
+}
 // bfloat16 volume products (precision DGX_FP32_BF16VOL)
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void stage_mfma_bf16_kernel_p3(StageParams sp) {
     stage_mfma_body<3, true>(sp);
 }
-__global__ __launch_bounds__(kBlock) void stage_mfma_bf16_kernel_p4(StageParams sp) { stage_mfma_body<4, true>(sp); }
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void stage_mfma_bf16_kernel_p4(StageParams sp) {
+    stage_mfma_body<4, true>(sp);
+}
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_stage_mfma_kernel_p3(
     const StageParams* params, const BatchItem* items, int n_items, const float* tab) {
     batch_stage_mfma_body<3>(params, items, n_items, tab);
 }
-__global__ __launch_bounds__(kBlock) void batch_stage_mfma_kernel_p4(const StageParams* params, const BatchItem* items,
-                                                                     int n_items, const float* tab) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_stage_mfma_kernel_p4(
+    const StageParams* params, const BatchItem* items, int n_items, const float* tab) {
     batch_stage_mfma_body<4>(params, items, n_items, tab);
 }
 
