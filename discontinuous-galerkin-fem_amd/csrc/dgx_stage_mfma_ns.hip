@@ -323,32 +323,70 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
     }
 
     // ---------------- faces --------------------------------------------------------------
+    // Interior faces first, for every slot and without branches (a boundary slot computes a
+    // finite value from its own record that the pass after it replaces; a padding slot is
+    // selected away); then the boundary slots, only in waves that have one.
     v4f Hs[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Hs[v] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    bool any_boundary = false;
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         const int fp = mf_fpoint(NFQ, g, f);
-        if (fp < 0) continue;
-        const int face = fp / NFQ;
+        const bool slot_valid = fp >= 0;
+        const int face = slot_valid ? fp / NFQ : 0;
         const uint32_t c = (code >> (4 * face)) & 15u;
-        const bool boundary = (c & 3u) == kFaceBoundary;
         const bool isL = (c >> 3) & 1u;
-        const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
+        any_boundary |= slot_valid && (c & 3u) == kFaceBoundary;
         const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
         const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
         const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
+        const float sigma = face == 0 ? fsg[0] : face == 1 ? fsg[1] : fsg[2];
         const float Uo[4] = {ro[f].u.x, ro[f].u.y, ro[f].u.z, ro[f].u.w};
         const float vo[4] = {ro[f].v.x, ro[f].v.y, ro[f].v.z, ro[f].v.w};
-        const float sigma = face == 0 ? fsg[0] : face == 1 ? fsg[1] : fsg[2];
-        float Un[4], Fn[4];
-        float sfe = sf, sgn = 1.0f;
-        if (boundary) {
-            // ghost state and viscous boundary flux Fv.n - sigma (U_bc - U) (:249-264)
+        const float Un[4] = {rn[f].u.x, rn[f].u.y, rn[f].u.z, rn[f].u.w};
+        const float vn[4] = {rn[f].v.x, rn[f].v.y, rn[f].v.z, rn[f].v.w};
+        // canonical orientation: A side = the L element, normal = the L-side outward normal;
+        // Fn = 1/2 Fv_L.n - 1/2 Fv_R.n - sigma (U_R - U_L) (:192-196) with the own Fv.n_L =
+        // sgn vo and the neighbour's = -sgn vn, so both sides form identical numbers
+        const float sgn = isL ? 1.0f : -1.0f;
+        float Aq[4], Bq[4], H[4], Fn[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Aq[v] = isL ? Uo[v] : Un[v];
+            Bq[v] = isL ? Un[v] : Uo[v];
+            const float fo = sgn * vo[v], fnb = -sgn * vn[v];
+            const float fL = isL ? fo : fnb, fR = isL ? fnb : fo;
+            Fn[v] = 0.5f * fL - 0.5f * fR - sigma * (Bq[v] - Aq[v]);
+        }
+        rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
+        const float hs = sgn * sf;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Hs[v][f] = slot_valid ? hs * (H[v] + Fn[v]) : 0.0f;
+    }
+    if (__builtin_amdgcn_ballot_w64(any_boundary) != 0) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int fp = mf_fpoint(NFQ, g, f);
+            if (fp < 0) continue;
+            const int face = fp / NFQ;
+            const uint32_t c = (code >> (4 * face)) & 15u;
+            if ((c & 3u) != kFaceBoundary) continue;
+            const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
             const double* bcr = sp.bc + 5 * nb;
-            if (bcr[0] < 0.0) continue;  // no condition: the face contributes nothing
+            if (bcr[0] < 0.0) {  // no condition: the face contributes nothing
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Hs[v][f] = 0.0f;
+                continue;
+            }
+            const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
+            const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
+            const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
+            const float sigma = face == 0 ? fsg[0] : face == 1 ? fsg[1] : fsg[2];
+            const float Uo[4] = {ro[f].u.x, ro[f].u.y, ro[f].u.z, ro[f].u.w};
+            const float vo[4] = {ro[f].v.x, ro[f].v.y, ro[f].v.z, ro[f].v.w};
+            // ghost state and viscous boundary flux Fv.n - sigma (U_bc - U) (:249-264), U_bc the
+            // condition's data taken as conserved except for NO_SLIP
+            float Un[4], Ub[4], H[4];
             ghost_state(Uo, nx, ny, bcr, gam, Un);
-            float Ub[4];
             if (int(bcr[0]) == DGX_BC_NO_SLIP_WALL) {
                 const float Wb[4] = {float(bcr[1]), float(bcr[2]), float(bcr[3]), float(bcr[4])};
                 prim2cons(Wb, gam, Ub);
@@ -356,32 +394,10 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Ub[v] = float(bcr[1 + v]);
             }
+            rusanov(Uo, Un, nx, ny, gam, H);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Fn[v] = vo[v] - sigma * (Ub[v] - Uo[v]);
-        } else {
-            sgn = isL ? 1.0f : -1.0f;
-            Un[0] = rn[f].u.x; Un[1] = rn[f].u.y; Un[2] = rn[f].u.z; Un[3] = rn[f].u.w;
-            const float vn[4] = {rn[f].v.x, rn[f].v.y, rn[f].v.z, rn[f].v.w};
-            // Fn = 1/2 Fv_L.n - 1/2 Fv_R.n - sigma (U_R - U_L), n the L-side normal (:192-196):
-            // own Fv.n_L = sgn vo, neighbour's Fv.n_L = -sgn vn
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const float fo = sgn * vo[v], fnb = -sgn * vn[v];
-                const float fL = isL ? fo : fnb, fR = isL ? fnb : fo;
-                const float UL = isL ? Uo[v] : Un[v], UR = isL ? Un[v] : Uo[v];
-                Fn[v] = 0.5f * fL - 0.5f * fR - sigma * (UR - UL);
-            }
+            for (int v = 0; v < 4; ++v) Hs[v][f] = sf * (H[v] + (vo[v] - sigma * (Ub[v] - Uo[v])));
         }
-        float Aq[4], Bq[4], H[4];
-        const bool own_is_a = isL || boundary;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            Aq[v] = own_is_a ? Uo[v] : Un[v];
-            Bq[v] = own_is_a ? Un[v] : Uo[v];
-        }
-        rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Hs[v][f] = sgn * sfe * (H[v] + Fn[v]);
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v)
