@@ -33,9 +33,14 @@ template <int P, bool BF16>
 struct MfmaLds {
     using S = MfmaShape<P>;
     static constexpr int NTAB = BF16 ? S::TOTAL_BF : S::TOTAL;
+    // p = 3: u^n of the second and third RK stages is prefetched into LDS by the tile's first
+    // loads (16 B per lane and mode slot, lane-ordered), so the output does not wait a memory
+    // round trip; p = 4 keeps the late register load (the LDS would cost a block per CU)
+    static constexpr bool UN_LDS = P == 3;
     float tab[NTAB];
     double geo[kGeoRows][kTileEl];
     float4 tr[3 * S::NFQ * kTileEl];
+    float4 un[UN_LDS ? 4 * S::R : 1][64];  // [wave * R + slot][lane]
 };
 
 // One 64-element tile starting at element eb0.  TABLES: the tile's prologue also stages the
@@ -109,6 +114,18 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int eb0, 
         const v4u x = load_x4(r_uk, slot_off(s, unsigned(ec)));
 #pragma unroll
         for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
+    }
+    // u^n into LDS (p = 3; see MfmaLds): wave-uniform destination, lane L's 16 bytes at 16 L
+    const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
+    if constexpr (MfmaLds<P, BF16>::UN_LDS) {
+        if (rk_blend) {
+            const auto r_un = make_rsrc(sp.un, state_bytes);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    r_un, (__attribute__((address_space(3))) void*)&L.un[wu * R + r][0], 16, slot_off(r, unsigned(ec)), 0,
+                    0, 0);
+        }
     }
     // geometry rows of the block: wave w loads rows w + 4k of the block's elements (rows past
     // the last read past the resource: zeros, no traffic)
@@ -340,19 +357,34 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int eb0, 
         for (int t = 0; t < RF; ++t) acc[v] = mfma16x16x4(tLF[t], Hs[v][t], acc[v]);
 
     // ---------------- output ---------------------------------------------------------------
+    if constexpr (MfmaLds<P, BF16>::UN_LDS) {
+        // every wave drains its u^n prefetch here, also lanes and waves past the mesh end, so no
+        // LDS transfer is outstanding when a wave leaves the tile
+        if (rk_blend) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
     if (!valid) return;  // (no barrier follows inside the tile)
     const auto r_out = make_rsrc(sp.out, state_bytes);
     const float dt = float(sp.dt), ca = float(sp.ca), cb = float(sp.cb);
     // u^n for the second and third RK stages
-    const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
     float unv[4][R];
     if (rk_blend) {
-        const auto r_un = make_rsrc(sp.un, state_bytes);
+        if constexpr (MfmaLds<P, BF16>::UN_LDS) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const v4u x = load_x4(r_un, slot_off(r, unsigned(e)));
+            for (int r = 0; r < R; ++r) {
+                const float4 x = L.un[wu * R + r][lane];
+                unv[0][r] = x.x; unv[1][r] = x.y; unv[2][r] = x.z; unv[3][r] = x.w;
+            }
+        } else {
+            const auto r_un = make_rsrc(sp.un, state_bytes);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) unv[v][r] = __uint_as_float(x[v]);
+            for (int r = 0; r < R; ++r) {
+                const v4u x = load_x4(r_un, slot_off(r, unsigned(e)));
+#pragma unroll
+                for (int v = 0; v < 4; ++v) unv[v][r] = __uint_as_float(x[v]);
+            }
         }
     }
 #pragma unroll
