@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -535,6 +537,97 @@ dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
                 xy[((size_t)e * c->nq + q) * 2 + 1] = n0 * V[1] + n1 * V[3] + n2 * V[5];
             }
         }
+    });
+}
+
+// Legacy-VTK output of an Euler state (VTKWriter::write_euler_solution and its helpers,
+// S/utils/vtk_writer.cpp:72-176,232-311,323-489): every element subdivided at
+// `resolution` (reference points (i/r, j/r), j outer, sub-triangles lower-left then
+// upper-right), the DG solution evaluated there, primitive and derived fields plus the
+// conserved ones as point data, ASCII, 10 significant digits in scientific notation.
+dgx_status dgx_write_vtk(dgx_ctx* c, const double* u_or_null, const char* filename, double gamma, int32_t resolution) {
+    return guarded([&] {
+        if (!c || !filename) throw std::invalid_argument("null argument");
+        if (resolution < 1) throw std::invalid_argument("resolution factor must be at least 1");
+        const int K = c->K, np = c->np, r = resolution, n1 = r + 1;
+        std::vector<double> u;
+        if (!u_or_null) {
+            u.resize((size_t)K * np * 4);
+            if (dgx_get_state(c, u.data()) != DGX_OK) throw std::runtime_error(g_error);
+            u_or_null = u.data();
+        }
+        std::vector<double> rp;  // reference subdivision points
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1 - j; ++i) {
+                rp.push_back(double(i) / r);
+                rp.push_back(double(j) / r);
+            }
+        const int npe = (int)rp.size() / 2;
+        auto tri_index = [&](int i, int j) {
+            int idx = 0;
+            for (int jj = 0; jj < j; ++jj) idx += n1 - jj;
+            return idx + i;
+        };
+        std::vector<int> sub;  // reference sub-triangles
+        for (int j = 0; j < r; ++j)
+            for (int i = 0; i < r - j; ++i) {
+                sub.insert(sub.end(), {tri_index(i, j), tri_index(i + 1, j), tri_index(i, j + 1)});
+                if (i < r - j - 1) sub.insert(sub.end(), {tri_index(i + 1, j), tri_index(i + 1, j + 1), tri_index(i, j + 1)});
+            }
+        const int nse = (int)sub.size() / 3;
+        TriBasis B(c->basis, c->p);
+        std::vector<double> phi((size_t)npe * np);
+        for (int q = 0; q < npe; ++q) B.values(rp[2 * q], rp[2 * q + 1], &phi[(size_t)q * np]);
+        const size_t npts = (size_t)K * npe;
+        std::vector<double> xy(npts * 2), U(npts * 4, 0.0);
+        for (int e = 0; e < K; ++e) {
+            const double* V = &c->h_verts[(size_t)e * 6];
+            for (int q = 0; q < npe; ++q) {
+                const double xi = rp[2 * q], eta = rp[2 * q + 1], n0 = 1.0 - xi - eta;
+                const size_t k = (size_t)e * npe + q;
+                xy[2 * k] = n0 * V[0] + xi * V[2] + eta * V[4];
+                xy[2 * k + 1] = n0 * V[1] + xi * V[3] + eta * V[5];
+                for (int i = 0; i < np; ++i)
+                    for (int v = 0; v < 4; ++v) U[4 * k + v] += u_or_null[((size_t)e * np + i) * 4 + v] * phi[(size_t)q * np + i];
+            }
+        }
+        std::ofstream f(std::string(filename) + ".vtk");
+        if (!f.is_open()) throw std::runtime_error(std::string("Failed to open file: ") + filename + ".vtk");
+        f << "# vtk DataFile Version 3.0\nDGFEM Euler Solution\nASCII\nDATASET UNSTRUCTURED_GRID\n";
+        f << "POINTS " << npts << " double\n" << std::scientific << std::setprecision(10);
+        for (size_t k = 0; k < npts; ++k) f << xy[2 * k] << " " << xy[2 * k + 1] << " 0.0\n";
+        const size_t ncells = (size_t)K * nse;
+        f << "CELLS " << ncells << " " << ncells * 4 << "\n";
+        for (int e = 0; e < K; ++e)
+            for (int t = 0; t < nse; ++t) {
+                const size_t o = (size_t)e * npe;
+                f << 3 << " " << o + sub[3 * t] << " " << o + sub[3 * t + 1] << " " << o + sub[3 * t + 2] << "\n";
+            }
+        f << "CELL_TYPES " << ncells << "\n";
+        for (size_t t = 0; t < ncells; ++t) f << 5 << "\n";
+        f << "POINT_DATA " << npts << "\n" << std::scientific << std::setprecision(10);
+        auto field = [&](const char* name, auto fn) {
+            f << "SCALARS " << name << " double 1\nLOOKUP_TABLE default\n";
+            for (size_t k = 0; k < npts; ++k) {
+                const double rho = U[4 * k], uu = U[4 * k + 1] / rho, vv = U[4 * k + 2] / rho;
+                const double p = (gamma - 1.0) * (U[4 * k + 3] - 0.5 * rho * (uu * uu + vv * vv));
+                f << fn(k, rho, uu, vv, p) << "\n";
+            }
+        };
+        field("density", [](size_t, double rho, double, double, double) { return rho; });
+        field("u_velocity", [](size_t, double, double uu, double, double) { return uu; });
+        field("v_velocity", [](size_t, double, double, double vv, double) { return vv; });
+        field("velocity_magnitude", [](size_t, double, double uu, double vv, double) { return std::sqrt(uu * uu + vv * vv); });
+        field("pressure", [](size_t, double, double, double, double p) { return p; });
+        field("temperature", [](size_t, double rho, double, double, double p) { return p / rho; });
+        field("mach", [&](size_t, double rho, double uu, double vv, double p) {
+            return std::sqrt(uu * uu + vv * vv) / std::sqrt(gamma * p / rho);
+        });
+        field("rho", [&](size_t k, double, double, double, double) { return U[4 * k]; });
+        field("rho_u", [&](size_t k, double, double, double, double) { return U[4 * k + 1]; });
+        field("rho_v", [&](size_t k, double, double, double, double) { return U[4 * k + 2]; });
+        field("E", [&](size_t k, double, double, double, double) { return U[4 * k + 3]; });
+        if (!f) throw std::runtime_error(std::string("Failed to write file: ") + filename + ".vtk");
     });
 }
 

@@ -34,7 +34,7 @@ QUIRK_NS_FACE_GRAD_REFERENCE = 1
 EXPORTS = [
     "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_create", "dgx_destroy", "dgx_last_error",
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
-    "dgx_eval_volume", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
+    "dgx_eval_volume", "dgx_write_vtk", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
     "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
     "dgx_density_range", "dgx_kinetic_energy", "dgx_residual_vjp", "dgx_rhs_vjp",
     "dgx_kinetic_energy_grad", "dgx_rk3_vjp", "dgx_rk3_vjp_profile",
@@ -105,6 +105,7 @@ def load():
     for n in ("dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project", "dgx_eval_volume",
               "dgx_apply_minv", "dgx_conservation_sums", "dgx_density_range", "dgx_kinetic_energy"):
         getattr(L, n).argtypes = [vp, dp]
+    L.dgx_write_vtk.argtypes = [vp, dp, C.c_char_p, C.c_double, C.c_int32]
     L.dgx_residual.argtypes = [vp, dp, dp]
     L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]
     L.dgx_rk3_steps_async.argtypes = [vp, C.c_double, C.c_int32]
@@ -277,6 +278,13 @@ class Context:
         Uq = np.zeros((self.K, self.Nq, 4))
         _check(load().dgx_eval_volume(self._h, _dp(Uq)))
         return Uq
+
+    def write_vtk(self, filename, u=None, gamma=1.4, resolution_factor=3):
+        """Legacy-VTK file `filename`.vtk of the state u ([K][Np][4], default: the current one);
+        VTKWriter::write_euler_solution (S/utils/vtk_writer.cpp:72-176)."""
+        ua = None if u is None else np.ascontiguousarray(u, dtype=np.float64).reshape(self.K, self.Np, 4)
+        _check(load().dgx_write_vtk(self._h, None if ua is None else _dp(ua), str(filename).encode(), float(gamma),
+                                    int(resolution_factor)))
 
     # hot path
     def residual(self, u=None):

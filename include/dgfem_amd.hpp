@@ -187,6 +187,17 @@ private:
     DGContext& ctx_;
 };
 
+// VTKWriter::write_euler_solution (S/utils/vtk_writer.cpp:72-176): `solution` is a frame in
+// the (K, 4 Np) layout that solve() returns; the file is `filename`.vtk.
+struct VTKWriter {
+    static void write_euler_solution(DGContext& ctx, const std::vector<double>& solution, const std::string& filename,
+                                     double gamma = 1.4, int resolution_factor = 3) {
+        if (solution.size() != size_t(ctx.info().n_elements) * ctx.info().n_modes * 4)
+            throw std::invalid_argument("solution has the wrong size");
+        check(dgx_write_vtk(ctx.get(), solution.data(), filename.c_str(), gamma, resolution_factor));
+    }
+};
+
 // CompressibleDGSolverBase / EulerDGSolver / NavierStokesDGSolver.
 class CompressibleDGSolver {
 public:

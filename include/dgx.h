@@ -141,6 +141,14 @@ dgx_status dgx_volume_points(dgx_ctx* ctx, double* xy);
 dgx_status dgx_project(dgx_ctx* ctx, const double* Uq);
 /* Current state evaluated at the volume points, [K][Nq][4]. */
 dgx_status dgx_eval_volume(dgx_ctx* ctx, double* Uq);
+/* Legacy-VTK file `filename`.vtk of an Euler state (host layout [K][Np][4], or the context's
+ * current state when u is NULL): each element subdivided at `resolution_factor`, fields
+ * density, u_velocity, v_velocity, velocity_magnitude, pressure, temperature, mach, rho,
+ * rho_u, rho_v, E at the subdivision points.  Replaces VTKWriter::write_euler_solution
+ * (S/utils/vtk_writer.cpp:72-176; defaults gamma 1.4, resolution 3 in
+ * I/utils/vtk_writer.hpp:78-81). */
+dgx_status dgx_write_vtk(dgx_ctx* ctx, const double* u, const char* filename, double gamma,
+                         int32_t resolution_factor);
 
 /* ---- the hot path -------------------------------------------------------------------------- */
 
