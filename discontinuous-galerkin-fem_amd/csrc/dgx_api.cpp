@@ -540,6 +540,39 @@ dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
     });
 }
 
+// Relative L2 error per primitive field at the volume points against caller-supplied exact
+// values (compute_variable_error, T/solver/shear_flow_accuracy_test.cpp:32-79): sqrt(sum w |J|
+// (W - W_exact)^2 / sum w |J| W_exact^2), or the absolute sqrt(sum w |J| (W - W_exact)^2) when
+// the exact field's norm is below 1e-14 (as there).  FP64 block partials summed in order.
+dgx_status dgx_l2_error(dgx_ctx* c, const double* exact, int32_t exact_is_primitive, double err[4]) {
+    return guarded([&] {
+        if (!c || !exact || !err) throw std::invalid_argument("null argument");
+        TabLayout L(c->p, c->rule);
+        const size_t n = (size_t)c->K * c->nq * 4;
+        DevBuf ex, part;
+        ex.alloc(n * sizeof(double));
+        const int nb = c->red_blocks();
+        part.alloc((size_t)nb * 8 * sizeof(double));
+        hip_check(hipMemcpyAsync(ex.p, exact, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+        hip_check(launch_eval(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>() + L.phiV, c->K, c->Kp,
+                              c->stage.as<double>(), c->uref, c->stream),
+                  "eval");
+        hip_check(launch_l2_error(c->stage.as<double>(), ex.as<double>(), exact_is_primitive ? 1 : 0, c->geo.as<double>(),
+                                  c->qw.as<double>(), c->nq, c->K, c->Kp, c->gamma, part.as<double>(), nb, c->stream),
+                  "l2 error");
+        std::vector<double> h((size_t)nb * 8);
+        hip_check(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        double e2[4] = {0, 0, 0, 0}, n2[4] = {0, 0, 0, 0};
+        for (int b = 0; b < nb; ++b)
+            for (int v = 0; v < 4; ++v) {
+                e2[v] += h[(size_t)b * 8 + v];
+                n2[v] += h[(size_t)b * 8 + 4 + v];
+            }
+        for (int v = 0; v < 4; ++v) err[v] = n2[v] < 1e-14 ? std::sqrt(e2[v]) : std::sqrt(e2[v] / n2[v]);
+    });
+}
+
 // Legacy-VTK output of an Euler state (VTKWriter::write_euler_solution and its helpers,
 // S/utils/vtk_writer.cpp:72-176,232-311,323-489): every element subdivided at
 // `resolution` (reference points (i/r, j/r), j outer, sub-triangles lower-left then

@@ -151,6 +151,53 @@ __global__ __launch_bounds__(kRed) void reduce_kernel(int kind, const Real* __re
         for (int k = 0; k < kMaxOut; ++k) partial[blockIdx.x * kMaxOut + k] = sh[k][0];
 }
 
+// Relative-L2 error of the primitive variables W = (rho, u, v, p) at the volume points
+// (compute_variable_error of the reference's accuracy tests,
+// T/solver/shear_flow_accuracy_test.cpp:32-79, X/taylor_green_test.cpp:44-86): per block, FP64
+// partial sums of w_q |J| (W - W_exact)^2 (entries 0..3) and w_q |J| W_exact^2 (entries 4..7).
+// Uq holds the numerical point values [K][nq][4]; Ex the exact ones, conserved or primitive.
+__device__ __forceinline__ void to_primitive(const double* U, double g, double* W) {
+    W[0] = U[0];
+    W[1] = U[1] / U[0];
+    W[2] = U[2] / U[0];
+    W[3] = (g - 1.0) * (U[3] - 0.5 * U[0] * (W[1] * W[1] + W[2] * W[2]));
+}
+__global__ __launch_bounds__(kRed) void l2err_kernel(const double* __restrict__ Uq, const double* __restrict__ Ex,
+                                                     int ex_prim, const double* __restrict__ geo,
+                                                     const double* __restrict__ qw, int nq, int K, int Kp,
+                                                     double gamma, double* __restrict__ partial) {
+    __shared__ double sh[8][kRed];
+    const int e = blockIdx.x * kRed + threadIdx.x;
+    double val[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (e < K) {
+        const double det = geo[G_DET * Kp + e];
+        for (int q = 0; q < nq; ++q) {
+            const size_t o = ((size_t)e * nq + q) * 4;
+            double W[4], X[4];
+            to_primitive(Uq + o, gamma, W);
+            if (ex_prim)
+                for (int v = 0; v < 4; ++v) X[v] = Ex[o + v];
+            else
+                to_primitive(Ex + o, gamma, X);
+            const double dw = qw[q] * det;
+            for (int v = 0; v < 4; ++v) {
+                const double d = W[v] - X[v];
+                val[v] += d * d * dw;
+                val[4 + v] += X[v] * X[v] * dw;
+            }
+        }
+    }
+    for (int k = 0; k < 8; ++k) sh[k][threadIdx.x] = val[k];
+    __syncthreads();
+    for (int s = kRed / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 8; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) partial[blockIdx.x * 8 + k] = sh[k][0];
+}
+
 inline dim3 grid_for(int K, int b) { return dim3((K + b - 1) / b); }
 
 }  // namespace
@@ -189,6 +236,12 @@ hipError_t launch_eval(int precision, int p, int rule, const void* u, const void
         hipLaunchKernelGGL(eval_kernel<double>, grid_for(K, 256), dim3(256), 0, s, (const double*)u, ph, Uq, np, nq, K, Kp, uref);
     else
         hipLaunchKernelGGL(eval_kernel<float>, grid_for(K, 256), dim3(256), 0, s, (const float*)u, ph, Uq, np, nq, K, Kp, uref);
+    return hipGetLastError();
+}
+
+hipError_t launch_l2_error(const double* Uq, const double* Ex, int ex_prim, const double* geo, const double* qw, int nq,
+                           int K, int Kp, double gamma, double* partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(l2err_kernel, dim3(nblocks), dim3(kRed), 0, s, Uq, Ex, ex_prim, geo, qw, nq, K, Kp, gamma, partial);
     return hipGetLastError();
 }
 

@@ -110,6 +110,9 @@ hipError_t launch_unpack(int precision, int np, const void* u, double* host_layo
 hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u, const void* tab,
                          const double* geo, const double* hmin, const double* qw, int K, int Kp,
                          double gamma, Off4 uref, Off4 off, double* partial, int nblocks, hipStream_t s);
+// Relative-L2 error partial sums (dgx_aux.hip): 8 doubles per block of 256 elements.
+hipError_t launch_l2_error(const double* Uq, const double* Ex, int ex_prim, const double* geo, const double* qw, int nq,
+                           int K, int Kp, double gamma, double* partial, int nblocks, hipStream_t s);
 hipError_t launch_apply_minv(int precision, int np, const double* minv_ref, const double* geo,
                              double* r_dev, int K, int Kp, hipStream_t s);
 // Parameters of one stage-adjoint launch (dgx_vjp.hip, dgx_vjp_mfma.hip).

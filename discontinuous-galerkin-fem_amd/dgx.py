@@ -34,7 +34,7 @@ QUIRK_NS_FACE_GRAD_REFERENCE = 1
 EXPORTS = [
     "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_create", "dgx_destroy", "dgx_last_error",
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
-    "dgx_eval_volume", "dgx_write_vtk", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
+    "dgx_eval_volume", "dgx_l2_error", "dgx_write_vtk", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
     "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
     "dgx_density_range", "dgx_kinetic_energy", "dgx_residual_vjp", "dgx_rhs_vjp",
     "dgx_kinetic_energy_grad", "dgx_rk3_vjp", "dgx_rk3_vjp_profile",
@@ -105,6 +105,7 @@ def load():
     for n in ("dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project", "dgx_eval_volume",
               "dgx_apply_minv", "dgx_conservation_sums", "dgx_density_range", "dgx_kinetic_energy"):
         getattr(L, n).argtypes = [vp, dp]
+    L.dgx_l2_error.argtypes = [vp, dp, C.c_int32, dp]
     L.dgx_write_vtk.argtypes = [vp, dp, C.c_char_p, C.c_double, C.c_int32]
     L.dgx_residual.argtypes = [vp, dp, dp]
     L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]
@@ -278,6 +279,14 @@ class Context:
         Uq = np.zeros((self.K, self.Nq, 4))
         _check(load().dgx_eval_volume(self._h, _dp(Uq)))
         return Uq
+
+    def l2_error(self, exact, primitive=False):
+        """Relative L2 error per primitive field (rho, u, v, p) of the current state at the volume
+        points against exact values there, [K][Nq][4] conserved (or primitive)."""
+        ex = np.ascontiguousarray(exact, dtype=np.float64).reshape(self.K, self.Nq, 4)
+        err = np.zeros(4)
+        _check(load().dgx_l2_error(self._h, _dp(ex), 1 if primitive else 0, _dp(err)))
+        return err
 
     def write_vtk(self, filename, u=None, gamma=1.4, resolution_factor=3):
         """Legacy-VTK file `filename`.vtk of the state u ([K][Np][4], default: the current one);

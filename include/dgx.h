@@ -141,6 +141,13 @@ dgx_status dgx_volume_points(dgx_ctx* ctx, double* xy);
 dgx_status dgx_project(dgx_ctx* ctx, const double* Uq);
 /* Current state evaluated at the volume points, [K][Nq][4]. */
 dgx_status dgx_eval_volume(dgx_ctx* ctx, double* Uq);
+/* Relative L2 error of the primitive fields (rho, u, v, p) of the current state at the volume
+ * points, weighted by w_q |J|, against exact values there ([K][Nq][4], conserved, or primitive
+ * when exact_is_primitive != 0); err[f] = sqrt(sum w|J| (W - W_ex)^2 / sum w|J| W_ex^2), the
+ * absolute root when the exact field's norm is below 1e-14.  Computed on the device in FP64.
+ * Replaces compute_variable_error of T/solver/shear_flow_accuracy_test.cpp:32-79 and
+ * X/taylor_green_test.cpp:44-86. */
+dgx_status dgx_l2_error(dgx_ctx* ctx, const double* exact, int32_t exact_is_primitive, double err[4]);
 /* Legacy-VTK file `filename`.vtk of an Euler state (host layout [K][Np][4], or the context's
  * current state when u is NULL): each element subdivided at `resolution_factor`, fields
  * density, u_velocity, v_velocity, velocity_magnitude, pressure, temperature, mach, rho,

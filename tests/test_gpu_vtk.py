@@ -111,3 +111,65 @@ def test_vtk_output_of_an_explicit_state_and_errors(tmp_path):
         ctx.write_vtk(str(tmp_path / "bad"), resolution_factor=0)
     with pytest.raises(dgx.DGXError):
         ctx.write_vtk(str(tmp_path / "missing_dir" / "x"))
+
+
+# ---------------- relative L2 error norm on the device -----------------------------------------
+
+def _l2_reference(Uq_num, Uq_ex_prim, verts, tris, p):
+    # the reference tests' compute_variable_error restated in numpy: weights w_q |J|
+    _, _, w = R.quad_tri(2 * p)
+    V = verts[tris]
+    det = np.abs((V[:, 1, 0] - V[:, 0, 0]) * (V[:, 2, 1] - V[:, 0, 1])
+                 - (V[:, 2, 0] - V[:, 0, 0]) * (V[:, 1, 1] - V[:, 0, 1]))
+    dw = w[None, :] * det[:, None]
+    rho = Uq_num[..., 0]
+    W = np.stack([rho, Uq_num[..., 1] / rho, Uq_num[..., 2] / rho,
+                  0.4 * (Uq_num[..., 3] - 0.5 * (Uq_num[..., 1] ** 2 + Uq_num[..., 2] ** 2) / rho)], axis=-1)
+    e2 = ((W - Uq_ex_prim) ** 2 * dw[..., None]).sum(axis=(0, 1))
+    n2 = (Uq_ex_prim ** 2 * dw[..., None]).sum(axis=(0, 1))
+    return np.where(n2 < 1e-14, np.sqrt(e2), np.sqrt(e2 / np.where(n2 < 1e-14, 1.0, n2)))
+
+
+def _prim(U):
+    rho = U[..., 0]
+    return np.stack([rho, U[..., 1] / rho, U[..., 2] / rho,
+                     0.4 * (U[..., 3] - 0.5 * (U[..., 1] ** 2 + U[..., 2] ** 2) / rho)], axis=-1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_l2_error_matches_the_reference_formula(precision):
+    verts, tris = R.split_quad_mesh(8)
+    p = 3
+    ora = R.Oracle(verts, tris, p, basis="dubiner")
+    ctx = dgx.Context(verts, tris, p, basis="dubiner", precision=precision)
+    xy = ora.volume_points()
+    exact0 = R.vortex_state(xy)
+    u0 = ora.project(exact0)
+    ctx.set_state(u0)
+    dt = dgx.stable_dt(ctx, 0.3)
+    ctx.rk3_steps(dt, 10)
+    # exact values at the context's own volume points (its point order is its own)
+    exact = R.vortex_state(ctx.volume_points(), t=10 * dt)
+    got = ctx.l2_error(exact)
+    # the same norm from the oracle's evaluation of the GPU state at the oracle's points
+    want = _l2_reference(ora.eval_volume(ctx.get_state()), _prim(R.vortex_state(xy, t=10 * dt)), verts, tris, p)
+    assert np.all(want > 0)
+    assert np.abs(got - want).max() < 1e-10 * want.max(), (got, want)
+    assert np.allclose(ctx.l2_error(_prim(exact), primitive=True), got, rtol=1e-12, atol=0)
+
+
+def test_l2_error_of_a_zero_field_is_absolute():
+    # free stream with v = 0: the v field's exact norm is zero, so its error is the absolute root
+    verts, tris = R.split_quad_mesh(4)
+    ctx = dgx.Context(verts, tris, 2, precision="fp64")
+    xy = ctx.volume_points()
+    Uq = np.zeros(xy.shape[:2] + (4,))
+    Uq[..., 0], Uq[..., 1], Uq[..., 3] = 1.0, 1.0, 1.0 / 0.4 + 0.5
+    ctx.project(Uq)
+    err = ctx.l2_error(Uq)
+    assert np.all(err < 1e-12), err
+    ex = Uq.copy()
+    ex[..., 2] = 0.0
+    ex[..., 1] = 1.1  # u off by 0.1 everywhere: relative error 0.1/1.1 in the u field
+    err = ctx.l2_error(ex)
+    assert abs(err[1] - 0.1 / 1.1) < 1e-12 and err[2] < 1e-12
