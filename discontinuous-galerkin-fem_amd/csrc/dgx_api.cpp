@@ -52,6 +52,49 @@ std::vector<double> invert(std::vector<double> M, int n) {
 
 namespace {
 
+// Boundary data rows (bc_row) of every boundary face point: the condition's type, its value
+// at the point the Euler boundary face reads (with DGX_QUIRK_EULER_BC_POINTS_REFERENCE the
+// reference's row-major read of the column-major storage [x_0..x_n-1, y_0..y_n-1],
+// euler_weak_formulation.cpp:188-194, else the point) and at the point itself (the viscous
+// boundary flux, navier_stokes_weak_formulation.cpp:244-247).  Face points run along the face
+// from vertex f to vertex f+1 at t = (s+1)/2 (S/core/space.cpp:141-164).
+void upload_bc_table(dgx_ctx& c) {
+    const int nfq = c.nfq;
+    const Rule1D fr = gauss_legendre(nfq);
+    const bool q5 = (c.quirks & DGX_QUIRK_EULER_BC_POINTS_REFERENCE) != 0;
+    std::vector<double> tab((size_t)std::max(c.n_bfaces, 1) * nfq * kBcRow, 0.0);
+    std::vector<double> flat(2 * nfq);
+    for (int bf = 0; bf < c.n_bfaces; ++bf) {
+        const auto& B = c.h_bfaces[bf];
+        const double* hv = &c.h_verts[(size_t)B.e * 6];
+        const int a = B.f, b = (B.f + 1) % 3;
+        for (int q = 0; q < nfq; ++q) {
+            const double t = 0.5 * (fr.x[q] + 1.0);
+            flat[q] = (1.0 - t) * hv[2 * a] + t * hv[2 * b];
+            flat[q + nfq] = (1.0 - t) * hv[2 * a + 1] + t * hv[2 * b + 1];
+        }
+        const int type = c.h_bctype[B.bci];
+        const auto fn = c.h_bcfn[B.bci];
+        for (int q = 0; q < nfq; ++q) {
+            double* row = &tab[((size_t)bf * nfq + q) * kBcRow];
+            row[0] = type;
+            double ve[4], vv[4];
+            if (fn.first) {
+                const double xe = q5 ? flat[2 * q] : flat[q], ye = q5 ? flat[2 * q + 1] : flat[q + nfq];
+                fn.first(xe, ye, ve, fn.second);
+                fn.first(flat[q], flat[q + nfq], vv, fn.second);
+            } else {
+                for (int v = 0; v < 4; ++v) ve[v] = vv[v] = c.h_bcval[(size_t)B.bci * 4 + v];
+            }
+            for (int v = 0; v < 4; ++v) {
+                row[1 + v] = ve[v];
+                row[5 + v] = vv[v];
+            }
+        }
+    }
+    c.upload(c.bc, tab.data(), tab.size() * sizeof(double));
+}
+
 void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
     if (!m.vertices || !m.elements || !m.face_neighbors || m.n_elements <= 0 || m.n_vertices <= 0)
         throw std::invalid_argument("mesh descriptor is incomplete");
@@ -246,12 +289,12 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
             face_geom(e, f, nx, ny, len, a1, a2);
             uint32_t cfield = 0;
             if (en < 0) {
-                ++nbf;
                 int bci = fn;
                 if (bci < 0) bci = m.n_bcs;  // "no boundary condition" row
                 if (bci > m.n_bcs) throw std::invalid_argument("boundary face references a missing boundary condition");
                 if (bci < m.n_bcs && m.bc_types && m.bc_types[bci] == DGX_BC_SLIP_WALL) c.has_slip = true;
-                nbr[f * Kp + e] = bci;
+                nbr[f * Kp + e] = nbf++;  // the boundary face's running index
+                c.h_bfaces.push_back({e, f, bci});
                 cfield = kFaceBoundary | (1u << 3);
             } else {
                 if (en >= K || fn < 0 || fn > 2) throw std::invalid_argument("face neighbour out of range");
@@ -289,16 +332,17 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
             code[e] |= cfield << (4 * f);
         }
     c.n_bfaces = nbf;
-    std::vector<double> bct((size_t)(m.n_bcs + 1) * 5, 0.0);
+    c.h_bctype.assign(m.n_bcs + 1, -1);
+    c.h_bcval.assign((size_t)(m.n_bcs + 1) * 4, 0.0);
+    c.h_bcfn.assign(m.n_bcs + 1, {nullptr, nullptr});
     for (int b = 0; b < m.n_bcs; ++b) {
         int t = m.bc_types ? m.bc_types[b] : -1;
         if (t == DGX_BC_PERIODIC)
             throw std::invalid_argument("PERIODIC BC encountered on a boundary face - periodic boundaries should be paired");
         if (t < -1 || t > DGX_BC_NO_SLIP_WALL) throw std::invalid_argument("unknown boundary condition type");
-        bct[b * 5] = t;
-        for (int v = 0; v < 4; ++v) bct[b * 5 + 1 + v] = m.bc_values ? m.bc_values[b * 4 + v] : 0.0;
+        c.h_bctype[b] = t;
+        for (int v = 0; v < 4; ++v) c.h_bcval[b * 4 + v] = m.bc_values ? m.bc_values[b * 4 + v] : 0.0;
     }
-    bct[(size_t)m.n_bcs * 5] = -1.0;
 
     // ---------------- device ----------------
     int ndev = 0;
@@ -311,7 +355,7 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
     c.upload(c.nbr, nbr.data(), nbr.size() * sizeof(int32_t));
     c.upload(c.code, code.data(), code.size() * sizeof(uint32_t));
     c.upload(c.hmin, hmin.data(), hmin.size() * sizeof(double));
-    c.upload(c.bc, bct.data(), bct.size() * sizeof(double));
+    upload_bc_table(c);
     c.upload(c.tab64, tL.data(), tL.size() * sizeof(double));
     c.upload(c.minv, Mi.data(), Mi.size() * sizeof(double));
     c.upload(c.pm, Pm.data(), Pm.size() * sizeof(double));
@@ -537,6 +581,16 @@ dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
                 xy[((size_t)e * c->nq + q) * 2 + 1] = n0 * V[1] + n1 * V[3] + n2 * V[5];
             }
         }
+    });
+}
+
+dgx_status dgx_set_bc_function(dgx_ctx* c, int32_t bc_index, dgx_bc_fn fn, void* user) {
+    return guarded([&] {
+        if (!c || !fn) throw std::invalid_argument("null argument");
+        if (bc_index < 0 || bc_index >= (int)c->h_bctype.size() - 1)
+            throw std::invalid_argument("boundary condition index out of range");
+        c->h_bcfn[bc_index] = {fn, user};
+        upload_bc_table(*c);
     });
 }
 

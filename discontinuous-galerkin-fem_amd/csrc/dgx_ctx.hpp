@@ -90,6 +90,13 @@ struct dgx_ctx {
     uint32_t quirks = 0;
     int device = 0;
     int n_bfaces = 0;
+    // boundary faces (element, local face, condition row; n_bcs = no condition) in the order of
+    // their running index, the conditions' types, constant values and optional functions
+    struct BFace { int e, f, bci; };
+    std::vector<BFace> h_bfaces;
+    std::vector<int> h_bctype;
+    std::vector<double> h_bcval;
+    std::vector<std::pair<dgx_bc_fn, void*>> h_bcfn;
     bool has_slip = false;  // a boundary face uses a slip-wall condition
     double dt = 0.0;        // time step of this instance inside a batch
     double phi0 = 1.0;      // value of the constant basis function

@@ -38,6 +38,19 @@ enum GeoRow : int {
 //   bit 3 this element holds the canonical (L) orientation of the face.
 constexpr uint32_t kFaceBoundary = 3u;
 
+// Boundary data per boundary-face point.  A boundary face's entry in the neighbour array is its
+// running index bf; face point q of it owns the row bc + (bf * nfq + q) * kBcRow:
+//   [0]    condition type (dgx_bc_type; -1: no condition, the face contributes nothing)
+//   [1..4] the Euler ghost data (FAR_FIELD conserved, NO_SLIP primitive) evaluated at the
+//          point the reference's Euler boundary face reads (its transposed read of the face
+//          points, euler_weak_formulation.cpp:188-194, or the point itself)
+//   [5..8] the same condition at the point itself, for the viscous boundary flux
+//          (navier_stokes_weak_formulation.cpp:244-264)
+constexpr int kBcRow = 9;
+__host__ __device__ inline const double* bc_row(const double* bc, int bf, int nfq, int q) {
+    return bc + (size_t(bf) * nfq + q) * kBcRow;
+}
+
 // Table layout (Real), offsets in elements.  "L" tables carry the reference-element mass
 // inverse folded in; "R" tables produce M dU/dt.
 struct TabLayout {
@@ -83,7 +96,7 @@ struct StageParams {
     const void* tabm;       // lane-major tables of the MFMA kernel (FP32 Euler / Navier-Stokes), or null
     const void* tabt;       // face-trace tables of the MFMA Navier-Stokes kernel, or null
     void* rec;              // face-point records [3 Nfq][Kp] (U, Fv.n) of the MFMA Navier-Stokes stage
-    const double* bc;       // [n_bcs][5]: type, 4 values
+    const double* bc;       // boundary-face point rows (bc_row)
     int K, Kp;
     double dt, ca, cb;
     double gamma, mu, prandtl, sigma0;

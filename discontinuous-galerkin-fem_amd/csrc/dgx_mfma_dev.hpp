@@ -27,8 +27,9 @@ __device__ __forceinline__ unsigned pack_bf16x2(float lo, float hi) {
            (unsigned(__builtin_bit_cast(unsigned short, (__bf16)hi)) << 16);
 }
 
-// Ghost state of a boundary face point (boundary_face_residual, euler_weak_formulation.cpp:
-// 208-233); a face without a condition (type < 0) mirrors the interior state.
+// Ghost state of a boundary face point from its boundary-data row (bc_row: type, then the
+// Euler ghost data) (boundary_face_residual, euler_weak_formulation.cpp:208-233); a face
+// without a condition (type < 0) mirrors the interior state.
 __device__ __forceinline__ void ghost_state(const float* Uo, float nx, float ny, const double* bcr, float gam,
                                             float* Un) {
     const int type = int(bcr[0]);
@@ -71,15 +72,16 @@ __device__ __forceinline__ void rusanov2(const float* A, const float* B, float n
     H[3] = (fa3 + fb3) - s * (B[3] - A[3]);
 }
 
-// Flux of one face point in the canonical (L) orientation, scaled for this element's lift.
-__device__ __forceinline__ void face_point_flux(const float* Uo, float* Un, uint32_t c, int nb, float nx, float ny,
-                                                float sf, const StageParams& sp, float gam, float* Hs) {
+// Flux of one face point in the canonical (L) orientation, scaled for this element's lift;
+// bcr is the point's boundary-data row on a boundary face.
+__device__ __forceinline__ void face_point_flux(const float* Uo, float* Un, uint32_t c, const double* bcr, float nx,
+                                                float ny, float sf, float gam, float* Hs) {
     const bool boundary = (c & 3u) == kFaceBoundary;
     const bool isL = (c >> 3) & 1u;
-    if (boundary) ghost_state(Uo, nx, ny, sp.bc + 5 * nb, gam, Un);
+    if (boundary) ghost_state(Uo, nx, ny, bcr, gam, Un);
     // a boundary face without a boundary condition contributes nothing
     // (S/solver/assembler.cpp:233-243)
-    const float sfe = (boundary && sp.bc[5 * nb] < 0.0) ? 0.0f : sf;
+    const float sfe = (boundary && bcr[0] < 0.0) ? 0.0f : sf;
     const bool own_is_a = isL || boundary;
     const float sgn = own_is_a ? 1.0f : -1.0f;
     float Aq[4], Bq[4], H[4];

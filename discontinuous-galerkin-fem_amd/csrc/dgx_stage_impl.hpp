@@ -351,11 +351,12 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
                 for (int k = 0; k < 4; ++k) An[k] = geo[(G_A00 + k) * Kp + nb];
             }
         } else {
-            // ghost state from the boundary condition (boundary_face_residual, :208-233)
-            const double* bcr = sp.bc + 5 * nb;
-            const int type = int(bcr[0]);
+            // ghost state from the boundary condition at each face point (boundary_face_residual,
+            // :208-233)
+            const int type = int(bc_row(sp.bc, nb, NFQ, 0)[0]);
 #pragma unroll
             for (int q = 0; q < NFQ; ++q) {
+                const double* bcr = bc_row(sp.bc, nb, NFQ, q);
                 Real W[4];
                 Real ri = rcp(Uo[q][0]);
                 W[0] = Uo[q][0]; W[1] = Uo[q][1] * ri; W[2] = Uo[q][2] * ri;
@@ -379,7 +380,7 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
 
         // a boundary face without a boundary condition contributes nothing (the assembler
         // skips faces whose bc_euler is unset, S/solver/assembler.cpp:233-243)
-        const bool no_bc = boundary && sp.bc[5 * nb] < 0.0;
+        const bool no_bc = boundary && bc_row(sp.bc, nb, NFQ, 0)[0] < 0.0;
         const Real sfe = no_bc ? Real(0) : sf;
         // canonical orientation: A-side = L element, normal = L-side outward normal
         const Real sgn = (isL || boundary) ? Real(1) : Real(-1);
@@ -430,15 +431,15 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
                 } else {
                     // boundary: Fn = Fv_L.n - sigma (U_bc - U_L), U_bc from the BC data taken as
                     // conserved except for NO_SLIP (:249-264)
-                    const double* bcr = sp.bc + 5 * nb;
+                    const double* bcr = bc_row(sp.bc, nb, NFQ, q);
                     const int type = int(bcr[0]);
                     Real Ub[4];
                     if (type == DGX_BC_NO_SLIP_WALL) {
-                        Real Wb[4] = {Real(bcr[1]), Real(bcr[2]), Real(bcr[3]), Real(bcr[4])};
+                        Real Wb[4] = {Real(bcr[5]), Real(bcr[6]), Real(bcr[7]), Real(bcr[8])};
                         prim2cons(Wb, gam, Ub);
                     } else {
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) Ub[v] = Real(bcr[1 + v]);
+                        for (int v = 0; v < 4; ++v) Ub[v] = Real(bcr[5 + v]);
                     }
 #pragma unroll
                     for (int v = 0; v < 4; ++v)

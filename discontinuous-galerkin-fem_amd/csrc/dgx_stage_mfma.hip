@@ -344,7 +344,8 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int eb0, 
             const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
             const float Uo[4] = {Uf[0][f] + ur[0], Uf[1][f] + ur[1], Uf[2][f] + ur[2], Uf[3][f] + ur[3]};
             float Un[4], H[4];
-            face_point_flux(Uo, Un, c, nb, nx, ny, sf, sp, gam, H);
+            const int q = f < 3 ? g : 4;  // this slot's point on the face
+            face_point_flux(Uo, Un, c, bc_row(sp.bc, nb, NFQ, q), nx, ny, sf, gam, H);
 #pragma unroll
             for (int v = 0; v < 4; ++v) Hs[v][f] = H[v];
         }

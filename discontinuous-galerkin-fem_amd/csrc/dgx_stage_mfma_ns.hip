@@ -371,7 +371,7 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             const uint32_t c = (code >> (4 * face)) & 15u;
             if ((c & 3u) != kFaceBoundary) continue;
             const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
-            const double* bcr = sp.bc + 5 * nb;
+            const double* bcr = bc_row(sp.bc, nb, NFQ, fp % NFQ);
             if (bcr[0] < 0.0) {  // no condition: the face contributes nothing
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Hs[v][f] = 0.0f;
@@ -384,15 +384,15 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             const float Uo[4] = {ro[f].u.x, ro[f].u.y, ro[f].u.z, ro[f].u.w};
             const float vo[4] = {ro[f].v.x, ro[f].v.y, ro[f].v.z, ro[f].v.w};
             // ghost state and viscous boundary flux Fv.n - sigma (U_bc - U) (:249-264), U_bc the
-            // condition's data taken as conserved except for NO_SLIP
+            // condition's data at the point itself, taken as conserved except for NO_SLIP
             float Un[4], Ub[4], H[4];
             ghost_state(Uo, nx, ny, bcr, gam, Un);
             if (int(bcr[0]) == DGX_BC_NO_SLIP_WALL) {
-                const float Wb[4] = {float(bcr[1]), float(bcr[2]), float(bcr[3]), float(bcr[4])};
+                const float Wb[4] = {float(bcr[5]), float(bcr[6]), float(bcr[7]), float(bcr[8])};
                 prim2cons(Wb, gam, Ub);
             } else {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) Ub[v] = float(bcr[1 + v]);
+                for (int v = 0; v < 4; ++v) Ub[v] = float(bcr[5 + v]);
             }
             rusanov(Uo, Un, nx, ny, gam, H);
 #pragma unroll

@@ -166,23 +166,25 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
                 }
             }
         } else {
-            // constant ghost states: FAR_FIELD conserved, NO_SLIP_WALL from primitives (the
-            // host rejects slip walls, whose ghost depends on the interior state)
-            const double* bcr = sp.bc + 5 * nb;
-            const int type = int(bcr[0]);
+            // ghost states independent of the state: FAR_FIELD conserved, NO_SLIP_WALL from
+            // primitives, per face point (the host rejects slip walls, whose ghost depends on
+            // the interior state)
+            const int type = int(bc_row(sp.bc, nb, NFQ, 0)[0]);
             skip = type < 0;
             const Real gm1 = g - Real(1);
-            Real Ug[4] = {Real(bcr[1]), Real(bcr[2]), Real(bcr[3]), Real(bcr[4])};
-            if (type == DGX_BC_NO_SLIP_WALL) {
-                const Real r = Ug[0], uu = Ug[1], vv = Ug[2], pp = Ug[3];
-                Ug[1] = r * uu;
-                Ug[2] = r * vv;
-                Ug[3] = pp / gm1 + Real(0.5) * r * (uu * uu + vv * vv);
-            }
 #pragma unroll
-            for (int q = 0; q < NFQ; ++q)
+            for (int q = 0; q < NFQ; ++q) {
+                const double* bcr = bc_row(sp.bc, nb, NFQ, q);
+                Real Ug[4] = {Real(bcr[1]), Real(bcr[2]), Real(bcr[3]), Real(bcr[4])};
+                if (type == DGX_BC_NO_SLIP_WALL) {
+                    const Real r = Ug[0], uu = Ug[1], vv = Ug[2], pp = Ug[3];
+                    Ug[1] = r * uu;
+                    Ug[2] = r * vv;
+                    Ug[3] = pp / gm1 + Real(0.5) * r * (uu * uu + vv * vv);
+                }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Un[q][v] = skip ? Uo[q][v] : Ug[v];
+            }
         }
         if (skip) continue;
 

@@ -251,8 +251,9 @@ __device__ __forceinline__ void vjp_mfma_body(const VjpParams& sp) {
         if (boundary) {
             // a face without a condition contributes nothing; FAR_FIELD and NO_SLIP_WALL ghost
             // states are constant (slip walls are rejected by the host)
-            if (sp.bc[5 * nb] < 0.0) continue;
-            ghost_state(Uo, nx, ny, sp.bc + 5 * nb, gam, Un);
+            const double* bcr = bc_row(sp.bc, nb, NFQ, q);
+            if (bcr[0] < 0.0) continue;
+            ghost_state(Uo, nx, ny, bcr, gam, Un);
         } else {
             const float4 tu = s_tu[(face * NFQ + q) * EB + eloc];
             const float4 tp = s_tp[(face * NFQ + q) * EB + eloc];

@@ -29,12 +29,14 @@ PRECISION = {"fp64": 0, "fp32": 1, "fp32_bf16vol": 2}
 PHYSICS = {"euler": 0, "ns": 1}
 BC_TYPES = {"far_field": 0, "slip_wall": 1, "no_slip_wall": 2, "periodic": 3, "none": -1}
 QUIRK_NS_FACE_GRAD_REFERENCE = 1
+QUIRK_EULER_BC_POINTS_REFERENCE = 2
+BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 # exported symbols, in the order include/dgx.h declares them
 EXPORTS = [
     "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_create", "dgx_destroy", "dgx_last_error",
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
-    "dgx_eval_volume", "dgx_l2_error", "dgx_write_vtk", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
+    "dgx_eval_volume", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
     "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
     "dgx_density_range", "dgx_kinetic_energy", "dgx_residual_vjp", "dgx_rhs_vjp",
     "dgx_kinetic_energy_grad", "dgx_rk3_vjp", "dgx_rk3_vjp_profile",
@@ -106,6 +108,7 @@ def load():
               "dgx_apply_minv", "dgx_conservation_sums", "dgx_density_range", "dgx_kinetic_energy"):
         getattr(L, n).argtypes = [vp, dp]
     L.dgx_l2_error.argtypes = [vp, dp, C.c_int32, dp]
+    L.dgx_set_bc_function.argtypes = [vp, C.c_int32, BC_FN, vp]
     L.dgx_write_vtk.argtypes = [vp, dp, C.c_char_p, C.c_double, C.c_int32]
     L.dgx_residual.argtypes = [vp, dp, dp]
     L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]
@@ -216,7 +219,13 @@ class Context:
     [K][Np][4] FP64."""
 
     def __init__(self, verts, tris, order, weak_form=None, basis="dubiner", precision="fp32",
-                 rule=0, periodic=3, neighbors=None, bcs=None):
+                 rule=0, periodic=3, neighbors=None, bcs=None, euler_bc_points="reference"):
+        """bcs: {tag: (type, value)} with tag in Bottom/Right/Top/Left and value a constant state
+        (FAR_FIELD conserved, NO_SLIP primitive) or a function (x, y) -> state.
+        euler_bc_points: where Euler boundary faces evaluate a position-dependent condition:
+        "reference" (the reference's transposed read of the face points) or "correct"."""
+        if euler_bc_points not in ("reference", "correct"):
+            raise ValueError("euler_bc_points must be 'reference' or 'correct'")
         L = load()
         wf = weak_form or EulerWeakFormulation()
         self.weak_form = wf
@@ -230,15 +239,20 @@ class Context:
         tags = ["Bottom", "Right", "Top", "Left"]
         bc_t = np.full(4, -1, dtype=np.int32)
         bc_v = np.zeros((4, 4))
+        bc_fns = {}
         for name, (typ, val) in bcs.items():
             k = tags.index(name)
             bc_t[k] = BC_TYPES[typ]
-            bc_v[k] = val
+            if callable(val):
+                bc_fns[k] = val
+            else:
+                bc_v[k] = val
         self._bc_t, self._bc_v = bc_t, bc_v
         mesh = MeshDesc(len(self._verts), _dp(self._verts), len(self._tris), _ip(self._tris),
                         _ip(self._nbr), 4, _ip(bc_t), _dp(bc_v))
         space = SpaceDesc(order, BASIS[basis], rule)
-        phys = Physics(PHYSICS[wf.kind], wf.gamma, wf.mu, wf.prandtl, wf.sigma0, wf.quirks)
+        quirks = wf.quirks | (QUIRK_EULER_BC_POINTS_REFERENCE if euler_bc_points == "reference" else 0)
+        phys = Physics(PHYSICS[wf.kind], wf.gamma, wf.mu, wf.prandtl, wf.sigma0, quirks)
         h = C.c_void_p()
         _check(L.dgx_create(C.byref(mesh), C.byref(space), C.byref(phys), PRECISION[precision], C.byref(h)))
         self._h = h
@@ -247,6 +261,15 @@ class Context:
         self.info = info
         self.K, self.Np, self.Nq, self.Nfq = info.n_elements, info.n_modes, info.n_volume_points, info.n_face_points
         self.precision = precision
+        self._bc_cbs = []
+        for k, fn in bc_fns.items():
+            def _cb(x, y, out, _user, _fn=fn):
+                r = _fn(x, y)
+                for v in range(4):
+                    out[v] = float(r[v])
+            cb = BC_FN(_cb)
+            self._bc_cbs.append(cb)  # called again whenever a later condition is set
+            _check(L.dgx_set_bc_function(h, k, cb, None))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -36,9 +36,16 @@ inline void check(dgx_status st) {
 // --------------------------------------------------------------------------------------------
 // Mesh: vertices + triangles + face connectivity (periodic pairing or boundary tags).
 // --------------------------------------------------------------------------------------------
+// BoundaryConditionEuler (I/boundary/conditions.hpp:75-79): a constant state or a function of
+// position.  FAR_FIELD: conserved state; NO_SLIP: primitive (rho, u, v, p).
 struct BoundaryConditionEuler {
+    using Function = std::function<std::array<double, 4>(double x, double y)>;
+    BoundaryConditionEuler() = default;
+    BoundaryConditionEuler(dgx_bc_type t, const std::array<double, 4>& v) : type(t), value(v) {}
+    BoundaryConditionEuler(dgx_bc_type t, Function f) : type(t), function(std::move(f)) {}
     dgx_bc_type type = DGX_BC_FAR_FIELD;
-    std::array<double, 4> value{};  // FAR_FIELD: conserved state; NO_SLIP: primitive (rho,u,v,p)
+    std::array<double, 4> value{};
+    Function function;  // when set, replaces value
 };
 
 class DGMesh {
@@ -74,6 +81,7 @@ public:
                     throw std::runtime_error("PERIODIC BC: use set_periodic_boundaries");
                 bc_types_[k] = bc.type;
                 for (int v = 0; v < 4; ++v) bc_values_[4 * k + v] = bc.value[v];
+                bc_functions_[k] = bc.function;
                 return;
             }
         throw std::invalid_argument("Unknown boundary tag: " + tag);
@@ -81,6 +89,8 @@ public:
     int n_vertices() const { return int(vertices_.size() / 2); }
     int n_elements() const { return int(elements_.size() / 3); }
     const std::vector<double>& vertices() const { return vertices_; }
+    // the function-valued condition of boundary tag k (Bottom, Right, Top, Left), or empty
+    const BoundaryConditionEuler::Function& bc_function(int k) const { return bc_functions_[k]; }
 
     dgx_mesh_desc desc() const {
         dgx_mesh_desc d{};
@@ -102,6 +112,7 @@ private:
     int periodic_ = 0;
     std::array<int32_t, 4> bc_types_{-1, -1, -1, -1};
     std::array<double, 16> bc_values_{};
+    std::array<BoundaryConditionEuler::Function, 4> bc_functions_{};
 };
 
 // --------------------------------------------------------------------------------------------
@@ -114,10 +125,16 @@ public:
     double get_gamma() const { return gamma_; }
     int get_n_vars() const { return 4; }
     virtual bool has_viscous_terms() const { return false; }
-    virtual dgx_physics physics() const { return dgx_physics{DGX_EULER, gamma_, 0.0, 0.72, 5.0, 0u}; }
+    virtual dgx_physics physics() const { return dgx_physics{DGX_EULER, gamma_, 0.0, 0.72, 5.0, bc_point_quirk()}; }
+    // Where the Euler boundary flux evaluates a function-valued condition: the reference's
+    // row-major read of the face points (default, euler_weak_formulation.cpp:188-194) or the
+    // face points themselves.
+    void set_reference_bc_points(bool reference) { bc_points_reference_ = reference; }
 
 protected:
+    uint32_t bc_point_quirk() const { return bc_points_reference_ ? uint32_t(DGX_QUIRK_EULER_BC_POINTS_REFERENCE) : 0u; }
     double gamma_;
+    bool bc_points_reference_ = true;
 };
 
 class NavierStokesWeakFormulation : public EulerWeakFormulation {
@@ -137,7 +154,7 @@ public:
     }
     dgx_physics physics() const override {
         return dgx_physics{DGX_NAVIER_STOKES, gamma_, mu_, pr_, sigma0_,
-                           quirk_ ? uint32_t(DGX_QUIRK_NS_FACE_GRAD_REFERENCE) : 0u};
+                           (quirk_ ? uint32_t(DGX_QUIRK_NS_FACE_GRAD_REFERENCE) : 0u) | bc_point_quirk()};
     }
 
 private:
@@ -160,6 +177,13 @@ public:
         check(dgx_create(&m, &s, &ph, precision, &c));
         ctx_.reset(c);
         check(dgx_info(c, &info_));
+        // function-valued conditions: the library calls back while it tabulates the boundary
+        // face points; the functions live on the heap so a moved context keeps their addresses
+        for (int k = 0; k < 4; ++k)
+            if (mesh_.bc_function(k)) {
+                (*fns_)[k] = mesh_.bc_function(k);
+                check(dgx_set_bc_function(c, k, &call_bc, &(*fns_)[k]));
+            }
     }
     dgx_ctx* get() const { return ctx_.get(); }
     const dgx_info_t& info() const { return info_; }
@@ -167,7 +191,13 @@ public:
 
 private:
     struct Del { void operator()(dgx_ctx* c) const { dgx_destroy(c); } };
+    static void call_bc(double x, double y, double out[4], void* user) {
+        const auto U = (*static_cast<BoundaryConditionEuler::Function*>(user))(x, y);
+        for (int v = 0; v < 4; ++v) out[v] = U[v];
+    }
     DGMesh mesh_;
+    std::unique_ptr<std::array<BoundaryConditionEuler::Function, 4>> fns_ =
+        std::make_unique<std::array<BoundaryConditionEuler::Function, 4>>();
     std::unique_ptr<dgx_ctx, Del> ctx_;
     dgx_info_t info_{};
 };

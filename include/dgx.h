@@ -53,8 +53,19 @@ enum dgx_bc_type { DGX_BC_FAR_FIELD = 0, DGX_BC_SLIP_WALL = 1, DGX_BC_NO_SLIP_WA
 enum dgx_quirk {
     /* NS face gradients read through the reference's column-major Map of the interleaved
      * dphi_dx_face storage (S/core/space.cpp:143,153-158 vs S/core/mesh.cpp:187-195). */
-    DGX_QUIRK_NS_FACE_GRAD_REFERENCE = 1u << 0
+    DGX_QUIRK_NS_FACE_GRAD_REFERENCE = 1u << 0,
+    /* Euler boundary faces evaluate the condition at the reference's row-major read of the
+     * column-major face-point storage (S/core/space.cpp:141,148,164 vs
+     * S/weak_forms/euler_weak_formulation.cpp:188-194); without it, at the face points.  Only
+     * position-dependent conditions (dgx_set_bc_function) see a difference; the viscous
+     * boundary flux reads the points correctly either way (navier_stokes_...cpp:244-247). */
+    DGX_QUIRK_EULER_BC_POINTS_REFERENCE = 1u << 1
 };
+
+/* A boundary condition as a function of position (BoundaryConditionEuler's function,
+ * S/boundary/conditions.cpp:47-60): out = the FAR_FIELD conserved state or the NO_SLIP
+ * primitive state at (x, y). */
+typedef void (*dgx_bc_fn)(double x, double y, double out[4], void* user);
 
 typedef struct dgx_ctx dgx_ctx;
 
@@ -148,6 +159,13 @@ dgx_status dgx_eval_volume(dgx_ctx* ctx, double* Uq);
  * Replaces compute_variable_error of T/solver/shear_flow_accuracy_test.cpp:32-79 and
  * X/taylor_green_test.cpp:44-86. */
 dgx_status dgx_l2_error(dgx_ctx* ctx, const double* exact, int32_t exact_is_primitive, double err[4]);
+/* Make boundary condition `bc_index` (a row of the mesh description's bc_types / bc_values) a
+ * function of position: fn is evaluated now, on the host, at every face quadrature point of
+ * the faces carrying it (and at the reference's read of those points for the Euler boundary
+ * flux when DGX_QUIRK_EULER_BC_POINTS_REFERENCE is set) and replaces the constant value.
+ * Replaces DGMesh::set_boundary_condition_euler with a function-valued condition
+ * (S/core/mesh.cpp:366-401, S/boundary/conditions.cpp:47-60). */
+dgx_status dgx_set_bc_function(dgx_ctx* ctx, int32_t bc_index, dgx_bc_fn fn, void* user);
 /* Legacy-VTK file `filename`.vtk of an Euler state (host layout [K][Np][4], or the context's
  * current state when u is NULL): each element subdivided at `resolution_factor`, fields
  * density, u_velocity, v_velocity, velocity_magnitude, pressure, temperature, mach, rho,

@@ -368,6 +368,7 @@ struct BFace { int e, f; std::string tag; };
 
 struct Ctx {
     int K = 0, p = 1, np = 3, nq = 0, nfq = 0, physics = 0, ns_face_grad = 0;
+    int euler_bc_points_reference = 1;  // Q5: Euler boundary points read through the transposed map
     Basis basis;
     std::vector<double> vx, vy;
     std::vector<int> tri;             // K*3
@@ -756,7 +757,9 @@ void boundary_face_residual(const Ctx& c, const BFace& bf, const BC& bc, const d
         for (int i = 0; i < np; ++i)
             for (int v = 0; v < 4; ++v) UL[v] += ph[q * np + i] * u[i * 4 + v];
         V4 WL = cons2prim(UL, c.ns.gamma);
+        // Q5 (the reference's read, default) or the point itself
         double xq = F.qp_flat[2 * q], yq = F.qp_flat[2 * q + 1];
+        if (!c.euler_bc_points_reference) { xq = F.qp_flat[q]; yq = F.qp_flat[q + c.nfq]; }
         V4 UR;
         if (bc.type == 0) UR = bc.eval(xq, yq);
         else if (bc.type == 1) {
@@ -1040,6 +1043,13 @@ void ref_info(void* h, int32_t* o) {
     Ctx& c = *static_cast<Ctx*>(h);
     o[0] = c.K; o[1] = c.np; o[2] = c.nq; o[3] = c.nfq;
     o[4] = (int32_t)c.ifaces.size(); o[5] = (int32_t)c.bfaces.size();
+}
+
+// Q5 switch: 1 = the reference's transposed read of the Euler boundary points, 0 = the points.
+int ref_set_euler_bc_points(void* h, int reference) {
+    if (!h) return -1;
+    static_cast<Ctx*>(h)->euler_bc_points_reference = reference ? 1 : 0;
+    return 0;
 }
 
 int ref_set_bc(void* h, const char* tag, int type, const double* value4, ref_bc_fn fn, void* user) {

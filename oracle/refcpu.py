@@ -41,6 +41,7 @@ def lib():
         L.ref_last_error.restype = C.c_char_p
         L.ref_info.argtypes = [C.c_void_p, ip]
         L.ref_set_bc.argtypes = [C.c_void_p, C.c_char_p, C.c_int, dp, BC_FN, C.c_void_p]
+        L.ref_set_euler_bc_points.argtypes = [C.c_void_p, C.c_int]
         for name in ("ref_residual", "ref_rhs", "ref_project", "ref_eval_volume"):
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
         L.ref_apply_minv.argtypes = [C.c_void_p, dp]
@@ -172,7 +173,7 @@ class Oracle:
 
     def __init__(self, verts, tris, order, basis="monomial", vol_rule=0, periodic=3,
                  physics="euler", gamma=1.4, mu=1e-3, prandtl=0.72, sigma0=5.0,
-                 ns_face_grad="reference"):
+                 ns_face_grad="reference", euler_bc_points="reference"):
         L = lib()
         self._verts = np.ascontiguousarray(verts, dtype=np.float64)
         self._tris = np.ascontiguousarray(tris, dtype=np.int32)
@@ -188,6 +189,9 @@ class Oracle:
         self.K, self.Np, self.Nq, self.Nfq, self.n_interior, self.n_boundary = map(int, info)
         self.order = order
         self._cbs = []
+        # Q5: Euler boundary faces evaluate the condition at the reference's transposed read of
+        # the face points ("reference") or at the points ("correct")
+        L.ref_set_euler_bc_points(self._h, {"reference": 1, "correct": 0}[euler_bc_points])
 
     def __del__(self):
         if getattr(self, "_h", None):

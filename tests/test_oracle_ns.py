@@ -262,3 +262,45 @@ def test_euler_boundary_slip_wall_reflects_normal_velocity():
         Rb = o.boundary_face_residual(k, u[e])
         assert np.abs(Rb[:, 0]).max() < 1e-15          # no mass flux
         assert np.abs(Rb[:, 3]).max() < 1e-15          # no energy flux
+
+
+def _euler_flux(U, n, gam=GAMMA):
+    rho, ru, rv, E = U
+    u, v = ru / rho, rv / rho
+    p = (gam - 1) * (E - 0.5 * rho * (u * u + v * v))
+    un = u * n[0] + v * n[1]
+    return np.array([rho * un, ru * un + p * n[0], rv * un + p * n[1], (E + p) * un]), abs(un) + math.sqrt(gam * p / rho)
+
+
+@pytest.mark.parametrize("points", ["reference", "correct"])
+def test_euler_boundary_position_dependent_far_field(points):
+    # boundary_face_residual FAR_FIELD with a function of position (euler_weak_formulation.cpp:
+    # 188-213).  The face points are stored column-major, [x_0 .. x_{n-1}, y_0 .. y_{n-1}]
+    # (space.cpp:141-164), and read back through a row-major n x 2 map, so point q is evaluated
+    # at (flat[2q], flat[2q+1]): for two points (x_0, x_1) and (y_0, y_1).  "correct" reads (x_q, y_q).
+    verts, tris = mesh2()
+    o = R.Oracle(verts, tris, 1, basis="monomial", periodic=0, euler_bc_points=points)
+    bc = lambda x, y: prim2cons([1.0 + 0.2 * x, 0.3 + y, 0.1 * x, 1.0 + 0.5 * y])  # noqa: E731
+    for tag in ("Bottom", "Right", "Top", "Left"):
+        o.set_bc(tag, "far_field", fn=bc)
+    u = shear_coeffs(verts, tris)
+    s, wg = R.gauss_legendre(2)
+    seen_difference = False
+    for k, (e, f, has_bc) in enumerate(o.boundary_faces()):
+        V, _, _, _ = elem_geom(verts, tris, e)
+        n, L = outward_normal(V, f)
+        pts = face_points(V, f, s)
+        flat = [pts[0][0], pts[1][0], pts[0][1], pts[1][1]]
+        exp = np.zeros((3, 4))
+        for q, x in enumerate(pts):
+            xb = (flat[2 * q], flat[2 * q + 1]) if points == "reference" else tuple(x)
+            seen_difference |= not np.allclose(xb, x)
+            a, b = ref_point(V, x)
+            ph = np.array([1, b, a])
+            UL, UR = ph @ u[e], bc(*xb)
+            FL, sL = _euler_flux(UL, n)
+            FR, sR = _euler_flux(UR, n)
+            H = 0.5 * (FL + FR) - 0.5 * max(sL, sR) * (UR - UL)
+            exp -= wg[q] * L / 2 * np.outer(ph, H)
+        assert np.allclose(o.boundary_face_residual(k, u[e]), exp, rtol=0, atol=1e-12)
+    assert seen_difference == (points == "reference")
