@@ -95,6 +95,32 @@ void upload_bc_table(dgx_ctx& c) {
     c.upload(c.bc, tab.data(), tab.size() * sizeof(double));
 }
 
+// FP64 geometry, element sizes and face codes on the device from the mesh arrays (validated
+// on the host); a singular mapping or a degenerate edge is reported like the reference.
+void build_geometry(dgx_ctx& c, const dgx_mesh_desc& m) {
+    DevBuf dV, dT, dN, derr;
+    dV.alloc((size_t)m.n_vertices * 2 * sizeof(double));
+    dT.alloc((size_t)c.K * 3 * sizeof(int32_t));
+    dN.alloc((size_t)c.K * 6 * sizeof(int32_t));
+    derr.alloc(sizeof(int));
+    c.geo.alloc((size_t)kGeoRows * c.Kp * sizeof(double));
+    c.hmin.alloc((size_t)c.Kp * sizeof(double));
+    c.code.alloc((size_t)c.Kp * sizeof(uint32_t));
+    hip_check(hipMemcpyAsync(dV.p, m.vertices, (size_t)m.n_vertices * 2 * sizeof(double), hipMemcpyHostToDevice, c.stream), "copy");
+    hip_check(hipMemcpyAsync(dT.p, m.elements, (size_t)c.K * 3 * sizeof(int32_t), hipMemcpyHostToDevice, c.stream), "copy");
+    hip_check(hipMemcpyAsync(dN.p, m.face_neighbors, (size_t)c.K * 6 * sizeof(int32_t), hipMemcpyHostToDevice, c.stream), "copy");
+    hip_check(hipMemsetAsync(derr.p, 0, sizeof(int), c.stream), "memset");
+    hip_check(launch_geometry(dV.as<double>(), static_cast<const int32_t*>(dT.p), static_cast<const int32_t*>(dN.p), c.K,
+                              c.Kp, c.geo.as<double>(), c.hmin.as<double>(), static_cast<uint32_t*>(c.code.p),
+                              static_cast<int*>(derr.p), c.stream),
+              "geometry");
+    int err = 0;
+    hip_check(hipMemcpyAsync(&err, derr.p, sizeof(int), hipMemcpyDeviceToHost, c.stream), "copy");
+    hip_check(hipStreamSynchronize(c.stream), "sync");
+    if (err & 1) throw std::invalid_argument("Singular Jacobian matrix");
+    if (err & 2) throw std::invalid_argument("Degenerate edge detected");
+}
+
 void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
     if (!m.vertices || !m.elements || !m.face_neighbors || m.n_elements <= 0 || m.n_vertices <= 0)
         throw std::invalid_argument("mesh descriptor is incomplete");
@@ -232,62 +258,27 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
             Pm[q * np + i] = sacc;
         }
 
-    // ---------------- geometry (FP64) and connectivity codes ----------------
+    // ---------------- connectivity (host: integer work and validation) ----------------
+    // The FP64 geometry and the face codes are built on the device from the vertices (below).
     const int K = c.K, Kp = c.Kp;
     const double* V = m.vertices;
     const int32_t* T = m.elements;
     const int32_t* N = m.face_neighbors;
-    std::vector<double> geo((size_t)kGeoRows * Kp, 0.0), hmin(Kp, 1.0);
     std::vector<int32_t> nbr((size_t)3 * Kp, 0);
-    std::vector<uint32_t> code(Kp, 0);
-    for (int e = 0; e < K; ++e) geo[G_DET * Kp + e] = 1.0;
     c.h_verts.assign((size_t)K * 6, 0.0);
-    auto vert = [&](int e, int k, double& x, double& y) {
-        int id = T[e * 3 + k];
-        if (id < 0 || id >= m.n_vertices) throw std::invalid_argument("element references a missing vertex");
-        x = V[2 * id];
-        y = V[2 * id + 1];
-    };
-    // GeometricMapping::compute_mapping (S/reference/mapping.cpp:22-50), constant on a triangle
-    for (int e = 0; e < K; ++e) {
-        double x0, y0, x1, y1, x2, y2;
-        vert(e, 0, x0, y0); vert(e, 1, x1, y1); vert(e, 2, x2, y2);
-        double jt00 = x1 - x0, jt01 = y1 - y0, jt10 = x2 - x0, jt11 = y2 - y0;
-        double det = jt00 * jt11 - jt01 * jt10;
-        if (std::fabs(det) < 1e-12) throw std::invalid_argument("Singular Jacobian matrix");
-        double id = 1.0 / det;
-        geo[G_A00 * Kp + e] = jt11 * id;
-        geo[G_A01 * Kp + e] = -jt01 * id;
-        geo[G_A10 * Kp + e] = -jt10 * id;
-        geo[G_A11 * Kp + e] = jt00 * id;
-        double* hv = &c.h_verts[(size_t)e * 6];
-        hv[0] = x0; hv[1] = y0; hv[2] = x1; hv[3] = y1; hv[4] = x2; hv[5] = y2;
-        geo[G_DET * Kp + e] = std::fabs(det);
-        double l01 = std::hypot(x1 - x0, y1 - y0), l12 = std::hypot(x2 - x1, y2 - y1), l20 = std::hypot(x0 - x2, y0 - y2);
-        hmin[e] = std::min(l01, std::min(l12, l20));
-    }
-    // face normal/length of element e, local face f (DGSpace::compute_face_data, S/core/space.cpp:104-137)
-    auto face_geom = [&](int e, int f, double& nx, double& ny, double& len, double* v1, double* v2) {
-        double xs[3], ys[3];
-        for (int k = 0; k < 3; ++k) vert(e, k, xs[k], ys[k]);
-        int a = f, b = (f + 1) % 3;
-        v1[0] = xs[a]; v1[1] = ys[a]; v2[0] = xs[b]; v2[1] = ys[b];
-        double tx = xs[b] - xs[a], ty = ys[b] - ys[a];
-        len = std::sqrt(tx * tx + ty * ty);
-        if (len < 1e-14) throw std::invalid_argument("Degenerate edge detected");
-        nx = ty / len;
-        ny = -tx / len;
-        double cx = (xs[0] + xs[1] + xs[2]) / 3.0, cy = (ys[0] + ys[1] + ys[2]) / 3.0;
-        if (nx * (0.5 * (xs[a] + xs[b]) - cx) + ny * (0.5 * (ys[a] + ys[b]) - cy) < 0) { nx = -nx; ny = -ny; }
-    };
-    std::map<std::pair<int, int>, int> pair_count;
-    int nbf = 0;
     for (int e = 0; e < K; ++e)
+        for (int k = 0; k < 3; ++k) {
+            const int id = T[e * 3 + k];
+            if (id < 0 || id >= m.n_vertices) throw std::invalid_argument("element references a missing vertex");
+            c.h_verts[(size_t)e * 6 + 2 * k] = V[2 * id];
+            c.h_verts[(size_t)e * 6 + 2 * k + 1] = V[2 * id + 1];
+        }
+    int nbf = 0;
+    for (int e = 0; e < K; ++e) {
+        int en3[3];
         for (int f = 0; f < 3; ++f) {
-            int en = N[(e * 3 + f) * 2], fn = N[(e * 3 + f) * 2 + 1];
-            double nx, ny, len, a1[2], a2[2];
-            face_geom(e, f, nx, ny, len, a1, a2);
-            uint32_t cfield = 0;
+            const int en = N[(e * 3 + f) * 2], fn = N[(e * 3 + f) * 2 + 1];
+            en3[f] = en;
             if (en < 0) {
                 int bci = fn;
                 if (bci < 0) bci = m.n_bcs;  // "no boundary condition" row
@@ -295,42 +286,23 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
                 if (bci < m.n_bcs && m.bc_types && m.bc_types[bci] == DGX_BC_SLIP_WALL) c.has_slip = true;
                 nbr[f * Kp + e] = nbf++;  // the boundary face's running index
                 c.h_bfaces.push_back({e, f, bci});
-                cfield = kFaceBoundary | (1u << 3);
             } else {
                 if (en >= K || fn < 0 || fn > 2) throw std::invalid_argument("face neighbour out of range");
                 if (N[(en * 3 + fn) * 2] != e || N[(en * 3 + fn) * 2 + 1] != f)
                     throw std::invalid_argument("face_neighbors is not symmetric");
-                if (++pair_count[{std::min(e, en), std::max(e, en)}] > 2)
-                    throw Fail(DGX_E_UNSUPPORTED, "two elements share more than one face (periodic mesh with n < 2)");
-                bool isL = (e < en) || (e == en && f < fn);
-                int eL = isL ? e : en, fL = isL ? f : fn, eR = isL ? en : e, fR = isL ? fn : f;
-                double nLx, nLy, lenL, L1[2], L2[2], nRx, nRy, lenR, R1[2], R2[2];
-                face_geom(eL, fL, nLx, nLy, lenL, L1, L2);
-                face_geom(eR, fR, nRx, nRy, lenR, R1, R2);
-                // DGSpace::compute_face_permutation (S/core/space.cpp:173-228)
-                const double tol = 1e-10;
-                auto d = [](const double* p, const double* q) { return std::hypot(p[0] - q[0], p[1] - q[1]); };
-                bool rev = d(L1, R2) < tol && d(L2, R1) < tol;
-                bool same = d(L1, R1) < tol && d(L2, R2) < tol;
-                bool reverse;
-                if (rev || same) reverse = rev;
-                else {
-                    double t1x = L2[0] - L1[0], t1y = L2[1] - L1[1], t2x = R2[0] - R1[0], t2y = R2[1] - R1[1];
-                    double n1 = std::hypot(t1x, t1y), n2 = std::hypot(t2x, t2y);
-                    reverse = (t1x / n1 * t2x / n2 + t1y / n1 * t2y / n2) < 0.0;
-                }
-                // the face is evaluated with the L side's normal and length on both elements
-                nx = isL ? nLx : -nLx;
-                ny = isL ? nLy : -nLy;
-                len = lenL;
                 nbr[f * Kp + e] = en;
-                cfield = uint32_t(fn) | (reverse ? 4u : 0u) | (isL ? 8u : 0u);
             }
-            geo[(G_N0X + 2 * f) * Kp + e] = nx;
-            geo[(G_N0X + 2 * f + 1) * Kp + e] = ny;
-            geo[(G_L0 + f) * Kp + e] = len;
-            code[e] |= cfield << (4 * f);
         }
+        // a pair of elements shares at most one face (an element paired with itself: one face
+        // pair, i.e. two of its faces)
+        for (int f = 0; f < 3; ++f) {
+            if (en3[f] < 0) continue;
+            int shared = 0;
+            for (int g = 0; g < 3; ++g) shared += en3[g] == en3[f];
+            if (shared > (en3[f] == e ? 2 : 1))
+                throw Fail(DGX_E_UNSUPPORTED, "two elements share more than one face (periodic mesh with n < 2)");
+        }
+    }
     c.n_bfaces = nbf;
     c.h_bctype.assign(m.n_bcs + 1, -1);
     c.h_bcval.assign((size_t)(m.n_bcs + 1) * 4, 0.0);
@@ -351,10 +323,8 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
     hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&c.ev0), "hipEventCreate");
     hip_check(hipEventCreate(&c.ev1), "hipEventCreate");
-    c.upload(c.geo, geo.data(), geo.size() * sizeof(double));
     c.upload(c.nbr, nbr.data(), nbr.size() * sizeof(int32_t));
-    c.upload(c.code, code.data(), code.size() * sizeof(uint32_t));
-    c.upload(c.hmin, hmin.data(), hmin.size() * sizeof(double));
+    build_geometry(c, m);
     upload_bc_table(c);
     c.upload(c.tab64, tL.data(), tL.size() * sizeof(double));
     c.upload(c.minv, Mi.data(), Mi.size() * sizeof(double));
@@ -581,6 +551,25 @@ dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
                 xy[((size_t)e * c->nq + q) * 2 + 1] = n0 * V[1] + n1 * V[3] + n2 * V[5];
             }
         }
+    });
+}
+
+dgx_status dgx_get_geometry(dgx_ctx* c, double* geo, double* hmin, uint32_t* codes) {
+    return guarded([&] {
+        if (!c) throw std::invalid_argument("null argument");
+        const size_t K = c->K, Kp = c->Kp;
+        if (geo) {
+            std::vector<double> g((size_t)kGeoRows * Kp);
+            hip_check(hipMemcpyAsync(g.data(), c->geo.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+            hip_check(hipStreamSynchronize(c->stream), "sync");
+            for (size_t e = 0; e < K; ++e)
+                for (int r = 0; r < kGeoRows; ++r) geo[e * kGeoRows + r] = g[r * Kp + e];
+        }
+        if (hmin)
+            hip_check(hipMemcpyAsync(hmin, c->hmin.p, K * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+        if (codes)
+            hip_check(hipMemcpyAsync(codes, c->code.p, K * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
     });
 }
 

@@ -36,7 +36,8 @@ BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_voi
 EXPORTS = [
     "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_create", "dgx_destroy", "dgx_last_error",
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
-    "dgx_eval_volume", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk", "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
+    "dgx_eval_volume", "dgx_get_geometry", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk",
+    "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
     "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
     "dgx_density_range", "dgx_kinetic_energy", "dgx_residual_vjp", "dgx_rhs_vjp",
     "dgx_kinetic_energy_grad", "dgx_rk3_vjp", "dgx_rk3_vjp_profile",
@@ -109,6 +110,7 @@ def load():
         getattr(L, n).argtypes = [vp, dp]
     L.dgx_l2_error.argtypes = [vp, dp, C.c_int32, dp]
     L.dgx_set_bc_function.argtypes = [vp, C.c_int32, BC_FN, vp]
+    L.dgx_get_geometry.argtypes = [vp, dp, dp, C.POINTER(C.c_uint32)]
     L.dgx_write_vtk.argtypes = [vp, dp, C.c_char_p, C.c_double, C.c_int32]
     L.dgx_residual.argtypes = [vp, dp, dp]
     L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]
@@ -293,6 +295,13 @@ class Context:
         xy = np.zeros((self.K, self.Nq, 2))
         _check(load().dgx_volume_points(self._h, _dp(xy)))
         return xy
+
+    def geometry(self):
+        """(geo [K][14], hmin [K], codes [K]) as the kernels read them (built on the device)."""
+        geo, hmin = np.zeros((self.K, 14)), np.zeros(self.K)
+        codes = np.zeros(self.K, dtype=np.uint32)
+        _check(load().dgx_get_geometry(self._h, _dp(geo), _dp(hmin), codes.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return geo, hmin, codes
 
     def project(self, Uq):
         Uq = np.ascontiguousarray(Uq, dtype=np.float64)

@@ -147,6 +147,14 @@ dgx_status dgx_get_state(dgx_ctx* ctx, double* u);        /* device -> [K][Np][4
 
 /* Physical coordinates of the volume quadrature points, [K][Nq][2]. */
 dgx_status dgx_volume_points(dgx_ctx* ctx, double* xy);
+/* The per-element geometry the kernels read, built on the device (any pointer may be NULL):
+ * geo [K][14] FP64 = dxi/dx row-major (4), |det J|, outward unit normals of faces 0..2 (6;
+ * on an interior face the canonical side's normal, negated on the other side), face lengths
+ * (3); hmin [K] = smallest edge length; codes [K] = 4 bits per face f at bit 4f: neighbour's
+ * local face (3 = boundary), reversed point order (bit 2), canonical side (bit 3).  The data
+ * of DGMesh::get_element_data / get_face_data (I/core/mesh.hpp:56-62; S/core/space.cpp:104-137,
+ * 173-228; S/reference/mapping.cpp:22-50). */
+dgx_status dgx_get_geometry(dgx_ctx* ctx, double* geo, double* hmin, uint32_t* codes);
 /* L2 projection of point values Uq [K][Nq][4] into the state (project_initial_condition,
  * S/solver/compressible_solver_base.cpp:63-104). */
 dgx_status dgx_project(dgx_ctx* ctx, const double* Uq);
