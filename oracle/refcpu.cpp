@@ -155,10 +155,37 @@ double jacobi_d(int n, double al, double be, double x) {
     return 0.5 * (n + al + be + 1.0) * jacobi(n - 1, al + 1.0, be + 1.0, x);
 }
 
+// Legendre polynomials by the three-term recurrence (I/basis/legendre.hpp:55-72) and their
+// derivative (S/basis/legendre.cpp:31-49): n (x P_n - P_{n-1}) / (x^2 - 1) inside, and at
+// |x| = 1 the endpoint value 0.5 n (n+1) times parity(n) times sign(x), which is -P'_n(1) for
+// odd n at x = +1 (quirk Q16); `ref_endpoint` = false gives P'_n(+1) = n(n+1)/2 instead.
+double legendre(int n, double x) {
+    if (n == 0) return 1.0;
+    if (n == 1) return x;
+    double pm1 = 1.0, pn = x;
+    for (int i = 2; i <= n; ++i) {
+        double pp = ((2 * i - 1) * x * pn - (i - 1) * pm1) / i;
+        pm1 = pn;
+        pn = pp;
+    }
+    return pn;
+}
+double legendre_d(int n, double x, bool ref_endpoint) {
+    if (n == 0) return 0.0;
+    if (std::fabs(std::fabs(x) - 1.0) < 1e-12) {
+        double sign = x > 0 ? 1.0 : -1.0, parity = (n % 2 == 0) ? 1.0 : -1.0;
+        if (!ref_endpoint && x > 0) parity = 1.0;
+        return 0.5 * n * (n + 1) * parity * sign;
+    }
+    return n * (x * legendre(n, x) - legendre(n - 1, x)) / (x * x - 1.0);
+}
+
+// kind 0 monomial, 1 Dubiner (triangles), 2 Legendre tensor product (quads).
 struct Basis {
     int kind = 0, p = 1, np = 3;
+    bool ref_endpoint = true;  // Q16 for the Legendre derivative at x = +1
     Basis() = default;
-    Basis(int kind_, int p_) : kind(kind_), p(p_), np((p_ + 1) * (p_ + 2) / 2) {}
+    Basis(int kind_, int p_) : kind(kind_), p(p_), np(kind_ == 2 ? (p_ + 1) * (p_ + 1) : (p_ + 1) * (p_ + 2) / 2) {}
 
     void ij(int idx, int& i, int& j) const {  // Dubiner index order (S/basis/dubiner.cpp:77-90)
         int k = 0;
@@ -176,6 +203,12 @@ struct Basis {
         }
     }
     void eval(double x, double y, double* phi) const {
+        if (kind == 2) {  // phi_{i,j} = P_i(x) P_j(y), j outer (S/basis/legendre.cpp:69-86)
+            int k = 0;
+            for (int j = 0; j <= p; ++j)
+                for (int i = 0; i <= p; ++i) phi[k++] = legendre(i, x) * legendre(j, y);
+            return;
+        }
         if (kind == 0) {
             int k = 0;
             for (int d = 0; d <= p; ++d)
@@ -199,6 +232,16 @@ struct Basis {
     }
     // grad[k*2 + d]
     void grad(double x, double y, double* g) const {
+        if (kind == 2) {  // (S/basis/legendre.cpp:88-112)
+            int k = 0;
+            for (int j = 0; j <= p; ++j)
+                for (int i = 0; i <= p; ++i) {
+                    g[2 * k] = legendre_d(i, x, ref_endpoint) * legendre(j, y);
+                    g[2 * k + 1] = legendre(i, x) * legendre_d(j, y, ref_endpoint);
+                    ++k;
+                }
+            return;
+        }
         if (kind == 0) {
             int k = 0;
             for (int d = 0; d <= p; ++d)
@@ -235,8 +278,19 @@ struct Basis {
 };
 
 // Reference triangle (0,0),(1,0),(0,1); edges (0,1),(1,2),(2,0) (S/reference/elements.cpp:18-49).
-// Face point for s in [-1,1], t = (s+1)/2 (S/core/space.cpp:259-271).
-void face_point(int f, double s, double& x, double& y) {
+// Reference quad [-1,1]^2, vertices (-1,-1),(1,-1),(1,1),(-1,1), edges (f, f+1) (:61-87).
+// Face point for s in [-1,1] (map_face_quad_point, S/core/space.cpp:259-287).
+void face_point(int nvpe, int f, double s, double& x, double& y) {
+    if (nvpe == 4) {
+        switch (f) {
+        case 0: x = s; y = -1.0; break;
+        case 1: x = 1.0; y = s; break;
+        case 2: x = -s; y = 1.0; break;
+        case 3: x = -1.0; y = -s; break;
+        default: throw std::out_of_range("Invalid face for quad");
+        }
+        return;
+    }
     double t = 0.5 * (s + 1.0);
     switch (f) {
     case 0: x = t; y = 0.0; break;
@@ -246,18 +300,39 @@ void face_point(int f, double s, double& x, double& y) {
     }
 }
 
-// Affine mapping (S/reference/mapping.cpp:22-50): J_T = dN^T X, det, dxi_dx = inv(J_T).
+// Mapping (S/reference/mapping.cpp:22-50): x = X^T N, J_T = dN^T X, det, dxi_dx = inv(J_T);
+// linear shape functions on the triangle, bilinear on the quad (S/reference/elements.cpp:33-47,
+// 89-108).
 struct Mapping {
     double det, A[4], x, y;  // A row-major: A[r*2+c]
 };
-Mapping compute_mapping(const double* V, double xi, double eta) {
+Mapping compute_mapping(int nvpe, const double* V, double xi, double eta) {
     Mapping m;
-    double N0 = 1.0 - xi - eta, N1 = xi, N2 = eta;
-    m.x = N0 * V[0] + N1 * V[2] + N2 * V[4];
-    m.y = N0 * V[1] + N1 * V[3] + N2 * V[5];
+    double N[4], dN[4][2];
+    if (nvpe == 4) {
+        N[0] = 0.25 * (1.0 - xi) * (1.0 - eta);
+        N[1] = 0.25 * (1.0 + xi) * (1.0 - eta);
+        N[2] = 0.25 * (1.0 + xi) * (1.0 + eta);
+        N[3] = 0.25 * (1.0 - xi) * (1.0 + eta);
+        dN[0][0] = -0.25 * (1.0 - eta); dN[0][1] = -0.25 * (1.0 - xi);
+        dN[1][0] = 0.25 * (1.0 - eta);  dN[1][1] = -0.25 * (1.0 + xi);
+        dN[2][0] = 0.25 * (1.0 + eta);  dN[2][1] = 0.25 * (1.0 + xi);
+        dN[3][0] = -0.25 * (1.0 + eta); dN[3][1] = 0.25 * (1.0 - xi);
+    } else {
+        N[0] = 1.0 - xi - eta; N[1] = xi; N[2] = eta;
+        dN[0][0] = -1.0; dN[0][1] = -1.0; dN[1][0] = 1.0; dN[1][1] = 0.0; dN[2][0] = 0.0; dN[2][1] = 1.0;
+    }
+    m.x = 0.0; m.y = 0.0;
     // J_T[a][b] = sum_n dN_n/dxi_a * X_n,b
-    double JT00 = -V[0] + V[2], JT01 = -V[1] + V[3];
-    double JT10 = -V[0] + V[4], JT11 = -V[1] + V[5];
+    double JT00 = 0.0, JT01 = 0.0, JT10 = 0.0, JT11 = 0.0;
+    for (int n = 0; n < nvpe; ++n) {
+        m.x += N[n] * V[2 * n];
+        m.y += N[n] * V[2 * n + 1];
+        JT00 += dN[n][0] * V[2 * n];
+        JT01 += dN[n][0] * V[2 * n + 1];
+        JT10 += dN[n][1] * V[2 * n];
+        JT11 += dN[n][1] * V[2 * n + 1];
+    }
     m.det = JT00 * JT11 - JT01 * JT10;
     if (std::fabs(m.det) < 1e-12) throw std::runtime_error("Singular Jacobian matrix");
     double id = 1.0 / m.det;
@@ -359,7 +434,7 @@ struct FaceData {  // S/core/space.cpp:104-171
     std::vector<double> dphi_dx_face;  // [(q*Np+i)*2 + d]
 };
 struct ElemData {  // S/core/space.cpp:69-102
-    double V[6];
+    double V[8];
     std::vector<double> jdet;     // Nq
     std::vector<double> dphi_dx;  // [(q*Np+i)*2 + d]
 };
@@ -368,18 +443,19 @@ struct BFace { int e, f; std::string tag; };
 
 struct Ctx {
     int K = 0, p = 1, np = 3, nq = 0, nfq = 0, physics = 0, ns_face_grad = 0;
+    int nv = 3;  // vertices = faces per element: 3 triangles, 4 quads
     int euler_bc_points_reference = 1;  // Q5: Euler boundary points read through the transposed map
     Basis basis;
     std::vector<double> vx, vy;
-    std::vector<int> tri;             // K*3
-    std::vector<int> nbr;             // K*3*2  (elem, face) or -1
-    std::vector<std::string> btag;    // K*3 boundary tag name ("" if none / interior)
+    std::vector<int> tri;             // K*nv
+    std::vector<int> nbr;             // K*nv*2  (elem, face) or -1
+    std::vector<std::string> btag;    // K*nv boundary tag name ("" if none / interior)
     std::vector<double> qxi, qeta, qw, gs, gw;
     std::vector<double> phi_vol;      // nq*np
     std::vector<double> dphi_ref;     // nq*np*2
-    std::vector<double> phi_face;     // 3*nfq*np
+    std::vector<double> phi_face;     // nv*nfq*np
     std::vector<ElemData> ed;
-    std::vector<FaceData> fd;         // K*3
+    std::vector<FaceData> fd;         // K*nv
     std::vector<IFace> ifaces;
     std::vector<BFace> bfaces;
     std::map<std::string, BC> bcs;
@@ -395,20 +471,21 @@ struct Ctx {
 // Connectivity through a map keyed by the sorted vertex pair (S/core/mesh.cpp:312-359 builds
 // the same neighbour table with an O(K^2) rescan; this one is O(K log K)).
 void build_connectivity(Ctx& c) {
-    c.nbr.assign(c.K * 6, -1);
+    const int nv = c.nv;
+    c.nbr.assign(c.K * nv * 2, -1);
     std::map<std::pair<int, int>, std::vector<std::pair<int, int>>> fm;
     for (int e = 0; e < c.K; ++e)
-        for (int f = 0; f < 3; ++f) {
-            int v1 = c.tri[e * 3 + f], v2 = c.tri[e * 3 + (f + 1) % 3];
+        for (int f = 0; f < nv; ++f) {
+            int v1 = c.tri[e * nv + f], v2 = c.tri[e * nv + (f + 1) % nv];
             fm[{std::min(v1, v2), std::max(v1, v2)}].push_back({e, f});
         }
     for (auto& kv : fm) {
         if (kv.second.size() == 2) {
             auto a = kv.second[0], b = kv.second[1];
-            c.nbr[(a.first * 3 + a.second) * 2] = b.first;
-            c.nbr[(a.first * 3 + a.second) * 2 + 1] = b.second;
-            c.nbr[(b.first * 3 + b.second) * 2] = a.first;
-            c.nbr[(b.first * 3 + b.second) * 2 + 1] = a.second;
+            c.nbr[(a.first * nv + a.second) * 2] = b.first;
+            c.nbr[(a.first * nv + a.second) * 2 + 1] = b.second;
+            c.nbr[(b.first * nv + b.second) * 2] = a.first;
+            c.nbr[(b.first * nv + b.second) * 2 + 1] = a.second;
         }
     }
 }
@@ -422,29 +499,31 @@ void tag_boundaries(Ctx& c) {
         ymin = std::min(ymin, c.vy[i]); ymax = std::max(ymax, c.vy[i]);
     }
     double tol = 1e-10 * std::max(xmax - xmin, ymax - ymin);
-    c.btag.assign(c.K * 3, "");
+    const int nv = c.nv;
+    c.btag.assign(c.K * nv, "");
     for (int e = 0; e < c.K; ++e)
-        for (int f = 0; f < 3; ++f) {
-            if (c.nbr[(e * 3 + f) * 2] >= 0) continue;
-            int v1 = c.tri[e * 3 + f], v2 = c.tri[e * 3 + (f + 1) % 3];
+        for (int f = 0; f < nv; ++f) {
+            if (c.nbr[(e * nv + f) * 2] >= 0) continue;
+            int v1 = c.tri[e * nv + f], v2 = c.tri[e * nv + (f + 1) % nv];
             double x1 = c.vx[v1], x2 = c.vx[v2], y1 = c.vy[v1], y2 = c.vy[v2];
             std::string t;
             if (std::fabs(y1 - ymin) < tol && std::fabs(y2 - ymin) < tol) t = "Bottom";
             else if (std::fabs(x1 - xmax) < tol && std::fabs(x2 - xmax) < tol) t = "Right";
             else if (std::fabs(y1 - ymax) < tol && std::fabs(y2 - ymax) < tol) t = "Top";
             else if (std::fabs(x1 - xmin) < tol && std::fabs(x2 - xmin) < tol) t = "Left";
-            c.btag[e * 3 + f] = t;
+            c.btag[e * nv + f] = t;
         }
 }
 
 // Periodic pairing by the transverse coordinate of the face midpoint (S/core/mesh.cpp:482-588:
 // Left/Right match y, Bottom/Top match x).  Sorted instead of the reference's O(n^2) scan.
 void pair_periodic(Ctx& c, const char* t1, const char* t2, bool match_y) {
+    const int nv = c.nv;
     std::vector<std::pair<double, int>> a, b;
-    for (int k = 0; k < c.K * 3; ++k) {
+    for (int k = 0; k < c.K * nv; ++k) {
         if (c.nbr[k * 2] >= 0) continue;
-        int e = k / 3, f = k % 3;
-        int v1 = c.tri[e * 3 + f], v2 = c.tri[e * 3 + (f + 1) % 3];
+        int e = k / nv, f = k % nv;
+        int v1 = c.tri[e * nv + f], v2 = c.tri[e * nv + (f + 1) % nv];
         double m = match_y ? 0.5 * (c.vy[v1] + c.vy[v2]) : 0.5 * (c.vx[v1] + c.vx[v2]);
         if (c.btag[k] == t1) a.push_back({m, k});
         else if (c.btag[k] == t2) b.push_back({m, k});
@@ -455,17 +534,29 @@ void pair_periodic(Ctx& c, const char* t1, const char* t2, bool match_y) {
     std::sort(b.begin(), b.end());
     for (size_t i = 0; i < a.size(); ++i) {
         int ka = a[i].second, kb = b[i].second;
-        c.nbr[ka * 2] = kb / 3; c.nbr[ka * 2 + 1] = kb % 3;
-        c.nbr[kb * 2] = ka / 3; c.nbr[kb * 2 + 1] = ka % 3;
+        c.nbr[ka * 2] = kb / nv; c.nbr[ka * 2 + 1] = kb % nv;
+        c.nbr[kb * 2] = ka / nv; c.nbr[kb * 2 + 1] = ka % nv;
         c.btag[ka] = ""; c.btag[kb] = "";
     }
 }
 
 void build_space(Ctx& c, int vol_rule) {
-    // Volume rule exact to degree 2p, face rule GL with p+1 points (S/core/space.cpp:37-46).
-    int deg = 2 * c.p + (vol_rule ? 1 : 0);
-    triangle_rule(deg, c.qxi, c.qeta, c.qw);
+    // Triangles: volume rule exact to degree 2p; quads: tensor Gauss-Legendre with p+1 points
+    // per direction, j outer (S/quadrature/factory.cpp:33-59); faces: GL with p+1 points
+    // (S/core/space.cpp:37-55).
     gauss_legendre(c.p + 1, c.gs, c.gw);
+    if (c.nv == 4) {
+        c.qxi.clear(); c.qeta.clear(); c.qw.clear();
+        for (int j = 0; j <= c.p; ++j)
+            for (int i = 0; i <= c.p; ++i) {
+                c.qxi.push_back(c.gs[i]);
+                c.qeta.push_back(c.gs[j]);
+                c.qw.push_back(c.gw[i] * c.gw[j]);
+            }
+    } else {
+        int deg = 2 * c.p + (vol_rule ? 1 : 0);
+        triangle_rule(deg, c.qxi, c.qeta, c.qw);
+    }
     c.nq = (int)c.qw.size();
     c.nfq = (int)c.gw.size();
     c.phi_vol.assign(c.nq * c.np, 0);
@@ -474,26 +565,27 @@ void build_space(Ctx& c, int vol_rule) {
         c.basis.eval(c.qxi[q], c.qeta[q], &c.phi_vol[q * c.np]);
         c.basis.grad(c.qxi[q], c.qeta[q], &c.dphi_ref[q * c.np * 2]);
     }
-    c.phi_face.assign(3 * c.nfq * c.np, 0);
-    for (int f = 0; f < 3; ++f)
+    c.phi_face.assign(c.nv * c.nfq * c.np, 0);
+    for (int f = 0; f < c.nv; ++f)
         for (int q = 0; q < c.nfq; ++q) {
             double x, y;
-            face_point(f, c.gs[q], x, y);
+            face_point(c.nv, f, c.gs[q], x, y);
             c.basis.eval(x, y, &c.phi_face[(f * c.nfq + q) * c.np]);
         }
 }
 
 void build_geometry(Ctx& c) {
+    const int nv = c.nv;
     c.ed.resize(c.K);
-    c.fd.resize(c.K * 3);
+    c.fd.resize(c.K * nv);
     std::vector<double> g(c.np * 2);
     for (int e = 0; e < c.K; ++e) {
         ElemData& E = c.ed[e];
-        for (int k = 0; k < 3; ++k) { E.V[2 * k] = c.vx[c.tri[e * 3 + k]]; E.V[2 * k + 1] = c.vy[c.tri[e * 3 + k]]; }
+        for (int k = 0; k < nv; ++k) { E.V[2 * k] = c.vx[c.tri[e * nv + k]]; E.V[2 * k + 1] = c.vy[c.tri[e * nv + k]]; }
         E.jdet.resize(c.nq);
         E.dphi_dx.resize(c.nq * c.np * 2);
         for (int q = 0; q < c.nq; ++q) {
-            Mapping m = compute_mapping(E.V, c.qxi[q], c.qeta[q]);
+            Mapping m = compute_mapping(nv, E.V, c.qxi[q], c.qeta[q]);
             E.jdet[q] = m.det;
             for (int i = 0; i < c.np; ++i) {
                 const double* gr = &c.dphi_ref[(q * c.np + i) * 2];
@@ -501,10 +593,14 @@ void build_geometry(Ctx& c) {
                 E.dphi_dx[(q * c.np + i) * 2 + 1] = m.A[2] * gr[0] + m.A[3] * gr[1];
             }
         }
-        double cx = (E.V[0] + E.V[2] + E.V[4]) / 3.0, cy = (E.V[1] + E.V[3] + E.V[5]) / 3.0;
-        for (int f = 0; f < 3; ++f) {
-            FaceData& F = c.fd[e * 3 + f];
-            int a = f, b = (f + 1) % 3;
+        // centroid = the vertex mean (colwise().mean())
+        double cx = 0.0, cy = 0.0;
+        for (int k = 0; k < nv; ++k) { cx += E.V[2 * k]; cy += E.V[2 * k + 1]; }
+        cx /= nv;
+        cy /= nv;
+        for (int f = 0; f < nv; ++f) {
+            FaceData& F = c.fd[e * nv + f];
+            int a = f, b = (f + 1) % nv;
             double v1x = E.V[2 * a], v1y = E.V[2 * a + 1], v2x = E.V[2 * b], v2y = E.V[2 * b + 1];
             double tx = v2x - v1x, ty = v2y - v1y;
             double len = std::sqrt(tx * tx + ty * ty);
@@ -520,8 +616,8 @@ void build_geometry(Ctx& c) {
                 F.qp_flat[q] = (1.0 - t) * v1x + t * v2x;
                 F.qp_flat[q + c.nfq] = (1.0 - t) * v1y + t * v2y;
                 double xi, eta;
-                face_point(f, c.gs[q], xi, eta);
-                Mapping m = compute_mapping(E.V, xi, eta);
+                face_point(nv, f, c.gs[q], xi, eta);
+                Mapping m = compute_mapping(nv, E.V, xi, eta);
                 c.basis.grad(xi, eta, g.data());
                 for (int i = 0; i < c.np; ++i) {
                     int row = q * c.np + i;
@@ -537,7 +633,7 @@ void build_geometry(Ctx& c) {
 std::vector<int> face_permutation(const Ctx& c, int eL, int fL, int eR, int fR) {
     const double* VL = c.ed[eL].V;
     const double* VR = c.ed[eR].V;
-    int a1 = fL, a2 = (fL + 1) % 3, b1 = fR, b2 = (fR + 1) % 3;
+    int a1 = fL, a2 = (fL + 1) % c.nv, b1 = fR, b2 = (fR + 1) % c.nv;
     double e1v1[2] = {VL[2 * a1], VL[2 * a1 + 1]}, e1v2[2] = {VL[2 * a2], VL[2 * a2 + 1]};
     double e2v1[2] = {VR[2 * b1], VR[2 * b1 + 1]}, e2v2[2] = {VR[2 * b2], VR[2 * b2 + 1]};
     auto dist = [](const double* a, const double* b) { return std::hypot(a[0] - b[0], a[1] - b[1]); };
@@ -563,15 +659,15 @@ void build_face_lists(Ctx& c) {
     c.bfaces.clear();
     std::set<std::pair<int, int>> done;
     for (int eL = 0; eL < c.K; ++eL)
-        for (int fL = 0; fL < 3; ++fL) {
-            int eR = c.nbr[(eL * 3 + fL) * 2], fR = c.nbr[(eL * 3 + fL) * 2 + 1];
+        for (int fL = 0; fL < c.nv; ++fL) {
+            int eR = c.nbr[(eL * c.nv + fL) * 2], fR = c.nbr[(eL * c.nv + fL) * 2 + 1];
             if (eR >= 0) {
                 auto sig = std::make_pair(std::min(eL, eR), std::max(eL, eR));
                 if (done.count(sig)) continue;
                 done.insert(sig);
                 c.ifaces.push_back({eL, fL, eR, fR, face_permutation(c, eL, fL, eR, fR)});
             } else {
-                c.bfaces.push_back({eL, fL, c.btag[eL * 3 + fL]});
+                c.bfaces.push_back({eL, fL, c.btag[eL * c.nv + fL]});
             }
         }
 }
@@ -671,7 +767,7 @@ void interior_face_residual(const Ctx& c, const IFace& fc, const double* uL, con
     const int np = c.np;
     std::fill(RL, RL + np * 4, 0.0);
     std::fill(RR, RR + np * 4, 0.0);
-    const FaceData& FL = c.fd[fc.eL * 3 + fc.fL];
+    const FaceData& FL = c.fd[fc.eL * c.nv + fc.fL];
     const double* phL = &c.phi_face[fc.fL * c.nfq * np];
     const double* phR = &c.phi_face[fc.fR * c.nfq * np];
     for (int q = 0; q < c.nfq; ++q) {
@@ -702,8 +798,8 @@ void viscous_interior_face_residual(const Ctx& c, const IFace& fc, const double*
     const int np = c.np;
     std::fill(RL, RL + np * 4, 0.0);
     std::fill(RR, RR + np * 4, 0.0);
-    const FaceData& FL = c.fd[fc.eL * 3 + fc.fL];
-    const FaceData& FR = c.fd[fc.eR * 3 + fc.fR];
+    const FaceData& FL = c.fd[fc.eL * c.nv + fc.fL];
+    const FaceData& FR = c.fd[fc.eR * c.nv + fc.fR];
     const double* phL = &c.phi_face[fc.fL * c.nfq * np];
     const double* phR = &c.phi_face[fc.fR * c.nfq * np];
     double sigma = penalty(c, FL.length);
@@ -749,7 +845,7 @@ void viscous_interior_face_residual(const Ctx& c, const IFace& fc, const double*
 void boundary_face_residual(const Ctx& c, const BFace& bf, const BC& bc, const double* u, double* R) {
     const int np = c.np;
     std::fill(R, R + np * 4, 0.0);
-    const FaceData& F = c.fd[bf.e * 3 + bf.f];
+    const FaceData& F = c.fd[bf.e * c.nv + bf.f];
     const double* ph = &c.phi_face[bf.f * c.nfq * np];
     const double* n = F.normal;
     for (int q = 0; q < c.nfq; ++q) {
@@ -782,7 +878,7 @@ void boundary_face_residual(const Ctx& c, const BFace& bf, const BC& bc, const d
 void viscous_boundary_face_residual(const Ctx& c, const BFace& bf, const BC* bc, const double* u, double* R) {
     const int np = c.np;
     std::fill(R, R + np * 4, 0.0);
-    const FaceData& F = c.fd[bf.e * 3 + bf.f];
+    const FaceData& F = c.fd[bf.e * c.nv + bf.f];
     const double* ph = &c.phi_face[bf.f * c.nfq * np];
     const double* n = F.normal;
     double sigma = penalty(c, F.length);
@@ -1003,9 +1099,12 @@ const char* ref_last_error(void) { return g_err.c_str(); }
 
 void* ref_create(int nv, const double* verts, int K, const int32_t* elems, int order, int basis,
                  int vol_rule, int periodic_mask, int physics, double gamma, double mu,
-                 double prandtl, double sigma0, int ns_face_grad) {
+                 double prandtl, double sigma0, int ns_face_grad, int nvpe, int legendre_endpoint_ref) {
     try {
         if (order < 1 || order > 4) throw std::invalid_argument("order must be 1..4");
+        if (nvpe != 3 && nvpe != 4) throw std::invalid_argument("elements must be triangles or quads");
+        // the reference's quad space always uses the Legendre basis (S/core/space.cpp:309-318)
+        if ((nvpe == 4) != (basis == 2)) throw std::invalid_argument("Legendre basis on quads only");
         if (physics == 1) {  // NavierStokesWeakFormulation ctor checks (:18-34)
             if (mu <= 0) throw std::invalid_argument("Dynamic viscosity must be positive");
             if (prandtl <= 0) throw std::invalid_argument("Prandtl number must be positive");
@@ -1014,14 +1113,16 @@ void* ref_create(int nv, const double* verts, int K, const int32_t* elems, int o
         auto* c = new Ctx();
         c->K = K;
         c->p = order;
+        c->nv = nvpe;
         c->basis = Basis(basis, order);
+        c->basis.ref_endpoint = legendre_endpoint_ref != 0;
         c->np = c->basis.np;
         c->physics = physics;
         c->ns_face_grad = ns_face_grad;
         c->ns = {gamma, mu, prandtl, sigma0, 1.0};
         c->vx.resize(nv); c->vy.resize(nv);
         for (int i = 0; i < nv; ++i) { c->vx[i] = verts[2 * i]; c->vy[i] = verts[2 * i + 1]; }
-        c->tri.assign(elems, elems + 3 * K);
+        c->tri.assign(elems, elems + (size_t)nvpe * K);
         build_connectivity(*c);
         tag_boundaries(*c);
         if (periodic_mask & 1) pair_periodic(*c, "Left", "Right", true);
@@ -1116,10 +1217,9 @@ int ref_volume_points(void* h, double* xy) {
         Ctx& c = *static_cast<Ctx*>(h);
         for (int e = 0; e < c.K; ++e)
             for (int q = 0; q < c.nq; ++q) {
-                const double* V = c.ed[e].V;
-                double N0 = 1.0 - c.qxi[q] - c.qeta[q], N1 = c.qxi[q], N2 = c.qeta[q];
-                xy[((size_t)e * c.nq + q) * 2] = N0 * V[0] + N1 * V[2] + N2 * V[4];
-                xy[((size_t)e * c.nq + q) * 2 + 1] = N0 * V[1] + N1 * V[3] + N2 * V[5];
+                Mapping m = compute_mapping(c.nv, c.ed[e].V, c.qxi[q], c.qeta[q]);
+                xy[((size_t)e * c.nq + q) * 2] = m.x;
+                xy[((size_t)e * c.nq + q) * 2 + 1] = m.y;
             }
     })
 }
@@ -1144,8 +1244,8 @@ double ref_max_cfl(void* h, const double* u, double dt) {
     for (int e = 0; e < c.K; ++e) {
         const double* V = c.ed[e].V;
         double he = 1e300;
-        for (int i = 0; i < 3; ++i)
-            for (int j = i + 1; j < 3; ++j) he = std::min(he, std::hypot(V[2 * i] - V[2 * j], V[2 * i + 1] - V[2 * j + 1]));
+        for (int i = 0; i < c.nv; ++i)
+            for (int j = i + 1; j < c.nv; ++j) he = std::min(he, std::hypot(V[2 * i] - V[2 * j], V[2 * i + 1] - V[2 * j + 1]));
         double ws = 0.0;
         for (int q = 0; q < c.nq; ++q) {
             double U[4] = {0, 0, 0, 0};
@@ -1228,22 +1328,23 @@ int ref_gauss_legendre(int n, double* x, double* w) {
 }
 
 int ref_basis_eval(int kind, int order, double xi, double eta, double* phi, double* grad) {
-    Basis b(kind, order);
+    Basis b(kind % 4, order);
+    b.ref_endpoint = kind < 4;  // kind + 4: Legendre derivative with P'_n(+1) = n(n+1)/2
     if (phi) b.eval(xi, eta, phi);
     if (grad) b.grad(xi, eta, grad);
     return b.np;
 }
 
-int ref_mapping(const double* V, double xi, double eta, double* det, double* A, double* x2) {
+int ref_mapping(int nvpe, const double* V, double xi, double eta, double* det, double* A, double* x2) {
     REF_TRY({
-        Mapping m = compute_mapping(V, xi, eta);
+        Mapping m = compute_mapping(nvpe, V, xi, eta);
         *det = m.det;
         for (int k = 0; k < 4; ++k) A[k] = m.A[k];
         x2[0] = m.x; x2[1] = m.y;
     })
 }
 
-int ref_face_point(int f, double s, double* xi2) { REF_TRY(face_point(f, s, xi2[0], xi2[1])) }
+int ref_face_point(int nvpe, int f, double s, double* xi2) { REF_TRY(face_point(nvpe, f, s, xi2[0], xi2[1])) }
 
 int ref_elem_data(void* h, int e, double* jdet, double* dphi) {
     Ctx& c = *static_cast<Ctx*>(h);
@@ -1254,7 +1355,7 @@ int ref_elem_data(void* h, int e, double* jdet, double* dphi) {
 
 int ref_face_geom(void* h, int e, int f, double* n2, double* len) {
     Ctx& c = *static_cast<Ctx*>(h);
-    const FaceData& F = c.fd[e * 3 + f];
+    const FaceData& F = c.fd[e * c.nv + f];
     n2[0] = F.normal[0]; n2[1] = F.normal[1]; *len = F.length;
     return 0;
 }

@@ -27,12 +27,14 @@ extern "C" {
 typedef void (*ref_bc_fn)(double x, double y, double* out4, void* user);
 
 // Mesh + space + physics.  periodic_mask bit0 = pair Left/Right, bit1 = pair Bottom/Top.
-// basis: 0 = monomial (reference default), 1 = Dubiner.  vol_rule: 0 = degree 2p (reference),
-// 1 = degree 2p+1.  physics: 0 = Euler, 1 = Navier-Stokes.  ns_face_grad: 0 = reference
-// (scrambled Map read of dphi_dx_face), 1 = correct gradients.
+// basis: 0 = monomial (reference default), 1 = Dubiner (triangles), 2 = Legendre (quads).
+// vol_rule: 0 = degree 2p (reference), 1 = degree 2p+1 (triangles).  physics: 0 = Euler,
+// 1 = Navier-Stokes.  ns_face_grad: 0 = reference (scrambled Map read of dphi_dx_face),
+// 1 = correct gradients.  nvpe: 3 = triangles, 4 = quads (elems [K][nvpe]).
+// legendre_endpoint_ref: 1 = the reference's Legendre derivative at x = +1 (quirk Q16).
 void* ref_create(int nv, const double* verts, int K, const int32_t* elems, int order, int basis,
                  int vol_rule, int periodic_mask, int physics, double gamma, double mu,
-                 double prandtl, double sigma0, int ns_face_grad);
+                 double prandtl, double sigma0, int ns_face_grad, int nvpe, int legendre_endpoint_ref);
 void ref_destroy(void* h);
 const char* ref_last_error(void);
 
@@ -63,10 +65,11 @@ double ref_kinetic_energy(void* h, const double* u);
 // Tables for known-answer tests.
 int ref_quad_tri(int degree, double* xi, double* eta, double* w);  // returns n points
 int ref_gauss_legendre(int n, double* x, double* w);
+// kind 0 monomial, 1 Dubiner, 2 Legendre (6: Legendre with P'_n(+1) = n(n+1)/2); returns Np
 int ref_basis_eval(int kind, int order, double xi, double eta, double* phi, double* grad);
-int ref_mapping(const double* verts3x2, double xi, double eta, double* det, double* dxi_dx4,
+int ref_mapping(int nvpe, const double* verts, double xi, double eta, double* det, double* dxi_dx4,
                 double* x2);
-int ref_face_point(int face, double s, double* xi2);
+int ref_face_point(int nvpe, int face, double s, double* xi2);
 
 // Element / face level pieces (the weak-form virtuals), for KATs.
 int ref_elem_data(void* h, int e, double* jdet, double* dphi_dx);  // jdet[Nq], dphi_dx[Nq*Np*2]

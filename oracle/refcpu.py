@@ -36,7 +36,8 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.ref_create.restype = C.c_void_p
         L.ref_create.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, C.c_int, C.c_int, C.c_int,
-                                 C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+                                 C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                 C.c_int, C.c_int]
         L.ref_destroy.argtypes = [C.c_void_p]
         L.ref_last_error.restype = C.c_char_p
         L.ref_info.argtypes = [C.c_void_p, ip]
@@ -59,8 +60,8 @@ def lib():
         L.ref_quad_tri.argtypes = [C.c_int, dp, dp, dp]
         L.ref_gauss_legendre.argtypes = [C.c_int, dp, dp]
         L.ref_basis_eval.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp, dp]
-        L.ref_mapping.argtypes = [dp, C.c_double, C.c_double, dp, dp, dp]
-        L.ref_face_point.argtypes = [C.c_int, C.c_double, dp]
+        L.ref_mapping.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp, dp, dp]
+        L.ref_face_point.argtypes = [C.c_int, C.c_int, C.c_double, dp]
         L.ref_elem_data.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.ref_face_geom.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp]
         L.ref_interior_faces.argtypes = [C.c_void_p, ip]
@@ -89,6 +90,26 @@ def _ip(a):
 # ----------------------------------------------------------------------------------------------
 # Synthetic inputs
 # ----------------------------------------------------------------------------------------------
+def rect_quad_mesh(xmin, xmax, ymin, ymax, dx):
+    """Structured quads on [xmin, xmax] x [ymin, ymax] as the reference's transfinite
+    create_rectangular_mesh(dx, use_triangles=false) lays them out (S/utils/mesh_creation.cpp:
+    212-243: nx = int((xmax - xmin)/dx), ny = int((ymax - ymin)/dx) divisions): vertex (i, j) at
+    (xmin + i (xmax - xmin)/nx, ymin + j (ymax - ymin)/ny), quad (i, j) = (v_ij, v_i+1j, v_i+1j+1,
+    v_ij+1) counter-clockwise, row-major (j outer)."""
+    nx, ny = int((xmax - xmin) / dx), int((ymax - ymin) / dx)
+    hx, hy = (xmax - xmin) / nx, (ymax - ymin) / ny
+    X, Y = np.meshgrid(xmin + np.arange(nx + 1) * hx, ymin + np.arange(ny + 1) * hy, indexing="xy")
+    verts = np.stack([X.ravel(), Y.ravel()], axis=1).astype(np.float64)
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+
+    def vid(i, j):
+        return j * (nx + 1) + i
+
+    quads = np.stack([vid(ii, jj), vid(ii + 1, jj), vid(ii + 1, jj + 1), vid(ii, jj + 1)], axis=1).astype(np.int32)
+    return verts, quads
+
+
 def split_quad_mesh(n, length=10.0, x0=0.0, y0=0.0):
     """N x N squares on [x0, x0+L] x [y0, y0+L]; square (i,j) split along its (i,j)->(i+1,j+1)
     diagonal into T1 = (v_ij, v_i+1j, v_i+1j+1) and T2 = (v_ij, v_i+1j+1, v_ij+1), both CCW,
@@ -171,17 +192,22 @@ class Oracle:
     """FP64 CPU restatement of the reference's DGMesh + DGSpace + weak form + assembler +
     SSP-RK3 driver.  State layout [K][Np][4]."""
 
-    def __init__(self, verts, tris, order, basis="monomial", vol_rule=0, periodic=3,
+    def __init__(self, verts, tris, order, basis=None, vol_rule=0, periodic=3,
                  physics="euler", gamma=1.4, mu=1e-3, prandtl=0.72, sigma0=5.0,
-                 ns_face_grad="reference", euler_bc_points="reference"):
+                 ns_face_grad="reference", euler_bc_points="reference", legendre_endpoint="reference"):
         L = lib()
         self._verts = np.ascontiguousarray(verts, dtype=np.float64)
         self._tris = np.ascontiguousarray(tris, dtype=np.int32)
         self.gamma = gamma
+        nvpe = self._tris.shape[1]
+        self.nvpe = nvpe
+        # the reference's spaces: monomial basis on triangles, Legendre on quads
+        basis = basis or ("legendre" if nvpe == 4 else "monomial")
         self._h = L.ref_create(len(self._verts), _dp(self._verts), len(self._tris), _ip(self._tris),
-                               order, {"monomial": 0, "dubiner": 1}[basis], vol_rule, periodic,
+                               order, {"monomial": 0, "dubiner": 1, "legendre": 2}[basis], vol_rule, periodic,
                                {"euler": 0, "ns": 1}[physics], gamma, mu, prandtl, sigma0,
-                               {"reference": 0, "correct": 1}[ns_face_grad])
+                               {"reference": 0, "correct": 1}[ns_face_grad], nvpe,
+                               {"reference": 1, "correct": 0}[legendre_endpoint])
         if not self._h:
             raise RuntimeError(L.ref_last_error().decode())
         info = np.zeros(6, dtype=np.int32)
@@ -364,28 +390,30 @@ def gauss_legendre(n):
     return x, w
 
 
-def basis_eval(kind, order, xi, eta):
-    np_ = (order + 1) * (order + 2) // 2
+def basis_eval(kind, order, xi, eta, legendre_endpoint="reference"):
+    np_ = (order + 1) ** 2 if kind == "legendre" else (order + 1) * (order + 2) // 2
     phi = np.zeros(np_)
     g = np.zeros(2 * np_)
-    lib().ref_basis_eval({"monomial": 0, "dubiner": 1}[kind], order, xi, eta, _dp(phi), _dp(g))
+    k = {"monomial": 0, "dubiner": 1, "legendre": 2}[kind] + (4 if legendre_endpoint == "correct" else 0)
+    lib().ref_basis_eval(k, order, xi, eta, _dp(phi), _dp(g))
     return phi, g.reshape(np_, 2)
 
 
-def mapping(verts3x2, xi, eta):
-    V = np.ascontiguousarray(verts3x2, dtype=np.float64).reshape(6)
+def mapping(verts, xi, eta):
+    """verts [3][2] (triangle) or [4][2] (quad)."""
+    V = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
     det = np.zeros(1)
     A = np.zeros(4)
     x = np.zeros(2)
-    rc = lib().ref_mapping(_dp(V), xi, eta, _dp(det), _dp(A), _dp(x))
+    rc = lib().ref_mapping(len(V) // 2, _dp(V), xi, eta, _dp(det), _dp(A), _dp(x))
     if rc:
         raise RuntimeError(lib().ref_last_error().decode())
     return float(det[0]), A.reshape(2, 2), x
 
 
-def face_point(f, s):
+def face_point(f, s, nvpe=3):
     o = np.zeros(2)
-    lib().ref_face_point(f, s, _dp(o))
+    lib().ref_face_point(nvpe, f, s, _dp(o))
     return o
 
 
