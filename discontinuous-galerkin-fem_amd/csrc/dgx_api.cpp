@@ -66,8 +66,8 @@ void upload_bc_table(dgx_ctx& c) {
     std::vector<double> flat(2 * nfq);
     for (int bf = 0; bf < c.n_bfaces; ++bf) {
         const auto& B = c.h_bfaces[bf];
-        const double* hv = &c.h_verts[(size_t)B.e * 6];
-        const int a = B.f, b = (B.f + 1) % 3;
+        const double* hv = &c.h_verts[(size_t)B.e * 2 * c.nf];
+        const int a = B.f, b = (B.f + 1) % c.nf;
         for (int q = 0; q < nfq; ++q) {
             const double t = 0.5 * (fr.x[q] + 1.0);
             flat[q] = (1.0 - t) * hv[2 * a] + t * hv[2 * b];
@@ -121,11 +121,289 @@ void build_geometry(dgx_ctx& c, const dgx_mesh_desc& m) {
     if (err & 2) throw std::invalid_argument("Degenerate edge detected");
 }
 
+// Inverse of the n x n matrix M (row-major) by Gauss-Jordan elimination with partial pivoting.
+std::vector<double> invert_gj(std::vector<double> M, int n) {
+    std::vector<double> I(n * n, 0.0);
+    for (int i = 0; i < n; ++i) I[i * n + i] = 1.0;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(M[r * n + col]) > std::fabs(M[piv * n + col])) piv = r;
+        if (M[piv * n + col] == 0.0) throw std::runtime_error("singular mass matrix");
+        if (piv != col)
+            for (int k = 0; k < n; ++k) { std::swap(M[col * n + k], M[piv * n + k]); std::swap(I[col * n + k], I[piv * n + k]); }
+        const double d = 1.0 / M[col * n + col];
+        for (int k = 0; k < n; ++k) { M[col * n + k] *= d; I[col * n + k] *= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = M[r * n + col];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) { M[r * n + k] -= f * M[col * n + k]; I[r * n + k] -= f * I[col * n + k]; }
+        }
+    }
+    return I;
+}
+
+// Bilinear map of the reference square (S/reference/elements.cpp:89-108, mapping.cpp:22-50):
+// x = sum N_n X_n, J_T = dN^T X, dxi/dx = inv(J_T).
+struct QuadMap {
+    double x, y, det, A[4];
+};
+QuadMap quad_map(const double* V, double xi, double eta) {
+    const double N[4] = {0.25 * (1.0 - xi) * (1.0 - eta), 0.25 * (1.0 + xi) * (1.0 - eta),
+                         0.25 * (1.0 + xi) * (1.0 + eta), 0.25 * (1.0 - xi) * (1.0 + eta)};
+    const double dN[4][2] = {{-0.25 * (1.0 - eta), -0.25 * (1.0 - xi)}, {0.25 * (1.0 - eta), -0.25 * (1.0 + xi)},
+                             {0.25 * (1.0 + eta), 0.25 * (1.0 + xi)}, {-0.25 * (1.0 + eta), 0.25 * (1.0 - xi)}};
+    QuadMap m{0, 0, 0, {0, 0, 0, 0}};
+    double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
+    for (int n = 0; n < 4; ++n) {
+        m.x += N[n] * V[2 * n];
+        m.y += N[n] * V[2 * n + 1];
+        j00 += dN[n][0] * V[2 * n];
+        j01 += dN[n][0] * V[2 * n + 1];
+        j10 += dN[n][1] * V[2 * n];
+        j11 += dN[n][1] * V[2 * n + 1];
+    }
+    m.det = j00 * j11 - j01 * j10;
+    if (std::fabs(m.det) < 1e-12) throw std::invalid_argument("Singular Jacobian matrix");
+    const double id = 1.0 / m.det;
+    m.A[0] = j11 * id;
+    m.A[1] = -j01 * id;
+    m.A[2] = -j10 * id;
+    m.A[3] = j00 * id;
+    return m;
+}
+
+// The quad space (DGSpace for element type "quad", S/core/space.cpp:47-55): tables, per-point
+// geometry and per-element inverse mass matrices on the host, then uploaded.
+void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
+    if (s.basis != DGX_BASIS_LEGENDRE) throw std::invalid_argument("quadrilateral elements use the Legendre basis");
+    if (s.rule != DGX_RULE_2P) throw Fail(DGX_E_UNSUPPORTED, "quadrilateral elements use the tensor Gauss-Legendre rule");
+    if (precision == DGX_FP32_BF16VOL) throw Fail(DGX_E_UNSUPPORTED, "the BF16-input volume variant covers triangles");
+    const int K = c.K, Kp = c.Kp, p = c.p;
+    const QuadDims D(p);
+    c.nf = 4;
+    c.np = D.np;
+    c.nq = D.nq;
+    c.nfq = D.nfq;
+    c.phi0 = 1.0;
+    const int np = D.np, nq = D.nq, nfq = D.nfq;
+    const Rule1D fr = gauss_legendre(nfq);
+    const QuadBasis B(p, (ph.quirks & DGX_QUIRK_LEGENDRE_ENDPOINT_REFERENCE) != 0);
+
+    // ---------------- tables (tensor points j outer, i inner) ----------------
+    std::vector<double> t(D.total, 0.0), wq(nq), g(np * 2);
+    std::vector<double> qxi(nq), qeta(nq);
+    for (int j = 0; j < nfq; ++j)
+        for (int i = 0; i < nfq; ++i) {
+            const int q = j * nfq + i;
+            qxi[q] = fr.x[i];
+            qeta[q] = fr.x[j];
+            wq[q] = fr.w[i] * fr.w[j];
+            B.values(qxi[q], qeta[q], &t[D.phiV + q * np]);
+            B.gradients(qxi[q], qeta[q], g.data());
+            for (int k = 0; k < np; ++k) {
+                t[D.GXV + q * np + k] = g[2 * k];
+                t[D.GYV + q * np + k] = g[2 * k + 1];
+            }
+        }
+    std::vector<double> phf(np);
+    for (int f = 0; f < 4; ++f)
+        for (int q = 0; q < nfq; ++q) {
+            double xi, eta;
+            face_point_quad(f, fr.x[q], xi, eta);
+            B.values(xi, eta, phf.data());
+            B.gradients(xi, eta, g.data());
+            for (int k = 0; k < np; ++k) {
+                t[D.phiF + (f * nfq + q) * np + k] = phf[k];
+                t[D.LF + (f * nfq + q) * np + k] = -0.5 * fr.w[q] * phf[k];
+                t[D.GF + ((f * nfq + q) * np + k) * 2] = g[2 * k];
+                t[D.GF + ((f * nfq + q) * np + k) * 2 + 1] = g[2 * k + 1];
+            }
+        }
+    c.h_qw = wq;
+
+    // ---------------- connectivity (host) ----------------
+    const double* V = m.vertices;
+    const int32_t* T = m.elements;
+    const int32_t* N = m.face_neighbors;
+    std::vector<int32_t> nbr((size_t)4 * Kp, 0);
+    c.h_verts.assign((size_t)K * 8, 0.0);
+    for (int e = 0; e < K; ++e)
+        for (int k = 0; k < 4; ++k) {
+            const int id = T[e * 4 + k];
+            if (id < 0 || id >= m.n_vertices) throw std::invalid_argument("element references a missing vertex");
+            c.h_verts[(size_t)e * 8 + 2 * k] = V[2 * id];
+            c.h_verts[(size_t)e * 8 + 2 * k + 1] = V[2 * id + 1];
+        }
+    // outward unit normal and length of face f = edge (v_f, v_f+1), centroid = vertex mean
+    // (DGSpace::compute_face_data, S/core/space.cpp:104-137)
+    auto face_geom = [&](int e, int f, double& nx, double& ny, double& len) {
+        const double* hv = &c.h_verts[(size_t)e * 8];
+        const int a = f, b = (f + 1) % 4;
+        const double tx = hv[2 * b] - hv[2 * a], ty = hv[2 * b + 1] - hv[2 * a + 1];
+        len = std::sqrt(tx * tx + ty * ty);
+        if (len < 1e-14) throw std::invalid_argument("Degenerate edge detected");
+        nx = ty / len;
+        ny = -tx / len;
+        const double cx = (hv[0] + hv[2] + hv[4] + hv[6]) / 4.0, cy = (hv[1] + hv[3] + hv[5] + hv[7]) / 4.0;
+        if (nx * (0.5 * (hv[2 * a] + hv[2 * b]) - cx) + ny * (0.5 * (hv[2 * a + 1] + hv[2 * b + 1]) - cy) < 0) {
+            nx = -nx;
+            ny = -ny;
+        }
+    };
+    std::vector<double> geo((size_t)D.rows * Kp, 0.0), hmin(Kp, 1.0), minve((size_t)np * np * Kp, 0.0);
+    std::vector<uint32_t> code(Kp, 0);
+    for (int e = K; e < Kp; ++e)  // padding: unit weights and identity mass
+        for (int q = 0; q < nq; ++q) geo[(size_t)(D.gV + 5 * q + 4) * Kp + e] = 1.0;
+    int nbf = 0;
+    for (int e = 0; e < K; ++e) {
+        int en4[4];
+        for (int f = 0; f < 4; ++f) {
+            const int en = N[(e * 4 + f) * 2], fn = N[(e * 4 + f) * 2 + 1];
+            en4[f] = en;
+            double nx, ny, len;
+            face_geom(e, f, nx, ny, len);
+            uint32_t cf;
+            if (en < 0) {
+                int bci = fn < 0 ? m.n_bcs : fn;
+                if (bci > m.n_bcs) throw std::invalid_argument("boundary face references a missing boundary condition");
+                if (bci < m.n_bcs && m.bc_types && m.bc_types[bci] == DGX_BC_SLIP_WALL) c.has_slip = true;
+                nbr[f * Kp + e] = nbf++;
+                c.h_bfaces.push_back({e, f, bci});
+                cf = (1u << 3) | (1u << 4);
+            } else {
+                if (en >= K || fn < 0 || fn > 3) throw std::invalid_argument("face neighbour out of range");
+                if (N[(en * 4 + fn) * 2] != e || N[(en * 4 + fn) * 2 + 1] != f)
+                    throw std::invalid_argument("face_neighbors is not symmetric");
+                const bool isL = (e < en) || (e == en && f < fn);
+                const int eL = isL ? e : en, fL = isL ? f : fn, eR = isL ? en : e, fR = isL ? fn : f;
+                double nLx, nLy, lenL;
+                face_geom(eL, fL, nLx, nLy, lenL);
+                // compute_face_permutation (S/core/space.cpp:173-228)
+                const double* VL = &c.h_verts[(size_t)eL * 8];
+                const double* VR = &c.h_verts[(size_t)eR * 8];
+                const int a1 = fL, a2 = (fL + 1) % 4, b1 = fR, b2 = (fR + 1) % 4;
+                auto d = [](const double* P, int i, const double* Q, int j) {
+                    const double dx = P[2 * i] - Q[2 * j], dy = P[2 * i + 1] - Q[2 * j + 1];
+                    return std::sqrt(dx * dx + dy * dy);
+                };
+                const double tol = 1e-10;
+                const bool rev = d(VL, a1, VR, b2) < tol && d(VL, a2, VR, b1) < tol;
+                const bool same = d(VL, a1, VR, b1) < tol && d(VL, a2, VR, b2) < tol;
+                bool reverse;
+                if (rev || same) {
+                    reverse = rev;
+                } else {
+                    const double t1x = VL[2 * a2] - VL[2 * a1], t1y = VL[2 * a2 + 1] - VL[2 * a1 + 1];
+                    const double t2x = VR[2 * b2] - VR[2 * b1], t2y = VR[2 * b2 + 1] - VR[2 * b1 + 1];
+                    const double n1 = std::sqrt(t1x * t1x + t1y * t1y), n2 = std::sqrt(t2x * t2x + t2y * t2y);
+                    reverse = (t1x / n1 * t2x / n2 + t1y / n1 * t2y / n2) < 0.0;
+                }
+                nx = isL ? nLx : -nLx;
+                ny = isL ? nLy : -nLy;
+                len = lenL;
+                nbr[f * Kp + e] = en;
+                cf = uint32_t(fn) | (reverse ? 4u : 0u) | (isL ? 8u : 0u);
+            }
+            geo[(size_t)(D.gF + 3 * f) * Kp + e] = nx;
+            geo[(size_t)(D.gF + 3 * f + 1) * Kp + e] = ny;
+            geo[(size_t)(D.gF + 3 * f + 2) * Kp + e] = len;
+            code[e] |= cf << (5 * f);
+        }
+        for (int f = 0; f < 4; ++f) {
+            if (en4[f] < 0) continue;
+            int shared = 0;
+            for (int h = 0; h < 4; ++h) shared += en4[h] == en4[f];
+            if (shared > (en4[f] == e ? 2 : 1))
+                throw Fail(DGX_E_UNSUPPORTED, "two elements share more than one face (periodic mesh with n < 2)");
+        }
+        // volume points: dxi/dx and w |J|; face points: dxi/dx; mass matrix
+        const double* hv = &c.h_verts[(size_t)e * 8];
+        std::vector<double> M(np * np, 0.0);
+        for (int q = 0; q < nq; ++q) {
+            const QuadMap qm = quad_map(hv, qxi[q], qeta[q]);
+            const double wd = wq[q] * std::fabs(qm.det);
+            for (int k = 0; k < 4; ++k) geo[(size_t)(D.gV + 5 * q + k) * Kp + e] = qm.A[k];
+            geo[(size_t)(D.gV + 5 * q + 4) * Kp + e] = wd;
+            const double* ph = &t[D.phiV + q * np];
+            for (int i = 0; i < np; ++i)
+                for (int j = 0; j < np; ++j) M[i * np + j] += wd * (ph[i] * ph[j]);
+        }
+        for (int f = 0; f < 4; ++f)
+            for (int q = 0; q < nfq; ++q) {
+                double xi, eta;
+                face_point_quad(f, fr.x[q], xi, eta);
+                const QuadMap qm = quad_map(hv, xi, eta);
+                for (int k = 0; k < 4; ++k) geo[(size_t)(D.gFA + (f * nfq + q) * 4 + k) * Kp + e] = qm.A[k];
+            }
+        const std::vector<double> Mi = invert_gj(M, np);
+        for (int k = 0; k < np * np; ++k) minve[(size_t)k * Kp + e] = Mi[k];
+        // element size: smallest distance between two vertices (compute_max_cfl,
+        // S/solver/compressible_solver_base.cpp:125-135)
+        double h = 1e300;
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b) {
+                const double dx = hv[2 * a] - hv[2 * b], dy = hv[2 * a + 1] - hv[2 * b + 1];
+                h = std::min(h, std::sqrt(dx * dx + dy * dy));
+            }
+        hmin[e] = h;
+    }
+    for (int e = K; e < Kp; ++e)
+        for (int i = 0; i < np; ++i) minve[(size_t)(i * np + i) * Kp + e] = 1.0;
+    c.n_bfaces = nbf;
+    c.h_bctype.assign(m.n_bcs + 1, -1);
+    c.h_bcval.assign((size_t)(m.n_bcs + 1) * 4, 0.0);
+    c.h_bcfn.assign(m.n_bcs + 1, {nullptr, nullptr});
+    for (int b = 0; b < m.n_bcs; ++b) {
+        const int ty = m.bc_types ? m.bc_types[b] : -1;
+        if (ty == DGX_BC_PERIODIC)
+            throw std::invalid_argument("PERIODIC BC encountered on a boundary face - periodic boundaries should be paired");
+        if (ty < -1 || ty > DGX_BC_NO_SLIP_WALL) throw std::invalid_argument("unknown boundary condition type");
+        c.h_bctype[b] = ty;
+        for (int v = 0; v < 4; ++v) c.h_bcval[b * 4 + v] = m.bc_values ? m.bc_values[b * 4 + v] : 0.0;
+    }
+
+    // ---------------- device ----------------
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Fail(DGX_E_DEVICE, "no HIP device available");
+    hip_check(hipGetDevice(&c.device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&c.ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&c.ev1), "hipEventCreate");
+    c.upload(c.geo, geo.data(), geo.size() * sizeof(double));
+    c.upload(c.hmin, hmin.data(), hmin.size() * sizeof(double));
+    c.upload(c.code, code.data(), code.size() * sizeof(uint32_t));
+    c.upload(c.nbr, nbr.data(), nbr.size() * sizeof(int32_t));
+    c.upload(c.minve, minve.data(), minve.size() * sizeof(double));
+    upload_bc_table(c);
+    c.upload(c.tab64, t.data(), t.size() * sizeof(double));
+    c.upload(c.qw, wq.data(), wq.size() * sizeof(double));
+    if (precision == DGX_FP64) {
+        c.upload_real<double>(c.tabL, t);
+        c.upload_real<double>(c.tabR, t);
+    } else {
+        c.upload_real<float>(c.tabL, t);
+        c.upload_real<float>(c.tabR, t);
+    }
+    c.u.alloc(c.state_bytes());
+    c.u1.alloc(c.state_bytes());
+    c.u2.alloc(c.state_bytes());
+    hip_check(hipMemset(c.u.p, 0, c.state_bytes()), "hipMemset");
+    hip_check(hipMemset(c.u1.p, 0, c.state_bytes()), "hipMemset");
+    hip_check(hipMemset(c.u2.p, 0, c.state_bytes()), "hipMemset");
+    c.stage.alloc((size_t)K * std::max(np, nq) * 4 * sizeof(double));
+    c.partial.alloc((size_t)c.red_blocks() * 8 * sizeof(double));
+}
+
 void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
     if (!m.vertices || !m.elements || !m.face_neighbors || m.n_elements <= 0 || m.n_vertices <= 0)
         throw std::invalid_argument("mesh descriptor is incomplete");
     if (s.order < 1 || s.order > 4) throw std::invalid_argument("Element order must be between 1 and 4");
-    if (s.basis != DGX_BASIS_MONOMIAL && s.basis != DGX_BASIS_DUBINER) throw std::invalid_argument("unknown basis");
+    const int nvpe = m.vertices_per_element == 0 ? 3 : m.vertices_per_element;
+    if (nvpe != 3 && nvpe != 4) throw std::invalid_argument("elements must be triangles or quadrilaterals");
+    if (nvpe == 3 && s.basis == DGX_BASIS_LEGENDRE) throw std::invalid_argument("the Legendre basis is for quadrilaterals");
+    if (nvpe == 3 && s.basis != DGX_BASIS_MONOMIAL && s.basis != DGX_BASIS_DUBINER) throw std::invalid_argument("unknown basis");
     if (s.rule != DGX_RULE_2P && s.rule != DGX_RULE_2P1) throw std::invalid_argument("unknown volume rule");
     if (precision != DGX_FP64 && precision != DGX_FP32 && precision != DGX_FP32_BF16VOL)
         throw std::invalid_argument("unknown precision");
@@ -154,6 +432,10 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
     c.sigma0 = ph.sigma0;
     c.quirks = ph.quirks;
     c.rsz = precision == DGX_FP64 ? 8 : 4;
+    if (nvpe == 4) {
+        build_quad_context(c, m, s, ph, precision);
+        return;
+    }
 
     // ---------------- reference-element tables (FP64) ----------------
     RuleTri vr = dunavant(2 * c.p + c.rule);
@@ -409,18 +691,24 @@ dgx_status dgx_mesh_split_quads(int32_t n, double length, double x0, double y0, 
 
 dgx_status dgx_mesh_connect(int32_t nv, const double* V, int32_t K, const int32_t* T, int32_t periodic_mask,
                             int32_t* N) {
+    return dgx_mesh_connect_elements(nv, V, K, T, 3, periodic_mask, N);
+}
+
+dgx_status dgx_mesh_connect_elements(int32_t nv, const double* V, int32_t K, const int32_t* T, int32_t nf,
+                                     int32_t periodic_mask, int32_t* N) {
     return guarded([&] {
         if (nv <= 0 || K <= 0 || !V || !T || !N) throw std::invalid_argument("bad mesh");
-        for (int k = 0; k < K * 6; ++k) N[k] = -1;
+        if (nf != 3 && nf != 4) throw std::invalid_argument("elements must have 3 or 4 vertices");
+        for (size_t k = 0; k < (size_t)K * nf * 2; ++k) N[k] = -1;
         // sorted-vertex-pair signature -> face (DGMesh::build_face_connectivity, S/core/mesh.cpp:312-359)
         std::vector<std::pair<uint64_t, int>> sig;
-        sig.reserve((size_t)K * 3);
+        sig.reserve((size_t)K * nf);
         for (int e = 0; e < K; ++e)
-            for (int f = 0; f < 3; ++f) {
-                uint32_t a = (uint32_t)T[e * 3 + f], b = (uint32_t)T[e * 3 + (f + 1) % 3];
+            for (int f = 0; f < nf; ++f) {
+                uint32_t a = (uint32_t)T[e * nf + f], b = (uint32_t)T[e * nf + (f + 1) % nf];
                 if ((int)a >= nv || (int)b >= nv) throw std::invalid_argument("element references a missing vertex");
                 uint64_t s = ((uint64_t)std::min(a, b) << 32) | std::max(a, b);
-                sig.push_back({s, e * 3 + f});
+                sig.push_back({s, e * nf + f});
             }
         std::sort(sig.begin(), sig.end());
         for (size_t k = 0; k < sig.size();) {
@@ -428,8 +716,8 @@ dgx_status dgx_mesh_connect(int32_t nv, const double* V, int32_t K, const int32_
             while (j < sig.size() && sig[j].first == sig[k].first) ++j;
             if (j - k == 2) {
                 int A = sig[k].second, B = sig[k + 1].second;
-                N[A * 2] = B / 3; N[A * 2 + 1] = B % 3;
-                N[B * 2] = A / 3; N[B * 2 + 1] = A % 3;
+                N[A * 2] = B / nf; N[A * 2 + 1] = B % nf;
+                N[B * 2] = A / nf; N[B * 2 + 1] = A % nf;
             } else if (j - k > 2) {
                 throw std::invalid_argument("non-manifold mesh: an edge is shared by more than two elements");
             }
@@ -444,10 +732,10 @@ dgx_status dgx_mesh_connect(int32_t nv, const double* V, int32_t K, const int32_
         }
         const double tol = 1e-10 * std::max(xmax - xmin, ymax - ymin);
         std::vector<std::vector<std::pair<double, int>>> side(4);
-        for (int k = 0; k < K * 3; ++k) {
+        for (int k = 0; k < K * nf; ++k) {
             if (N[k * 2] >= 0) continue;
-            int e = k / 3, f = k % 3;
-            int a = T[e * 3 + f], b = T[e * 3 + (f + 1) % 3];
+            int e = k / nf, f = k % nf;
+            int a = T[e * nf + f], b = T[e * nf + (f + 1) % nf];
             double xa = V[2 * a], ya = V[2 * a + 1], xb = V[2 * b], yb = V[2 * b + 1];
             int tag = -1;
             if (std::fabs(ya - ymin) < tol && std::fabs(yb - ymin) < tol) tag = 0;
@@ -467,12 +755,32 @@ dgx_status dgx_mesh_connect(int32_t nv, const double* V, int32_t K, const int32_
             std::sort(B.begin(), B.end());
             for (size_t i = 0; i < A.size(); ++i) {
                 int a = A[i].second, b = B[i].second;
-                N[a * 2] = b / 3; N[a * 2 + 1] = b % 3;
-                N[b * 2] = a / 3; N[b * 2 + 1] = a % 3;
+                N[a * 2] = b / nf; N[a * 2 + 1] = b % nf;
+                N[b * 2] = a / nf; N[b * 2 + 1] = a % nf;
             }
         };
         if (periodic_mask & 1) pair_sides(3, 1);
         if (periodic_mask & 2) pair_sides(0, 2);
+    });
+}
+
+dgx_status dgx_mesh_rect_quads(int32_t nx, int32_t ny, double xmin, double xmax, double ymin, double ymax,
+                               double* vertices, int32_t* elements) {
+    return guarded([&] {
+        if (nx < 1 || ny < 1 || !(xmax > xmin) || !(ymax > ymin) || !vertices || !elements)
+            throw std::invalid_argument("bad mesh size");
+        const double hx = (xmax - xmin) / nx, hy = (ymax - ymin) / ny;
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                vertices[2 * (j * (nx + 1) + i)] = xmin + i * hx;
+                vertices[2 * (j * (nx + 1) + i) + 1] = ymin + j * hy;
+            }
+        auto vid = [nx](int i, int j) { return j * (nx + 1) + i; };
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                int32_t* q = elements + 4 * (j * nx + i);
+                q[0] = vid(i, j); q[1] = vid(i + 1, j); q[2] = vid(i + 1, j + 1); q[3] = vid(i, j + 1);
+            }
     });
 }
 
@@ -542,6 +850,18 @@ dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
     return guarded([&] {
         if (!c || !xy) throw std::invalid_argument("null argument");
         // GeometricMapping::map_to_physical (S/reference/mapping.cpp:52-58)
+        if (c->nf == 4) {
+            const Rule1D fr = gauss_legendre(c->nfq);
+            for (int e = 0; e < c->K; ++e)
+                for (int j = 0; j < c->nfq; ++j)
+                    for (int i = 0; i < c->nfq; ++i) {
+                        const QuadMap qm = quad_map(&c->h_verts[(size_t)e * 8], fr.x[i], fr.x[j]);
+                        const size_t o = ((size_t)e * c->nq + j * c->nfq + i) * 2;
+                        xy[o] = qm.x;
+                        xy[o + 1] = qm.y;
+                    }
+            return;
+        }
         RuleTri vr = dunavant(2 * c->p + c->rule);
         for (int e = 0; e < c->K; ++e) {
             const double* V = &c->h_verts[(size_t)e * 6];
@@ -557,6 +877,7 @@ dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
 dgx_status dgx_get_geometry(dgx_ctx* c, double* geo, double* hmin, uint32_t* codes) {
     return guarded([&] {
         if (!c) throw std::invalid_argument("null argument");
+        if (c->nf == 4) throw Fail(DGX_E_UNSUPPORTED, "the per-element geometry layout is the triangles'");
         const size_t K = c->K, Kp = c->Kp;
         if (geo) {
             std::vector<double> g((size_t)kGeoRows * Kp);
@@ -590,18 +911,18 @@ dgx_status dgx_set_bc_function(dgx_ctx* c, int32_t bc_index, dgx_bc_fn fn, void*
 dgx_status dgx_l2_error(dgx_ctx* c, const double* exact, int32_t exact_is_primitive, double err[4]) {
     return guarded([&] {
         if (!c || !exact || !err) throw std::invalid_argument("null argument");
-        TabLayout L(c->p, c->rule);
         const size_t n = (size_t)c->K * c->nq * 4;
         DevBuf ex, part;
         ex.alloc(n * sizeof(double));
         const int nb = c->red_blocks();
         part.alloc((size_t)nb * 8 * sizeof(double));
         hip_check(hipMemcpyAsync(ex.p, exact, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
-        hip_check(launch_eval(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>() + L.phiV, c->K, c->Kp,
-                              c->stage.as<double>(), c->uref, c->stream),
+        hip_check(launch_eval_n(c->precision, c->np, c->nq, c->u.p, c->tab64.as<double>(), c->K, c->Kp,
+                                c->stage.as<double>(), c->uref, c->stream),
                   "eval");
         hip_check(launch_l2_error(c->stage.as<double>(), ex.as<double>(), exact_is_primitive ? 1 : 0, c->geo.as<double>(),
-                                  c->qw.as<double>(), c->nq, c->K, c->Kp, c->gamma, part.as<double>(), nb, c->stream),
+                                  c->qw.as<double>(), c->nq, c->K, c->Kp, c->gamma, part.as<double>(), nb,
+                                  c->nf == 4 ? 1 : 0, c->stream),
                   "l2 error");
         std::vector<double> h((size_t)nb * 8);
         hip_check(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
@@ -624,6 +945,7 @@ dgx_status dgx_l2_error(dgx_ctx* c, const double* exact, int32_t exact_is_primit
 dgx_status dgx_write_vtk(dgx_ctx* c, const double* u_or_null, const char* filename, double gamma, int32_t resolution) {
     return guarded([&] {
         if (!c || !filename) throw std::invalid_argument("null argument");
+        if (c->nf == 4) throw Fail(DGX_E_UNSUPPORTED, "the VTK writer covers triangles");
         if (resolution < 1) throw std::invalid_argument("resolution factor must be at least 1");
         const int K = c->K, np = c->np, r = resolution, n1 = r + 1;
         std::vector<double> u;
@@ -718,9 +1040,15 @@ dgx_status dgx_project(dgx_ctx* c, const double* Uq) {
         for (int v = 0; v < 4; ++v) U[v] /= double(npts);
         c->set_offset(U);
         hip_check(hipMemcpyAsync(c->stage.p, Uq, bytes, hipMemcpyHostToDevice, c->stream), "copy");
-        hip_check(launch_project(c->precision, c->p, c->rule, c->stage.as<double>(), c->u.p, nullptr, c->K, c->Kp,
-                                 c->pm.as<double>(), c->off, c->stream),
-                  "project");
+        if (c->nf == 4)
+            hip_check(launch_project_quad(c->precision, c->np, c->nq, c->stage.as<double>(), c->u.p,
+                                          c->tab64.as<double>(), c->geo.as<double>(), c->minve.as<double>(), c->K,
+                                          c->Kp, c->off, c->stream),
+                      "project");
+        else
+            hip_check(launch_project(c->precision, c->p, c->rule, c->stage.as<double>(), c->u.p, nullptr, c->K, c->Kp,
+                                     c->pm.as<double>(), c->off, c->stream),
+                      "project");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
 }
@@ -728,10 +1056,9 @@ dgx_status dgx_project(dgx_ctx* c, const double* Uq) {
 dgx_status dgx_eval_volume(dgx_ctx* c, double* Uq) {
     return guarded([&] {
         if (!c || !Uq) throw std::invalid_argument("null argument");
-        TabLayout L(c->p, c->rule);
         size_t bytes = (size_t)c->K * c->nq * 4 * sizeof(double);
-        hip_check(launch_eval(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>() + L.phiV, c->K, c->Kp,
-                              c->stage.as<double>(), c->uref, c->stream),
+        hip_check(launch_eval_n(c->precision, c->np, c->nq, c->u.p, c->tab64.as<double>(), c->K, c->Kp,
+                                c->stage.as<double>(), c->uref, c->stream),
                   "eval");
         hip_check(hipMemcpyAsync(Uq, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
@@ -760,9 +1087,12 @@ dgx_status dgx_apply_minv(dgx_ctx* c, double* r) {
         if (!c || !r) throw std::invalid_argument("null argument");
         size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
         hip_check(hipMemcpyAsync(c->stage.p, r, bytes, hipMemcpyHostToDevice, c->stream), "copy");
-        hip_check(launch_apply_minv(c->precision, c->np, c->minv.as<double>(), c->geo.as<double>(),
-                                    c->stage.as<double>(), c->K, c->Kp, c->stream),
-                  "minv");
+        if (c->nf == 4)
+            hip_check(launch_minv_quad(c->np, c->minve.as<double>(), c->stage.as<double>(), c->K, c->Kp, c->stream), "minv");
+        else
+            hip_check(launch_apply_minv(c->precision, c->np, c->minv.as<double>(), c->geo.as<double>(),
+                                        c->stage.as<double>(), c->K, c->Kp, c->stream),
+                      "minv");
         hip_check(hipMemcpyAsync(r, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
@@ -848,6 +1178,7 @@ namespace {
 
 void adjoint_supported(const dgx_ctx* c) {
     if (!c) throw std::invalid_argument("null context");
+    if (c->nf == 4) throw Fail(DGX_E_UNSUPPORTED, "the adjoint is built for triangles");
     if (c->physics != DGX_EULER) throw Fail(DGX_E_UNSUPPORTED, "the adjoint is built for the Euler residual only");
     if (c->has_slip) throw Fail(DGX_E_UNSUPPORTED, "the adjoint does not differentiate slip-wall ghost states");
     if (c->precision == DGX_FP32_BF16VOL)
@@ -955,6 +1286,7 @@ dgx_status dgx_rhs_vjp(dgx_ctx* c, const double* u, const double* lam, double* o
 dgx_status dgx_kinetic_energy_grad(dgx_ctx* c, double* J, double* grad) {
     return guarded([&] {
         if (!c) throw std::invalid_argument("null context");
+        if (c->nf == 4) throw Fail(DGX_E_UNSUPPORTED, "the adjoint is built for triangles");
         if (J) {
             std::vector<double> h = c->reduce(1);
             double s = 0.0;

@@ -165,12 +165,12 @@ __device__ __forceinline__ void to_primitive(const double* U, double g, double* 
 __global__ __launch_bounds__(kRed) void l2err_kernel(const double* __restrict__ Uq, const double* __restrict__ Ex,
                                                      int ex_prim, const double* __restrict__ geo,
                                                      const double* __restrict__ qw, int nq, int K, int Kp,
-                                                     double gamma, double* __restrict__ partial) {
+                                                     double gamma, int per_point, double* __restrict__ partial) {
     __shared__ double sh[8][kRed];
     const int e = blockIdx.x * kRed + threadIdx.x;
     double val[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (e < K) {
-        const double det = geo[G_DET * Kp + e];
+        const double det = per_point ? 0.0 : geo[G_DET * Kp + e];
         for (int q = 0; q < nq; ++q) {
             const size_t o = ((size_t)e * nq + q) * 4;
             double W[4], X[4];
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kRed) void l2err_kernel(const double* __restrict__ 
                 for (int v = 0; v < 4; ++v) X[v] = Ex[o + v];
             else
                 to_primitive(Ex + o, gamma, X);
-            const double dw = qw[q] * det;
+            const double dw = per_point ? geo[(size_t)(5 * q + 4) * Kp + e] : qw[q] * det;
             for (int v = 0; v < 4; ++v) {
                 const double d = W[v] - X[v];
                 val[v] += d * d * dw;
@@ -230,7 +230,10 @@ hipError_t launch_project(int precision, int p, int rule, const double* Uq, void
 
 hipError_t launch_eval(int precision, int p, int rule, const void* u, const void* phiV64, int K,
                        int Kp, double* Uq, Off4 uref, hipStream_t s) {
-    int np = np_of(p), nq = nq_of(p, rule);
+    return launch_eval_n(precision, np_of(p), nq_of(p, rule), u, phiV64, K, Kp, Uq, uref, s);
+}
+hipError_t launch_eval_n(int precision, int np, int nq, const void* u, const void* phiV64, int K, int Kp, double* Uq,
+                         Off4 uref, hipStream_t s) {
     const double* ph = static_cast<const double*>(phiV64);
     if (precision == DGX_FP64)
         hipLaunchKernelGGL(eval_kernel<double>, grid_for(K, 256), dim3(256), 0, s, (const double*)u, ph, Uq, np, nq, K, Kp, uref);
@@ -240,8 +243,9 @@ hipError_t launch_eval(int precision, int p, int rule, const void* u, const void
 }
 
 hipError_t launch_l2_error(const double* Uq, const double* Ex, int ex_prim, const double* geo, const double* qw, int nq,
-                           int K, int Kp, double gamma, double* partial, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL(l2err_kernel, dim3(nblocks), dim3(kRed), 0, s, Uq, Ex, ex_prim, geo, qw, nq, K, Kp, gamma, partial);
+                           int K, int Kp, double gamma, double* partial, int nblocks, int per_point, hipStream_t s) {
+    hipLaunchKernelGGL(l2err_kernel, dim3(nblocks), dim3(kRed), 0, s, Uq, Ex, ex_prim, geo, qw, nq, K, Kp, gamma, per_point,
+                       partial);
     return hipGetLastError();
 }
 

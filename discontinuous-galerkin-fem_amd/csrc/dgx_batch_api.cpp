@@ -209,6 +209,8 @@ dgx_status dgx_batch_create(int32_t n, const dgx_mesh_desc* meshes, const dgx_sp
         for (int i = 0; i < n; ++i) {
             if (spaces[i].rule != DGX_RULE_2P) throw Fail(DGX_E_UNSUPPORTED, "batched instances use the degree-2p rule");
             dgx_ctx* c = nullptr;
+            if (meshes[i].vertices_per_element == 4)
+                throw Fail(DGX_E_UNSUPPORTED, "batched instances are triangle meshes");
             dgx_status s = dgx_create(&meshes[i], &spaces[i], physics, precision, &c);
             if (s != DGX_OK) throw Fail(s, std::string("instance ") + std::to_string(i) + ": " + g_error);
             b->inst.push_back(c);

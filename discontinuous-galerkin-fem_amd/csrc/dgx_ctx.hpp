@@ -86,6 +86,7 @@ using namespace dgx_detail;
 
 struct dgx_ctx {
     int K = 0, Kp = 0, p = 1, rule = 0, np = 3, nq = 3, nfq = 2, basis = 1, precision = 0, physics = 0;
+    int nf = 3;  // faces (= vertices) per element: 3 triangles, 4 quadrilaterals
     double gamma = 1.4, mu = 0, prandtl = 0.72, sigma0 = 5;
     uint32_t quirks = 0;
     int device = 0;
@@ -108,12 +109,13 @@ struct dgx_ctx {
     DevBuf tabML, tabMR;  // MFMA stage kernel lane tables (RK/RHS and residual variants)
     DevBuf tabVL, tabVR;  // MFMA stage adjoint lane tables (same two variants)
     DevBuf tabNT, rec;    // MFMA Navier-Stokes face-trace tables and face-point records
-    std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][3][2]
+    DevBuf minve;         // quads: per-element inverse mass matrices [Np*Np][Kp]
+    std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][nf][2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~dgx_ctx() {
         for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage,
-                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec})
+                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec, &minve})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -156,6 +158,7 @@ struct dgx_ctx {
         sp.tabt = tabNT.p;
         sp.rec = rec.p;
         sp.bc = bc.as<double>();
+        sp.minve = minve.as<double>();
         sp.K = K;
         sp.Kp = Kp;
         sp.dt = dt;
@@ -167,8 +170,14 @@ struct dgx_ctx {
         sp.sigma0 = sigma0;
         sp.mode = mode;
         sp.ns_face_quirk = (quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) ? 1 : 0;
-        for (int v = 0; v < 4; ++v) sp.uref[v] = uref.v[v];
-        hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
+        for (int v = 0; v < 4; ++v) {
+            sp.uref[v] = uref.v[v];
+            sp.uoff[v] = off.v[v];
+        }
+        if (nf == 4)
+            hip_check(launch_stage_quad(precision, physics, p, sp.ns_face_quirk != 0, sp, stream), "stage kernel launch");
+        else
+            hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
     }
 
     // One SSP-RK3 step (SSP_RK::step_rk3, I/solver/time_stepping.hpp:39-55):
@@ -182,9 +191,14 @@ struct dgx_ctx {
 
     std::vector<double> reduce(int kind) {
         int nb = red_blocks();
-        hip_check(launch_reduce(kind, precision, p, rule, u.p, tab64.p, geo.as<double>(), hmin.as<double>(),
-                                qw.as<double>(), K, Kp, gamma, uref, off, partial.as<double>(), nb, stream),
-                  "reduce kernel launch");
+        if (nf == 4)
+            hip_check(launch_reduce_quad(kind, precision, np, nq, u.p, tab64.as<double>(), geo.as<double>(),
+                                         hmin.as<double>(), K, Kp, gamma, uref, partial.as<double>(), nb, stream),
+                      "reduce kernel launch");
+        else
+            hip_check(launch_reduce(kind, precision, p, rule, u.p, tab64.p, geo.as<double>(), hmin.as<double>(),
+                                    qw.as<double>(), K, Kp, gamma, uref, off, partial.as<double>(), nb, stream),
+                      "reduce kernel launch");
         std::vector<double> h((size_t)nb * 4);
         hip_check(hipMemcpyAsync(h.data(), partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "copy");
         hip_check(hipStreamSynchronize(stream), "sync");

@@ -74,6 +74,24 @@ struct TabLayout {
     }
 };
 
+// Quadrilateral elements of order p (tensor Legendre basis, (p+1)^2 tensor GL points):
+// table offsets (state precision, plus an FP64 copy) and FP64 geometry rows [row][Kp].
+//   tables: phiV [nq][np]; GXV, GYV [nq][np] reference gradients; phiF [4][nfq][np];
+//           LF [4][nfq][np] = -w_q/2 phi; GF [4][nfq][np][2] reference gradients at face points
+//   geometry: per volume point q rows gV+5q..: dxi/dx (A00 A01 A10 A11), w_q |J_q|;
+//             per face f rows gF+3f..: outward normal (2), length (the canonical side's on both
+//             elements of an interior face, negated normal on the other side);
+//             per face point (f,q) rows gFA+4(f nfq+q)..: dxi/dx
+struct QuadDims {
+    int np, nq, nfq;
+    int phiV, GXV, GYV, phiF, LF, GF, total;
+    int gV, gF, gFA, rows;
+    __host__ __device__ constexpr explicit QuadDims(int p)
+        : np((p + 1) * (p + 1)), nq((p + 1) * (p + 1)), nfq(p + 1), phiV(0), GXV(nq * np), GYV(2 * nq * np),
+          phiF(3 * nq * np), LF(3 * nq * np + 4 * nfq * np), GF(3 * nq * np + 8 * nfq * np),
+          total(3 * nq * np + 16 * nfq * np), gV(0), gF(5 * nq), gFA(5 * nq + 12), rows(5 * nq + 12 + 16 * nfq) {}
+};
+
 // Launch parameters of one fused stage: out = ca*un + cb*(uk + dt*L(uk)) (mode RK),
 // out = L(uk) (mode RHS) or out = M dU/dt (mode RESIDUAL).
 enum StageMode : int { MODE_RK = 0, MODE_RHS = 1, MODE_RESIDUAL = 2 };
@@ -97,12 +115,14 @@ struct StageParams {
     const void* tabt;       // face-trace tables of the MFMA Navier-Stokes kernel, or null
     void* rec;              // face-point records [3 Nfq][Kp] (U, Fv.n) of the MFMA Navier-Stokes stage
     const double* bc;       // boundary-face point rows (bc_row)
+    const double* minve;    // quads: per-element inverse mass matrices [Np*Np][Kp] (FP64), else null
     int K, Kp;
     double dt, ca, cb;
     double gamma, mu, prandtl, sigma0;
     int mode;
     int ns_face_quirk;
     double uref[4];  // point-value storage offset (see Off4)
+    double uoff[4];  // the same as a constant-mode coefficient (uref / phi_0)
 };
 
 // Kernel launchers (dgx_kernels.hip / dgx_kernels_ns.hip).  Each returns hipSuccess or the
@@ -114,6 +134,8 @@ hipError_t launch_project(int precision, int p, int rule, const double* Uq, void
                           const void* tabL, int K, int Kp, const double* w_minv, Off4 off, hipStream_t s);
 hipError_t launch_eval(int precision, int p, int rule, const void* u, const void* tab, int K,
                        int Kp, double* Uq, Off4 uref, hipStream_t s);
+hipError_t launch_eval_n(int precision, int np, int nq, const void* u, const void* phiV64, int K, int Kp, double* Uq,
+                         Off4 uref, hipStream_t s);
 hipError_t launch_pack(int precision, int np, const double* host_layout_dev, void* u, int K,
                        int Kp, Off4 off, hipStream_t s);
 hipError_t launch_unpack(int precision, int np, const void* u, double* host_layout_dev, int K,
@@ -126,8 +148,17 @@ hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u
 // Relative-L2 error partial sums (dgx_aux.hip): 8 doubles per block of 256 elements.
 hipError_t launch_geometry(const double* V, const int32_t* T, const int32_t* N, int K, int Kp, double* geo,
                            double* hmin, uint32_t* code, int* err, hipStream_t s);
+// per_point: the weights are the quad geometry rows w_q |J_q| instead of qw[q] |J|.
 hipError_t launch_l2_error(const double* Uq, const double* Ex, int ex_prim, const double* geo, const double* qw, int nq,
-                           int K, int Kp, double gamma, double* partial, int nblocks, hipStream_t s);
+                           int K, int Kp, double gamma, double* partial, int nblocks, int per_point, hipStream_t s);
+// Quadrilateral elements (dgx_quad.hip).
+hipError_t launch_stage_quad(int precision, int physics, int p, bool quirk, const StageParams& sp, hipStream_t s);
+hipError_t launch_project_quad(int precision, int np, int nq, const double* Uq, void* u, const double* phiV,
+                               const double* geo, const double* minve, int K, int Kp, Off4 off, hipStream_t s);
+hipError_t launch_minv_quad(int np, const double* minve, double* r, int K, int Kp, hipStream_t s);
+hipError_t launch_reduce_quad(int kind, int precision, int np, int nq, const void* u, const double* phiV,
+                              const double* geo, const double* hmin, int K, int Kp, double gamma, Off4 uref,
+                              double* partial, int nblocks, hipStream_t s);
 hipError_t launch_apply_minv(int precision, int np, const double* minv_ref, const double* geo,
                              double* r_dev, int K, int Kp, hipStream_t s);
 // Parameters of one stage-adjoint launch (dgx_vjp.hip, dgx_vjp_mfma.hip).

@@ -290,4 +290,51 @@ void face_point(int f, double s, double& xi, double& eta) {
     else throw std::out_of_range("Invalid face for triangle");
 }
 
+// P_n and P'_n together by the three-term recurrence (P'_{k} = k P_{k-1} + x P'_{k-1}), which
+// is exact at x = +-1; the reference's endpoint rule is applied on top when asked for.
+static void legendre_pd(int n, double x, bool ref_endpoint, double* P, double* dP) {
+    P[0] = 1.0;
+    dP[0] = 0.0;
+    if (n >= 1) { P[1] = x; dP[1] = 1.0; }
+    for (int k = 2; k <= n; ++k) {
+        P[k] = ((2 * k - 1) * x * P[k - 1] - (k - 1) * P[k - 2]) / k;
+        dP[k] = k * P[k - 1] + x * dP[k - 1];
+    }
+    if (ref_endpoint && std::fabs(std::fabs(x) - 1.0) < 1e-12 && x > 0)
+        for (int k = 1; k <= n; k += 2) dP[k] = -dP[k];
+}
+
+QuadBasis::QuadBasis(int p_, bool ref) : p(p_), np((p_ + 1) * (p_ + 1)), ref_endpoint(ref) {
+    if (p < 1 || p > 4) throw std::invalid_argument("order out of range");
+}
+
+void QuadBasis::values(double xi, double eta, double* phi) const {
+    double Px[5], dPx[5], Py[5], dPy[5];
+    legendre_pd(p, xi, ref_endpoint, Px, dPx);
+    legendre_pd(p, eta, ref_endpoint, Py, dPy);
+    for (int j = 0; j <= p; ++j)
+        for (int i = 0; i <= p; ++i) phi[j * (p + 1) + i] = Px[i] * Py[j];
+}
+
+void QuadBasis::gradients(double xi, double eta, double* g) const {
+    double Px[5], dPx[5], Py[5], dPy[5];
+    legendre_pd(p, xi, ref_endpoint, Px, dPx);
+    legendre_pd(p, eta, ref_endpoint, Py, dPy);
+    for (int j = 0; j <= p; ++j)
+        for (int i = 0; i <= p; ++i) {
+            g[(j * (p + 1) + i) * 2] = dPx[i] * Py[j];
+            g[(j * (p + 1) + i) * 2 + 1] = Px[i] * dPy[j];
+        }
+}
+
+void face_point_quad(int f, double s, double& xi, double& eta) {
+    switch (f) {
+    case 0: xi = s; eta = -1.0; break;
+    case 1: xi = 1.0; eta = s; break;
+    case 2: xi = -s; eta = 1.0; break;
+    case 3: xi = -1.0; eta = -s; break;
+    default: throw std::out_of_range("Invalid face for quad");
+    }
+}
+
 }  // namespace dgx

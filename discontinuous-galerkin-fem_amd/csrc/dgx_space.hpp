@@ -32,4 +32,19 @@ struct TriBasis {
 // S/core/space.cpp:259-271): face 0 (t,0), face 1 (1-t,t), face 2 (0,1-t), t = (s+1)/2.
 void face_point(int f, double s, double& xi, double& eta);
 
+// Legendre tensor basis on [-1,1]^2 (LegendreBasis, S/basis/legendre.cpp:69-112):
+// phi_{i + (p+1) j} = P_i(xi) P_j(eta).  `ref_endpoint` reproduces the reference's derivative
+// at xi = +1 for odd degree (-P'_n(1), S/basis/legendre.cpp:31-42).
+struct QuadBasis {
+    int p, np;
+    bool ref_endpoint;
+    QuadBasis(int p, bool ref_endpoint);
+    void values(double xi, double eta, double* phi) const;
+    void gradients(double xi, double eta, double* g) const;  // g[np*2]
+};
+
+// Reference-square point of face f at GL abscissa s (map_face_quad_point, S/core/space.cpp:
+// 273-287): (s,-1), (1,s), (-s,1), (-1,-s) along the edges (v_f, v_{f+1}).
+void face_point_quad(int f, double s, double& xi, double& eta);
+
 }  // namespace dgx

@@ -103,11 +103,12 @@ __device__ __forceinline__ void prim2cons(const Real* W, Real g, Real* U) {
 
 // Gradient of U at one face point from the 8-slot face-gradient table
 // T[(c*2 + d)*2 + k][i]:  g[v*2+c] = sum_{d,k} A[d][k] sum_i T[c,d,k][i] u[i][v].
-// With the reference's face-gradient read (quirk) every slot is populated; otherwise only
-// d == c, and the zero slots are skipped at compile time.
+// With the reference's face-gradient read (quirk) every slot is populated -- the constant
+// mode's row included, so it takes the full constant-mode coefficient (stored value plus the
+// storage offset `off`); otherwise only d == c, and the zero slots are skipped at compile time.
 template <int NP, bool QUIRK, typename Real, typename TabFn>
 __device__ __forceinline__ void face_gradient(const Real (&u)[NP][4], const double* A, TabFn tab,
-                                              Real* g) {
+                                              const Real (&off)[4], Real* g) {
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         Real acc[4] = {0, 0, 0, 0};
@@ -121,7 +122,7 @@ __device__ __forceinline__ void face_gradient(const Real (&u)[NP][4], const doub
                 for (int i = 0; i < NP; ++i) {
                     Real t = tab(((c * 2 + d) * 2 + k), i);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) s[v] = fma(t, u[i][v], s[v]);
+                    for (int v = 0; v < 4; ++v) s[v] = fma(t, (QUIRK && i == 0) ? u[i][v] + off[v] : u[i][v], s[v]);
                 }
                 Real a = Real(A[d * 2 + k]);
 #pragma unroll
@@ -196,6 +197,7 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
     const bool residual_mode = sp.mode == MODE_RESIDUAL;
     // point values start from the storage offset (zero in FP64)
     const Real ur[4] = {Real(sp.uref[0]), Real(sp.uref[1]), Real(sp.uref[2]), Real(sp.uref[3])};
+    const Real uo[4] = {Real(sp.uoff[0]), Real(sp.uoff[1]), Real(sp.uoff[2]), Real(sp.uoff[3])};
 
     Real u[NP][4];
 #pragma unroll
@@ -410,14 +412,14 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
                 auto own_tab = [&](int slot, int i) {
                     return tab[O_FG + ((f * NFQ + q) * 8 + slot) * NP + i];
                 };
-                face_gradient<NP, QUIRK>(u, A, own_tab, go);
+                face_gradient<NP, QUIRK>(u, A, own_tab, uo, go);
                 Real Fvo[4], Gvo[4], Fvn[4] = {0, 0, 0, 0}, Gvn[4] = {0, 0, 0, 0};
                 viscous_flux(Uo[q], go, gam, mu, kappa, Fvo, Gvo);
                 Real Fn[4];
                 if (!boundary) {
                     const int qq = flip ? (NFQ - 1 - q) : q;
                     auto nb_tab = [&](int slot, int i) { return s_FG[((fn * NFQ + qq) * 8 + slot) * NP + i]; };
-                    face_gradient<NP, QUIRK>(un_c, An, nb_tab, gn);
+                    face_gradient<NP, QUIRK>(un_c, An, nb_tab, uo, gn);
                     viscous_flux(Un[q], gn, gam, mu, kappa, Fvn, Gvn);
                     // Fn = 1/2 Fv_L.n - 1/2 Fv_R.n - sigma (U_R - U_L), n = L normal (:192-196)
 #pragma unroll

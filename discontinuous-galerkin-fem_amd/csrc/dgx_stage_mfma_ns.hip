@@ -135,6 +135,20 @@ __device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
             gq[v * 2] = gr[v][0][r];
             gq[v * 2 + 1] = gr[v][1][r];
         }
+        if constexpr (QUIRK) {
+            // the reference's read gives the constant mode a nonzero row: add its product with
+            // the storage offset, which the stored constant-mode coefficients leave out
+            using T = StageTab<P, 0, true>;
+            const float* __restrict__ t0 = static_cast<const float*>(sp.tab) + T::FG + fp * 8 * NP;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                float G0 = 0.0f;
+#pragma unroll
+                for (int dk = 0; dk < 4; ++dk) G0 += float(gA[dk]) * t0[((c * 2 + (dk >> 1)) * 2 + (dk & 1)) * NP];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) gq[v * 2 + c] += G0 * float(sp.uoff[v]);
+            }
+        }
         float Fv[4], Gv[4];
         viscous_flux(U, gq, gam, mu, kappa, Fv, Gv);
         const float nx = float(face == 0 ? gN[0] : face == 1 ? gN[2] : gN[4]);

@@ -24,17 +24,18 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libdgx.so")
 
 DGX_OK, DGX_E_INVALID, DGX_E_UNSUPPORTED, DGX_E_DEVICE, DGX_E_NONFINITE, DGX_E_NOMEM = range(6)
-BASIS = {"monomial": 0, "dubiner": 1}
+BASIS = {"monomial": 0, "dubiner": 1, "legendre": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32_bf16vol": 2}
 PHYSICS = {"euler": 0, "ns": 1}
 BC_TYPES = {"far_field": 0, "slip_wall": 1, "no_slip_wall": 2, "periodic": 3, "none": -1}
 QUIRK_NS_FACE_GRAD_REFERENCE = 1
 QUIRK_EULER_BC_POINTS_REFERENCE = 2
+QUIRK_LEGENDRE_ENDPOINT_REFERENCE = 4
 BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 # exported symbols, in the order include/dgx.h declares them
 EXPORTS = [
-    "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_create", "dgx_destroy", "dgx_last_error",
+    "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_mesh_connect_elements", "dgx_mesh_rect_quads", "dgx_create", "dgx_destroy", "dgx_last_error",
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
     "dgx_eval_volume", "dgx_get_geometry", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk",
     "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
@@ -57,7 +58,8 @@ class MeshDesc(C.Structure):
     _fields_ = [("n_vertices", C.c_int32), ("vertices", C.POINTER(C.c_double)),
                 ("n_elements", C.c_int32), ("elements", C.POINTER(C.c_int32)),
                 ("face_neighbors", C.POINTER(C.c_int32)), ("n_bcs", C.c_int32),
-                ("bc_types", C.POINTER(C.c_int32)), ("bc_values", C.POINTER(C.c_double))]
+                ("bc_types", C.POINTER(C.c_int32)), ("bc_values", C.POINTER(C.c_double)),
+                ("vertices_per_element", C.c_int32)]
 
 
 class SpaceDesc(C.Structure):
@@ -99,6 +101,8 @@ def load():
     vp = C.c_void_p
     L.dgx_mesh_split_quads.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, dp, ip]
     L.dgx_mesh_connect.argtypes = [C.c_int32, dp, C.c_int32, ip, C.c_int32, ip]
+    L.dgx_mesh_connect_elements.argtypes = [C.c_int32, dp, C.c_int32, ip, C.c_int32, C.c_int32, ip]
+    L.dgx_mesh_rect_quads.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, dp, ip]
     L.dgx_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(SpaceDesc), C.POINTER(Physics), C.c_int32,
                              C.POINTER(vp)]
     L.dgx_destroy.argtypes = [vp]
@@ -165,11 +169,23 @@ def split_quad_mesh(n, length=10.0, x0=0.0, y0=0.0):
     return verts, tris
 
 
-def connect(verts, tris, periodic=3):
+def rect_quad_mesh(xmin, xmax, ymin, ymax, dx):
+    """Structured quads as MeshCreator::create_rectangular_mesh(dx, use_triangles=false) lays
+    them out: int((xmax - xmin)/dx) x int((ymax - ymin)/dx) cells."""
+    nx, ny = int((xmax - xmin) / dx), int((ymax - ymin) / dx)
+    verts = np.zeros(((nx + 1) * (ny + 1), 2))
+    quads = np.zeros((nx * ny, 4), dtype=np.int32)
+    _check(load().dgx_mesh_rect_quads(nx, ny, xmin, xmax, ymin, ymax, _dp(verts), _ip(quads)))
+    return verts, quads
+
+
+def connect(verts, elems, periodic=3):
+    """Face neighbours [K][nv][2] of triangles (nv = 3) or quadrilaterals (nv = 4)."""
     verts = np.ascontiguousarray(verts, dtype=np.float64)
-    tris = np.ascontiguousarray(tris, dtype=np.int32)
-    nbr = np.zeros((len(tris), 3, 2), dtype=np.int32)
-    _check(load().dgx_mesh_connect(len(verts), _dp(verts), len(tris), _ip(tris), periodic, _ip(nbr)))
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    nv = elems.shape[1]
+    nbr = np.zeros((len(elems), nv, 2), dtype=np.int32)
+    _check(load().dgx_mesh_connect_elements(len(verts), _dp(verts), len(elems), _ip(elems), nv, periodic, _ip(nbr)))
     return nbr
 
 
@@ -220,14 +236,21 @@ class Context:
     """One mesh + space + physics + precision on the GPU.  Layout of every host state array:
     [K][Np][4] FP64."""
 
-    def __init__(self, verts, tris, order, weak_form=None, basis="dubiner", precision="fp32",
-                 rule=0, periodic=3, neighbors=None, bcs=None, euler_bc_points="reference"):
-        """bcs: {tag: (type, value)} with tag in Bottom/Right/Top/Left and value a constant state
+    def __init__(self, verts, tris, order, weak_form=None, basis=None, precision="fp32",
+                 rule=0, periodic=3, neighbors=None, bcs=None, euler_bc_points="reference",
+                 legendre_endpoint="reference"):
+        """tris: [K][3] triangles or [K][4] quadrilaterals (counter-clockwise).  basis: dubiner
+        (default) or monomial on triangles, legendre on quadrilaterals.
+        bcs: {tag: (type, value)} with tag in Bottom/Right/Top/Left and value a constant state
         (FAR_FIELD conserved, NO_SLIP primitive) or a function (x, y) -> state.
         euler_bc_points: where Euler boundary faces evaluate a position-dependent condition:
-        "reference" (the reference's transposed read of the face points) or "correct"."""
+        "reference" (the reference's transposed read of the face points) or "correct".
+        legendre_endpoint: the Legendre derivative at +1 as the reference computes it
+        ("reference") or P'_n(1) = n(n+1)/2 ("correct")."""
         if euler_bc_points not in ("reference", "correct"):
             raise ValueError("euler_bc_points must be 'reference' or 'correct'")
+        if legendre_endpoint not in ("reference", "correct"):
+            raise ValueError("legendre_endpoint must be 'reference' or 'correct'")
         L = load()
         wf = weak_form or EulerWeakFormulation()
         self.weak_form = wf
@@ -250,10 +273,13 @@ class Context:
             else:
                 bc_v[k] = val
         self._bc_t, self._bc_v = bc_t, bc_v
+        nvpe = self._tris.shape[1]
         mesh = MeshDesc(len(self._verts), _dp(self._verts), len(self._tris), _ip(self._tris),
-                        _ip(self._nbr), 4, _ip(bc_t), _dp(bc_v))
+                        _ip(self._nbr), 4, _ip(bc_t), _dp(bc_v), nvpe)
+        basis = basis or ("legendre" if nvpe == 4 else "dubiner")
         space = SpaceDesc(order, BASIS[basis], rule)
-        quirks = wf.quirks | (QUIRK_EULER_BC_POINTS_REFERENCE if euler_bc_points == "reference" else 0)
+        quirks = (wf.quirks | (QUIRK_EULER_BC_POINTS_REFERENCE if euler_bc_points == "reference" else 0)
+                  | (QUIRK_LEGENDRE_ENDPOINT_REFERENCE if legendre_endpoint == "reference" else 0))
         phys = Physics(PHYSICS[wf.kind], wf.gamma, wf.mu, wf.prandtl, wf.sigma0, quirks)
         h = C.c_void_p()
         _check(L.dgx_create(C.byref(mesh), C.byref(space), C.byref(phys), PRECISION[precision], C.byref(h)))
@@ -500,6 +526,8 @@ def stable_dt(ctx, cfl=0.3):
     """dt = CFL h_min / ((2p+1) max(|u|+c)) for the current state: the reference's CFL
     measure (compute_max_cfl) scaled by the DG stability factor 1/(2p+1)."""
     lam = ctx.max_cfl(1.0)
+    if not (0.0 < lam < float("inf")):
+        raise ValueError("the state has no finite positive wave speed (set the state first)")
     return cfl / ((2 * ctx.order + 1) * lam)
 
 

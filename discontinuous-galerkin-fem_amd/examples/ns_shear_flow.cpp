@@ -1,13 +1,14 @@
 // Navier-Stokes shear-flow driver written against the reference-named C++ API (dgfem_amd.hpp),
-// the counterpart of the reference's tests/solver/shear_flow_accuracy_test.cpp:89-137 on the
-// triangle mesh: the manufactured shear W = (1, y, 0, 1) on [0,1]^2 (dx = 0.05), far-field
+// the counterpart of the reference's tests/solver/shear_flow_accuracy_test.cpp:89-137: the
+// manufactured shear W = (1, y, 0, 1) on [0,1]^2 (dx = 0.05), far-field
 // condition W(x) on every boundary tag as a function of position, mu = 5e-6, Pr = 0.72,
 // penalty 1e4, order 2, dt = 1e-5 to T = 5e-4 with frames every 50 steps, then the relative L2
 // error of each primitive field (compute_variable_error, :32-79) on the device.
 //
-// usage: ns_shear_flow [n] [fp32|fp64] [reference|correct]
-//   n: squares per side (20 = dx 0.05); the last argument selects where the Euler boundary
-//   flux evaluates the condition.
+// usage: ns_shear_flow [n] [fp32|fp64] [reference|correct] [tri|quad]
+//   n: squares per side (20 = dx 0.05); the third argument selects where the Euler boundary
+//   flux evaluates the condition; the last one the element type (the reference test's
+//   transfinite quads, or each square split into two triangles).
 // prints one line of key=value pairs: elements, steps, frames, err_0..err_3.
 #include <cstdio>
 #include <cstdlib>
@@ -30,8 +31,9 @@ int main(int argc, char** argv) {
     const int n = argc > 1 ? std::atoi(argv[1]) : 20;
     const dgx_precision prec = (argc > 2 && std::strcmp(argv[2], "fp32") == 0) ? DGX_FP32 : DGX_FP64;
     const bool reference_points = !(argc > 3 && std::strcmp(argv[3], "correct") == 0);
+    const bool quads = argc > 4 && std::strcmp(argv[4], "quad") == 0;
     try {
-        DGMesh mesh = DGMesh::split_quads(n, 1.0);
+        DGMesh mesh = quads ? DGMesh::rectangular_quads(1.0 / n, 0.0, 1.0, 0.0, 1.0) : DGMesh::split_quads(n, 1.0);
         BoundaryConditionEuler shear_bc(DGX_BC_FAR_FIELD, shear_conserved);
         for (const char* tag : {"Bottom", "Right", "Top", "Left"}) mesh.set_boundary_condition_euler(tag, shear_bc);
         NavierStokesWeakFormulation wf(1.4, 5.0e-6, 0.72, 1.0e4);

@@ -50,11 +50,24 @@ struct BoundaryConditionEuler {
 
 class DGMesh {
 public:
-    DGMesh(std::vector<double> vertices, std::vector<int32_t> elements)
-        : vertices_(std::move(vertices)), elements_(std::move(elements)) {
+    // elements: [K][nv] vertex indices, counter-clockwise; nv = 3 (triangles) or 4 (quads)
+    DGMesh(std::vector<double> vertices, std::vector<int32_t> elements, int vertices_per_element = 3)
+        : vertices_(std::move(vertices)), elements_(std::move(elements)), nv_(vertices_per_element) {
+        if (nv_ != 3 && nv_ != 4) throw std::invalid_argument("elements must be triangles or quads");
         neighbors_.assign(elements_.size() * 2, -1);
-        check(dgx_mesh_connect(n_vertices(), vertices_.data(), n_elements(), elements_.data(), 0,
-                               neighbors_.data()));
+        check(dgx_mesh_connect_elements(n_vertices(), vertices_.data(), n_elements(), elements_.data(), nv_, 0,
+                                        neighbors_.data()));
+    }
+    // MeshCreator::create_rectangular_mesh(dx, use_triangles = false, xmin, xmax, ymin, ymax)
+    // (S/utils/mesh_creation.cpp:49-74,212-243): int((xmax-xmin)/dx) x int((ymax-ymin)/dx)
+    // structured quads with the tags Bottom/Right/Top/Left.
+    static DGMesh rectangular_quads(double dx, double xmin, double xmax, double ymin, double ymax) {
+        const int nx = int((xmax - xmin) / dx), ny = int((ymax - ymin) / dx);
+        if (nx < 1 || ny < 1) throw std::invalid_argument("dx larger than the domain");
+        std::vector<double> v(2 * size_t(nx + 1) * (ny + 1));
+        std::vector<int32_t> q(4 * size_t(nx) * ny);
+        check(dgx_mesh_rect_quads(nx, ny, xmin, xmax, ymin, ymax, v.data(), q.data()));
+        return DGMesh(std::move(v), std::move(q), 4);
     }
     // n x n squares on [x0, x0+L]^2 split into 2 n^2 CCW triangles.
     static DGMesh split_quads(int n, double L, double x0 = 0.0, double y0 = 0.0) {
@@ -69,8 +82,8 @@ public:
         bool bt = (a == "Bottom" || a == "Top") && (b == "Bottom" || b == "Top");
         if (!lr && !bt) throw std::invalid_argument("Unknown boundary tag pair: " + a + "/" + b);
         periodic_ |= lr ? 1 : 2;
-        check(dgx_mesh_connect(n_vertices(), vertices_.data(), n_elements(), elements_.data(), periodic_,
-                               neighbors_.data()));
+        check(dgx_mesh_connect_elements(n_vertices(), vertices_.data(), n_elements(), elements_.data(), nv_,
+                                        periodic_, neighbors_.data()));
     }
     // DGMesh::set_boundary_condition_euler(tag, bc)
     void set_boundary_condition_euler(const std::string& tag, const BoundaryConditionEuler& bc) {
@@ -87,7 +100,8 @@ public:
         throw std::invalid_argument("Unknown boundary tag: " + tag);
     }
     int n_vertices() const { return int(vertices_.size() / 2); }
-    int n_elements() const { return int(elements_.size() / 3); }
+    int n_elements() const { return int(elements_.size() / nv_); }
+    int vertices_per_element() const { return nv_; }
     const std::vector<double>& vertices() const { return vertices_; }
     // the function-valued condition of boundary tag k (Bottom, Right, Top, Left), or empty
     const BoundaryConditionEuler::Function& bc_function(int k) const { return bc_functions_[k]; }
@@ -102,6 +116,7 @@ public:
         d.n_bcs = 4;
         d.bc_types = bc_types_.data();
         d.bc_values = bc_values_.data();
+        d.vertices_per_element = nv_;
         return d;
     }
 
@@ -109,6 +124,7 @@ private:
     std::vector<double> vertices_;
     std::vector<int32_t> elements_;
     std::vector<int32_t> neighbors_;
+    int nv_ = 3;
     int periodic_ = 0;
     std::array<int32_t, 4> bc_types_{-1, -1, -1, -1};
     std::array<double, 16> bc_values_{};
@@ -126,15 +142,22 @@ public:
     int get_n_vars() const { return 4; }
     virtual bool has_viscous_terms() const { return false; }
     virtual dgx_physics physics() const { return dgx_physics{DGX_EULER, gamma_, 0.0, 0.72, 5.0, bc_point_quirk()}; }
+    // The Legendre derivative at +1 as the reference evaluates it (default; enters the
+    // Navier-Stokes face gradients on quads) or P'_n(1) = n(n+1)/2.
+    void set_reference_legendre_endpoint(bool reference) { legendre_endpoint_reference_ = reference; }
     // Where the Euler boundary flux evaluates a function-valued condition: the reference's
     // row-major read of the face points (default, euler_weak_formulation.cpp:188-194) or the
     // face points themselves.
     void set_reference_bc_points(bool reference) { bc_points_reference_ = reference; }
 
 protected:
-    uint32_t bc_point_quirk() const { return bc_points_reference_ ? uint32_t(DGX_QUIRK_EULER_BC_POINTS_REFERENCE) : 0u; }
+    uint32_t bc_point_quirk() const {
+        return (bc_points_reference_ ? uint32_t(DGX_QUIRK_EULER_BC_POINTS_REFERENCE) : 0u) |
+               (legendre_endpoint_reference_ ? uint32_t(DGX_QUIRK_LEGENDRE_ENDPOINT_REFERENCE) : 0u);
+    }
     double gamma_;
     bool bc_points_reference_ = true;
+    bool legendre_endpoint_reference_ = true;
 };
 
 class NavierStokesWeakFormulation : public EulerWeakFormulation {
@@ -167,10 +190,12 @@ private:
 // --------------------------------------------------------------------------------------------
 class DGContext {
 public:
+    // basis: Dubiner by default on triangles; quads always use the Legendre basis
     DGContext(const DGMesh& mesh, int order, const EulerWeakFormulation& wf, dgx_precision precision = DGX_FP32,
               dgx_basis basis = DGX_BASIS_DUBINER, dgx_rule rule = DGX_RULE_2P)
         : mesh_(mesh) {
         dgx_mesh_desc m = mesh_.desc();
+        if (mesh_.vertices_per_element() == 4) basis = DGX_BASIS_LEGENDRE;
         dgx_space_desc s{order, basis, rule};
         dgx_physics ph = wf.physics();
         dgx_ctx* c = nullptr;

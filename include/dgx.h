@@ -38,9 +38,13 @@ enum {
     DGX_E_NOMEM = 5
 };
 
-enum dgx_basis { DGX_BASIS_MONOMIAL = 0, DGX_BASIS_DUBINER = 1 };
-/* Volume rule: DGX_RULE_2P = Dunavant exact to degree 2p, as the reference
- * (S/core/space.cpp:37-46); DGX_RULE_2P1 = exact to 2p+1 (variant). Face rule: GL, p+1 points. */
+/* Triangles: monomial or Dubiner; quads: the Legendre tensor basis (LegendreBasis,
+ * S/basis/legendre.cpp:69-112), the only quad basis of the reference (S/core/space.cpp:309-318). */
+enum dgx_basis { DGX_BASIS_MONOMIAL = 0, DGX_BASIS_DUBINER = 1, DGX_BASIS_LEGENDRE = 2 };
+/* Volume rule on triangles: DGX_RULE_2P = Dunavant exact to degree 2p, as the reference
+ * (S/core/space.cpp:37-46); DGX_RULE_2P1 = exact to 2p+1 (variant).  Quads: tensor
+ * Gauss-Legendre with p+1 points per direction (:47-55; DGX_RULE_2P only).  Face rule: GL, p+1
+ * points. */
 enum dgx_rule { DGX_RULE_2P = 0, DGX_RULE_2P1 = 1 };
 /* Precision of the state and the bulk arithmetic. Geometry, inverse-mass scaling and the
  * conservation sums are FP64 in every mode. DGX_FP32_BF16VOL rounds the operands of the
@@ -59,7 +63,11 @@ enum dgx_quirk {
      * S/weak_forms/euler_weak_formulation.cpp:188-194); without it, at the face points.  Only
      * position-dependent conditions (dgx_set_bc_function) see a difference; the viscous
      * boundary flux reads the points correctly either way (navier_stokes_...cpp:244-247). */
-    DGX_QUIRK_EULER_BC_POINTS_REFERENCE = 1u << 1
+    DGX_QUIRK_EULER_BC_POINTS_REFERENCE = 1u << 1,
+    /* Legendre derivative at x = +1 from the reference's endpoint formula, which returns
+     * -P'_n(1) for odd n (S/basis/legendre.cpp:31-42); it enters the Navier-Stokes face
+     * gradients on quads.  Without it, P'_n(1) = n(n+1)/2. */
+    DGX_QUIRK_LEGENDRE_ENDPOINT_REFERENCE = 1u << 2
 };
 
 /* A boundary condition as a function of position (BoundaryConditionEuler's function,
@@ -69,19 +77,21 @@ typedef void (*dgx_bc_fn)(double x, double y, double out[4], void* user);
 
 typedef struct dgx_ctx dgx_ctx;
 
-/* Mesh: straight-sided triangles.  face_neighbors[(e*3+f)*2 + {0,1}] = (neighbour element,
- * neighbour local face) for local face f = edge (v_f, v_{(f+1)%3}), or (-1, bc index) on a
- * boundary face.  Build it with dgx_mesh_connect (DGMesh::build_face_connectivity +
- * set_periodic_boundaries, S/core/mesh.cpp:312-359,482-588). */
+/* Mesh: straight-sided triangles or quadrilaterals (nv = 3 or 4 vertices per element,
+ * counter-clockwise).  face_neighbors[(e*nv+f)*2 + {0,1}] = (neighbour element, neighbour
+ * local face) for local face f = edge (v_f, v_{(f+1)%nv}), or (-1, bc index) on a boundary
+ * face.  Build it with dgx_mesh_connect / dgx_mesh_connect_elements
+ * (DGMesh::build_face_connectivity + set_periodic_boundaries, S/core/mesh.cpp:312-359,482-588). */
 typedef struct {
     int32_t n_vertices;
     const double* vertices;       /* [n_vertices][2] */
     int32_t n_elements;
-    const int32_t* elements;      /* [K][3] */
-    const int32_t* face_neighbors;/* [K][3][2] */
+    const int32_t* elements;      /* [K][nv] */
+    const int32_t* face_neighbors;/* [K][nv][2] */
     int32_t n_bcs;                /* boundary-condition table referenced by boundary faces */
     const int32_t* bc_types;      /* [n_bcs] dgx_bc_type */
     const double* bc_values;      /* [n_bcs][4]: FAR_FIELD conserved state, NO_SLIP primitive */
+    int32_t vertices_per_element; /* nv: 3 (or 0) triangles, 4 quadrilaterals */
 } dgx_mesh_desc;
 
 typedef struct {
@@ -128,6 +138,18 @@ dgx_status dgx_mesh_split_quads(int32_t n, double length, double x0, double y0, 
 dgx_status dgx_mesh_connect(int32_t n_vertices, const double* vertices, int32_t n_elements,
                             const int32_t* elements, int32_t periodic_mask,
                             int32_t* face_neighbors);
+/* The same for elements of nv = 3 or 4 vertices: face_neighbors [K][nv][2]. */
+dgx_status dgx_mesh_connect_elements(int32_t n_vertices, const double* vertices, int32_t n_elements,
+                                     const int32_t* elements, int32_t vertices_per_element,
+                                     int32_t periodic_mask, int32_t* face_neighbors);
+/* Structured quadrilaterals on [xmin, xmax] x [ymin, ymax], nx x ny cells: vertex (i, j) at
+ * (xmin + i (xmax - xmin)/nx, ymin + j (ymax - ymin)/ny), quad (i, j) = (v_ij, v_i+1j, v_i+1j+1,
+ * v_ij+1) counter-clockwise, row-major; vertices [(nx+1)(ny+1)][2], elements [nx ny][4].  The
+ * layout of MeshCreator::create_rectangular_mesh(dx, use_triangles = false), whose transfinite
+ * lines take nx = int((xmax - xmin)/dx), ny = int((ymax - ymin)/dx) divisions
+ * (S/utils/mesh_creation.cpp:49-74,212-243). */
+dgx_status dgx_mesh_rect_quads(int32_t nx, int32_t ny, double xmin, double xmax, double ymin, double ymax,
+                               double* vertices, int32_t* elements);
 
 /* ---- context ------------------------------------------------------------------------------- */
 

@@ -329,7 +329,7 @@ class Oracle:
         return o[: self.n_interior]
 
     def neighbors(self):
-        o = np.zeros((self.K, 3, 2), dtype=np.int32)
+        o = np.zeros((self.K, self.nvpe, 2), dtype=np.int32)
         lib().ref_neighbors(self._h, _ip(o))
         return o
 

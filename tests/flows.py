@@ -98,13 +98,17 @@ def volume_weights(ora):
 def l2_errors(ora, u, prim_fn, g=GAMMA):
     """compute_variable_error for the four primitive fields of the state u."""
     xy = ora.volume_points()
+    return l2_errors_at_points(ora, u, np.array([[prim_fn(*q) for q in e] for e in xy]), g)
+
+
+def l2_errors_at_points(ora, u, Wex, g=GAMMA):
+    """The same with the exact primitive values given at the oracle's volume points."""
     Uq = ora.eval_volume(u)
     w = volume_weights(ora)
     dw = np.array([w * np.abs(ora.elem_data(e)[0]) for e in range(ora.K)])[..., None]
     rho = Uq[..., 0]
     W = np.stack([rho, Uq[..., 1] / rho, Uq[..., 2] / rho,
                   (g - 1) * (Uq[..., 3] - 0.5 * (Uq[..., 1] ** 2 + Uq[..., 2] ** 2) / rho)], axis=-1)
-    Wex = np.array([[prim_fn(*q) for q in e] for e in xy])
     e2, n2 = ((W - Wex) ** 2 * dw).sum(axis=(0, 1)), (Wex ** 2 * dw).sum(axis=(0, 1))
     return np.where(n2 < 1e-14, np.sqrt(e2), np.sqrt(e2 / np.where(n2 < 1e-14, 1.0, n2)))
 

