@@ -257,8 +257,19 @@ def bench_ns(args):
     wf = dgx.NavierStokesWeakFormulation(1.4, 1.0 / 1600.0, 0.71, 5.0, face_grad="reference")
     ctx = dgx.Context(verts, tris, p, weak_form=wf, basis="dubiner", precision="fp32")
     ctx.project(W.taylor_green(ctx.volume_points()))
-    dt = dgx.stable_dt(ctx, 0.3)
+    # The reference's viscous treatment (half-difference interior average) lets this flow grow
+    # without bound from t ~ 0.015 whatever the step; CFL 0.05 keeps the 500 steps (t = 0.008)
+    # inside the bounded window.  The work per step does not depend on dt.
+    dt = dgx.stable_dt(ctx, 0.05)
     secs = timed_steps(ctx, dt, steps, args.warmup)
+    Uq32 = ctx.eval_volume()
+    extra = {}
+    if not args.no_fp64:
+        # the same run in FP64 (per-lane kernel): per-field relative L2 at the volume points
+        c64 = dgx.Context(verts, tris, p, weak_form=wf, basis="dubiner", precision="fp64")
+        c64.project(W.taylor_green(c64.volume_points()))
+        c64.rk3_steps(dt, steps)
+        extra["l2_err_fp32_vs_fp64"] = dict(zip(["rho", "rhou", "rhov", "E"], l2_rel(Uq32, c64.eval_volume())))
     ctx.set_state(ctx.get_state())
     stage_ms, _ = ctx.rk3_profile(dt, 20)
     K, Np = ctx.K, ctx.Np
@@ -274,7 +285,7 @@ def bench_ns(args):
                     "traffic": load_traffic(f"ns_taylor_green_n{n}_p{p}", kname), "kernel": kname,
                     "launch_ms": stage_ms, "alg_bytes_per_launch": bytes_launch},
           # NS p = 4 sits near the ridge: the compute side beside the bandwidth side
-          compute=compute_fraction(f"ns_taylor_green_n{n}_p{p}", kname, stage_ms))
+          compute=compute_fraction(f"ns_taylor_green_n{n}_p{p}", kname, stage_ms), **extra)
 
 
 def bench_adjoint(args):

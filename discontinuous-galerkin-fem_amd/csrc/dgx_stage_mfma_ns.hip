@@ -99,6 +99,15 @@ __device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
     // dxi_dx entry scales the operand (one column per element), so each (variable, component)
     // accumulates in a single product chain; without the reference's read only the d = c slots
     // are populated
+    // Under the reference's read the constant mode has a nonzero gradient row, so its operand
+    // is the full coefficient: the stored value plus the storage offset.
+    float bg[4][R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const bool m0 = QUIRK && mf_mode(R, NP, g, s) == 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) bg[v][s] = m0 ? b[v][s] + float(sp.uoff[v]) : b[v][s];
+    }
     v4f gr[4][2];
 #pragma unroll
     for (int v = 0; v < 4; ++v) gr[v][0] = gr[v][1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
@@ -110,7 +119,7 @@ __device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int s = 0; s < R; ++s) bs[v][s] = wA * b[v][s];
+            for (int s = 0; s < R; ++s) bs[v][s] = wA * bg[v][s];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             if (!QUIRK && d != c) continue;
@@ -135,20 +144,7 @@ __device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
             gq[v * 2] = gr[v][0][r];
             gq[v * 2 + 1] = gr[v][1][r];
         }
-        if constexpr (QUIRK) {
-            // the reference's read gives the constant mode a nonzero row: add its product with
-            // the storage offset, which the stored constant-mode coefficients leave out
-            using T = StageTab<P, 0, true>;
-            const float* __restrict__ t0 = static_cast<const float*>(sp.tab) + T::FG + fp * 8 * NP;
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                float G0 = 0.0f;
-#pragma unroll
-                for (int dk = 0; dk < 4; ++dk) G0 += float(gA[dk]) * t0[((c * 2 + (dk >> 1)) * 2 + (dk & 1)) * NP];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) gq[v * 2 + c] += G0 * float(sp.uoff[v]);
-            }
-        }
+
         float Fv[4], Gv[4];
         viscous_flux(U, gq, gam, mu, kappa, Fv, Gv);
         const float nx = float(face == 0 ? gN[0] : face == 1 ? gN[2] : gN[4]);
