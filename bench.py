@@ -257,9 +257,9 @@ def bench_ns(args):
     wf = dgx.NavierStokesWeakFormulation(1.4, 1.0 / 1600.0, 0.71, 5.0, face_grad="reference")
     ctx = dgx.Context(verts, tris, p, weak_form=wf, basis="dubiner", precision="fp32")
     ctx.project(W.taylor_green(ctx.volume_points()))
-    # The reference's viscous treatment (half-difference interior average) lets this flow grow
-    # without bound from t ~ 0.015 whatever the step; CFL 0.05 keeps the 500 steps (t = 0.008)
-    # inside the bounded window.  The work per step does not depend on dt.
+    # The reference's viscous treatment (its heat flux runs up the temperature gradient) lets
+    # this flow grow without bound from t ~ 0.015 whatever the step; CFL 0.05 keeps the 500
+    # steps (t = 0.008) inside the bounded window.  The work per step does not depend on dt.
     dt = dgx.stable_dt(ctx, 0.05)
     secs = timed_steps(ctx, dt, steps, args.warmup)
     Uq32 = ctx.eval_volume()

@@ -170,6 +170,8 @@ struct dgx_ctx {
         sp.sigma0 = sigma0;
         sp.mode = mode;
         sp.ns_face_quirk = (quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) ? 1 : 0;
+        sp.ns_avg_ref = (quirks & DGX_QUIRK_NS_VISCOUS_AVERAGE_REFERENCE) ? 1 : 0;
+        sp.kappa = ((quirks & DGX_QUIRK_NS_HEAT_FLUX_REFERENCE) ? 1.0 : -1.0) * mu * gamma / (prandtl * (gamma - 1.0));
         for (int v = 0; v < 4; ++v) {
             sp.uref[v] = uref.v[v];
             sp.uoff[v] = off.v[v];

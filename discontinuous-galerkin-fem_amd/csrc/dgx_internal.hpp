@@ -121,6 +121,9 @@ struct StageParams {
     double gamma, mu, prandtl, sigma0;
     int mode;
     int ns_face_quirk;
+    int ns_avg_ref;  // viscous face flux: 1 the reference's (half difference, inviscid sign), 0 consistent
+    double kappa;    // heat coefficient of q = -kappa grad T in the energy flux u.tau + q: mu gamma /
+                     // (Pr (gamma - 1)) as the reference, its negative for conduction
     double uref[4];  // point-value storage offset (see Off4)
     double uoff[4];  // the same as a constant-mode coefficient (uref / phi_0)
 };

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
     const int Kp = sp.Kp;
     const Real gam = Real(sp.gamma);
     const Real mu = Real(sp.mu);
-    const Real kappa = Real(sp.mu * sp.gamma / (sp.prandtl * (sp.gamma - 1.0)));
+    const Real kappa = Real(sp.kappa);
     const Real ur[4] = {Real(sp.uref[0]), Real(sp.uref[1]), Real(sp.uref[2]), Real(sp.uref[3])};
     const Real uo[4] = {Real(sp.uoff[0]), Real(sp.uoff[1]), Real(sp.uoff[2]), Real(sp.uoff[3])};
 
@@ -234,6 +234,9 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
         }
         if constexpr (NS) {
             const Real sigma = Real(sp.sigma0 * sp.mu * (P + 1) * (P + 1) / fmax(len, 1e-12));
+            // (1/2, -1/2): the reference's half difference; (-1/2, -1/2): the consistent -{Fv}.n
+            const Real wL = sp.ns_avg_ref ? Real(0.5) : Real(-0.5), wR = Real(-0.5);
+            const Real wB = sp.ns_avg_ref ? Real(1) : Real(-1);
 #pragma unroll
             for (int q = 0; q < NFQ; ++q) {
                 Real go[8], gn[8], Fvo[4], Gvo[4], Fn[4];
@@ -250,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
                         const Real fnb = Fvn[v] * cnx + Gvn[v] * cny;
                         const Real fL = isL ? fo : fnb, fR = isL ? fnb : fo;
                         const Real UL = isL ? Uo[q][v] : Un[q][v], UR = isL ? Un[q][v] : Uo[q][v];
-                        Fn[v] = Real(0.5) * fL - Real(0.5) * fR - sigma * (UR - UL);
+                        Fn[v] = wL * fL + wR * fR - sigma * (UR - UL);
                     }
                 } else {
                     const double* bcr = bc_row(sp.bc, nb, NFQ, q);
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
                         for (int v = 0; v < 4; ++v) Ub[v] = Real(bcr[5 + v]);
                     }
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) Fn[v] = (Fvo[v] * nx + Gvo[v] * ny) - sigma * (Ub[v] - Uo[q][v]);
+                    for (int v = 0; v < 4; ++v) Fn[v] = wB * (Fvo[v] * nx + Gvo[v] * ny) - sigma * (Ub[v] - Uo[q][v]);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Hs[q][v] += sgn * sfe * Fn[v];

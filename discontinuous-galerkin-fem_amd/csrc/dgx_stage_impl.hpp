@@ -193,7 +193,7 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
 
     const Real gam = Real(sp.gamma);
     const Real mu = Real(sp.mu);
-    const Real kappa = Real(sp.mu * sp.gamma / (sp.prandtl * (sp.gamma - 1.0)));
+    const Real kappa = Real(sp.kappa);
     const bool residual_mode = sp.mode == MODE_RESIDUAL;
     // point values start from the storage offset (zero in FP64)
     const Real ur[4] = {Real(sp.uref[0]), Real(sp.uref[1]), Real(sp.uref[2]), Real(sp.uref[3])};
@@ -406,6 +406,10 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
         if constexpr (NS) {
             // viscous face flux; penalty sigma0 mu (p+1)^2 / max(h, 1e-12) (:275-278)
             const Real sigma = Real(sp.sigma0 * sp.mu * (P + 1) * (P + 1) / fmax(len, 1e-12));
+            // weights of the two sides' fluxes along the L normal: (1/2, -1/2) the reference's
+            // half difference, (-1/2, -1/2) the consistent -{Fv}.n; boundary sign likewise
+            const Real wL = sp.ns_avg_ref ? Real(0.5) : Real(-0.5), wR = Real(-0.5);
+            const Real wB = sp.ns_avg_ref ? Real(1) : Real(-1);
 #pragma unroll
             for (int q = 0; q < NFQ; ++q) {
                 Real go[8], gn[8];
@@ -421,14 +425,14 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
                     auto nb_tab = [&](int slot, int i) { return s_FG[((fn * NFQ + qq) * 8 + slot) * NP + i]; };
                     face_gradient<NP, QUIRK>(un_c, An, nb_tab, uo, gn);
                     viscous_flux(Un[q], gn, gam, mu, kappa, Fvn, Gvn);
-                    // Fn = 1/2 Fv_L.n - 1/2 Fv_R.n - sigma (U_R - U_L), n = L normal (:192-196)
+                    // Fn = 1/2 Fv_L.n -+ 1/2 Fv_R.n - sigma (U_R - U_L), n = L normal (:192-196)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         Real fo = Fvo[v] * cnx + Gvo[v] * cny;
                         Real fnb = Fvn[v] * cnx + Gvn[v] * cny;
                         Real fL = isL ? fo : fnb, fR = isL ? fnb : fo;
                         Real UL = isL ? Uo[q][v] : Un[q][v], UR = isL ? Un[q][v] : Uo[q][v];
-                        Fn[v] = Real(0.5) * fL - Real(0.5) * fR - sigma * (UR - UL);
+                        Fn[v] = wL * fL + wR * fR - sigma * (UR - UL);
                     }
                 } else {
                     // boundary: Fn = Fv_L.n - sigma (U_bc - U_L), U_bc from the BC data taken as
@@ -445,7 +449,7 @@ __device__ __forceinline__ void stage_element(const StageParams& sp, const int e
                     }
 #pragma unroll
                     for (int v = 0; v < 4; ++v)
-                        Fn[v] = (Fvo[v] * nx + Gvo[v] * ny) - sigma * (Ub[v] - Uo[q][v]);
+                        Fn[v] = wB * (Fvo[v] * nx + Gvo[v] * ny) - sigma * (Ub[v] - Uo[q][v]);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Hs[q][v] += sgn * sfe * Fn[v];

@@ -46,7 +46,7 @@ __device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
     const int ec = valid ? e : sp.K - 1;
     const unsigned uKp = unsigned(sp.Kp);
     const float gam = float(sp.gamma), mu = float(sp.mu);
-    const float kappa = float(sp.mu * sp.gamma / (sp.prandtl * (sp.gamma - 1.0)));
+    const float kappa = float(sp.kappa);
     const float ur[4] = {float(sp.uref[0]), float(sp.uref[1]), float(sp.uref[2]), float(sp.uref[3])};
     const float* __restrict__ tt = static_cast<const float*>(sp.tabt);
 
@@ -176,7 +176,7 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
     const unsigned uKp = unsigned(sp.Kp);
     const bool residual_mode = sp.mode == MODE_RESIDUAL;
     const float gam = float(sp.gamma), mu = float(sp.mu);
-    const float kappa = float(sp.mu * sp.gamma / (sp.prandtl * (sp.gamma - 1.0)));
+    const float kappa = float(sp.kappa);
     const float ur[4] = {float(sp.uref[0]), float(sp.uref[1]), float(sp.uref[2]), float(sp.uref[3])};
     const float* __restrict__ tm = static_cast<const float*>(sp.tabm);
     const auto r_rec = make_rsrc(sp.rec, unsigned(3 * NFQ) * 32u * uKp);
@@ -355,9 +355,11 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
         const float Un[4] = {rn[f].u.x, rn[f].u.y, rn[f].u.z, rn[f].u.w};
         const float vn[4] = {rn[f].v.x, rn[f].v.y, rn[f].v.z, rn[f].v.w};
         // canonical orientation: A side = the L element, normal = the L-side outward normal;
-        // Fn = 1/2 Fv_L.n - 1/2 Fv_R.n - sigma (U_R - U_L) (:192-196) with the own Fv.n_L =
-        // sgn vo and the neighbour's = -sgn vn, so both sides form identical numbers
+        // Fn = 1/2 Fv_L.n -+ 1/2 Fv_R.n - sigma (U_R - U_L) (:192-196; the reference's half
+        // difference, or the average) with the own Fv.n_L = sgn vo and the neighbour's =
+        // -sgn vn, so both sides form identical numbers
         const float sgn = isL ? 1.0f : -1.0f;
+        const float wL = sp.ns_avg_ref ? 0.5f : -0.5f, wR = -0.5f;
         float Aq[4], Bq[4], H[4], Fn[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -365,7 +367,7 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             Bq[v] = isL ? Un[v] : Uo[v];
             const float fo = sgn * vo[v], fnb = -sgn * vn[v];
             const float fL = isL ? fo : fnb, fR = isL ? fnb : fo;
-            Fn[v] = 0.5f * fL - 0.5f * fR - sigma * (Bq[v] - Aq[v]);
+            Fn[v] = wL * fL + wR * fR - sigma * (Bq[v] - Aq[v]);
         }
         rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
         const float hs = sgn * sf;
@@ -405,8 +407,9 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
                 for (int v = 0; v < 4; ++v) Ub[v] = float(bcr[5 + v]);
             }
             rusanov(Uo, Un, nx, ny, gam, H);
+            const float wB = sp.ns_avg_ref ? 1.0f : -1.0f;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Hs[v][f] = sf * (H[v] + (vo[v] - sigma * (Ub[v] - Uo[v])));
+            for (int v = 0; v < 4; ++v) Hs[v][f] = sf * (H[v] + (wB * vo[v] - sigma * (Ub[v] - Uo[v])));
         }
     }
 #pragma unroll

@@ -31,6 +31,8 @@ BC_TYPES = {"far_field": 0, "slip_wall": 1, "no_slip_wall": 2, "periodic": 3, "n
 QUIRK_NS_FACE_GRAD_REFERENCE = 1
 QUIRK_EULER_BC_POINTS_REFERENCE = 2
 QUIRK_LEGENDRE_ENDPOINT_REFERENCE = 4
+QUIRK_NS_VISCOUS_AVERAGE_REFERENCE = 8
+QUIRK_NS_HEAT_FLUX_REFERENCE = 16
 BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 # exported symbols, in the order include/dgx.h declares them
@@ -211,7 +213,11 @@ class NavierStokesWeakFormulation(EulerWeakFormulation):
     kind = "ns"
 
     def __init__(self, gamma=1.4, dynamic_viscosity=1e-3, prandtl=0.72, penalty_prefactor=5.0,
-                 face_grad="reference"):
+                 face_grad="reference", viscous_average="reference", heat_flux="reference"):
+        """face_grad: the reference's read of the face gradients or the "correct" one;
+        viscous_average: the viscous face fluxes as the reference forms them (interior half
+        difference, subtracted like the inviscid flux) or the "consistent" +{Fv}.n; heat_flux: the energy flux u.tau + q as the reference forms it
+        (heat up the temperature gradient) or "conduction" (u.tau - q)."""
         super().__init__(gamma)
         if dynamic_viscosity <= 0:
             raise ValueError("Dynamic viscosity must be positive for Navier-Stokes weak formulation")
@@ -220,7 +226,13 @@ class NavierStokesWeakFormulation(EulerWeakFormulation):
         if penalty_prefactor <= 0:
             raise ValueError("Penalty prefactor must be positive for Navier-Stokes weak formulation")
         self.mu, self.prandtl, self.sigma0 = dynamic_viscosity, prandtl, penalty_prefactor
-        self.quirks = QUIRK_NS_FACE_GRAD_REFERENCE if face_grad == "reference" else 0
+        if (face_grad not in ("reference", "correct") or viscous_average not in ("reference", "consistent")
+                or heat_flux not in ("reference", "conduction")):
+            raise ValueError("face_grad: reference|correct, viscous_average: reference|consistent, "
+                             "heat_flux: reference|conduction")
+        self.quirks = ((QUIRK_NS_FACE_GRAD_REFERENCE if face_grad == "reference" else 0)
+                       | (QUIRK_NS_VISCOUS_AVERAGE_REFERENCE if viscous_average == "reference" else 0)
+                       | (QUIRK_NS_HEAT_FLUX_REFERENCE if heat_flux == "reference" else 0))
 
     def compute_penalty_parameter(self, p, h):
         return self.sigma0 * self.mu * (p + 1) * (p + 1) / max(h, 1e-12)

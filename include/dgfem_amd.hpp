@@ -164,9 +164,10 @@ class NavierStokesWeakFormulation : public EulerWeakFormulation {
 public:
     NavierStokesWeakFormulation(double gamma = 1.4, double dynamic_viscosity = 1.0e-3,
                                 double prandtl = 0.72, double penalty_prefactor = 5.0,
-                                bool reference_face_gradients = true)
+                                bool reference_face_gradients = true, bool reference_viscous_average = true,
+                                bool reference_heat_flux = true)
         : EulerWeakFormulation(gamma), mu_(dynamic_viscosity), pr_(prandtl), sigma0_(penalty_prefactor),
-          quirk_(reference_face_gradients) {
+          quirk_(reference_face_gradients), avg_quirk_(reference_viscous_average), heat_quirk_(reference_heat_flux) {
         if (mu_ <= 0.0) throw std::invalid_argument("Dynamic viscosity must be positive for Navier-Stokes weak formulation");
         if (pr_ <= 0.0) throw std::invalid_argument("Prandtl number must be positive for Navier-Stokes weak formulation");
         if (sigma0_ <= 0.0) throw std::invalid_argument("Penalty prefactor must be positive for Navier-Stokes weak formulation");
@@ -177,12 +178,14 @@ public:
     }
     dgx_physics physics() const override {
         return dgx_physics{DGX_NAVIER_STOKES, gamma_, mu_, pr_, sigma0_,
-                           (quirk_ ? uint32_t(DGX_QUIRK_NS_FACE_GRAD_REFERENCE) : 0u) | bc_point_quirk()};
+                           (quirk_ ? uint32_t(DGX_QUIRK_NS_FACE_GRAD_REFERENCE) : 0u) |
+                               (avg_quirk_ ? uint32_t(DGX_QUIRK_NS_VISCOUS_AVERAGE_REFERENCE) : 0u) |
+                               (heat_quirk_ ? uint32_t(DGX_QUIRK_NS_HEAT_FLUX_REFERENCE) : 0u) | bc_point_quirk()};
     }
 
 private:
     double mu_, pr_, sigma0_;
-    bool quirk_;
+    bool quirk_, avg_quirk_, heat_quirk_;
 };
 
 // --------------------------------------------------------------------------------------------

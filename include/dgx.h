@@ -67,7 +67,19 @@ enum dgx_quirk {
     /* Legendre derivative at x = +1 from the reference's endpoint formula, which returns
      * -P'_n(1) for odd n (S/basis/legendre.cpp:31-42); it enters the Navier-Stokes face
      * gradients on quads.  Without it, P'_n(1) = n(n+1)/2. */
-    DGX_QUIRK_LEGENDRE_ENDPOINT_REFERENCE = 1u << 2
+    DGX_QUIRK_LEGENDRE_ENDPOINT_REFERENCE = 1u << 2,
+    /* NS viscous face fluxes as the reference forms them: on interior faces each side's flux
+     * along its own outward normal, 1/2 Fv_L.n_L + 1/2 Fv_R.n_R = the half difference
+     * 1/2 (Fv_L - Fv_R).n_L, and on every face subtracted like the inviscid flux
+     * (S/weak_forms/navier_stokes_weak_formulation.cpp:192-203,262-271).  Without it, the
+     * consistent flux +{Fv}.n: M dU/dt gains 1/2 (Fv_L + Fv_R).n (interior) and Fv.n
+     * (boundary); the penalty is the same either way. */
+    DGX_QUIRK_NS_VISCOUS_AVERAGE_REFERENCE = 1u << 3,
+    /* NS energy viscous flux u.tau + q with q = -kappa grad T as the reference forms it
+     * (S/weak_forms/navier_stokes_weak_formulation.cpp:76-77,91,95): heat flows up the
+     * temperature gradient, so the temperature anti-diffuses at a rate ~ kappa/h^2.  Without
+     * it, u.tau - q (conduction). */
+    DGX_QUIRK_NS_HEAT_FLUX_REFERENCE = 1u << 4
 };
 
 /* A boundary condition as a function of position (BoundaryConditionEuler's function,

@@ -382,7 +382,9 @@ V4 rusanov(const V4& UL, const V4& UR, const double* n, double g) {
     return H;
 }
 
-struct NSParams { double gamma, mu, pr, sigma0, R; };
+// heat_sign: +1 the reference's energy flux u.tau + q with q = -kappa grad T
+// (navier_stokes_weak_formulation.cpp:76-77,91,95), -1 the conducting u.tau - q.
+struct NSParams { double gamma, mu, pr, sigma0, R, heat_sign = 1.0; };
 // compute_primitive_gradients + compute_viscous_fluxes
 // (S/weak_forms/navier_stokes_weak_formulation.cpp:36-98); gradU[v*2+d].
 void viscous_flux(const V4& U, const double* gU, const NSParams& P, V4& Fv, V4& Gv) {
@@ -407,8 +409,8 @@ void viscous_flux(const V4& U, const double* gU, const NSParams& P, V4& Fv, V4& 
     double txy = P.mu * (gu[1] + gv[0]);
     double kappa = P.mu * P.gamma / (P.pr * (P.gamma - 1.0));
     double qx = -kappa * gT[0], qy = -kappa * gT[1];
-    Fv = {0.0, txx, txy, u * txx + v * txy + qx};
-    Gv = {0.0, txy, tyy, u * txy + v * tyy + qy};
+    Fv = {0.0, txx, txy, u * txx + v * txy + P.heat_sign * qx};
+    Gv = {0.0, txy, tyy, u * txy + v * tyy + P.heat_sign * qy};
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -445,6 +447,7 @@ struct Ctx {
     int K = 0, p = 1, np = 3, nq = 0, nfq = 0, physics = 0, ns_face_grad = 0;
     int nv = 3;  // vertices = faces per element: 3 triangles, 4 quads
     int euler_bc_points_reference = 1;  // Q5: Euler boundary points read through the transposed map
+    int ns_average_reference = 1;       // Q4: viscous face fluxes as the reference signs them
     Basis basis;
     std::vector<double> vx, vy;
     std::vector<int> tri;             // K*nv
@@ -826,9 +829,13 @@ void viscous_interior_face_residual(const Ctx& c, const IFace& fc, const double*
         viscous_flux(UR, gR, c.ns, FvR, GvR);
         const double* n = FL.normal;
         V4 Fn;
+        // Q4: the reference takes each side's flux along its own outward normal (:192-196), a
+        // half difference, and subtracts it like the inviscid flux; the consistent flux enters
+        // M dU/dt as +{Fv}.n, i.e. Fn = -1/2 (Fv_L + Fv_R).n_L here
+        const double sL = c.ns_average_reference ? 1.0 : -1.0, sR = -1.0;
         for (int v = 0; v < 4; ++v) {
-            double avg = 0.5 * (FvL[v] * n[0] + GvL[v] * n[1]);
-            avg += 0.5 * (FvR[v] * (-n[0]) + GvR[v] * (-n[1]));
+            double avg = 0.5 * sL * (FvL[v] * n[0] + GvL[v] * n[1]);
+            avg += 0.5 * (FvR[v] * (sR * n[0]) + GvR[v] * (sR * n[1]));
             Fn[v] = avg - sigma * (UR[v] - UL[v]);
         }
         double w = c.gw[q] * FL.length * 0.5;
@@ -901,7 +908,8 @@ void viscous_boundary_face_residual(const Ctx& c, const BFace& bf, const BC* bc,
         }
         double w = c.gw[q] * F.length * 0.5;
         for (int v = 0; v < 4; ++v) {
-            double Fn = (FvL[v] * n[0] + GvL[v] * n[1]) - sigma * (Ubc[v] - UL[v]);
+            const double sb = c.ns_average_reference ? 1.0 : -1.0;  // Q4 (the consistent sign)
+            double Fn = sb * (FvL[v] * n[0] + GvL[v] * n[1]) - sigma * (Ubc[v] - UL[v]);
             for (int i = 0; i < np; ++i) R[i * 4 + v] -= w * ph[q * np + i] * Fn;
         }
     }
@@ -1150,6 +1158,21 @@ void ref_info(void* h, int32_t* o) {
 int ref_set_euler_bc_points(void* h, int reference) {
     if (!h) return -1;
     static_cast<Ctx*>(h)->euler_bc_points_reference = reference ? 1 : 0;
+    return 0;
+}
+
+// Q17 switch: 1 = the reference's heat-flux sign in the energy viscous flux, 0 = conduction.
+int ref_set_heat_flux(void* h, int reference) {
+    if (!h) return -1;
+    static_cast<Ctx*>(h)->ns.heat_sign = reference ? 1.0 : -1.0;
+    return 0;
+}
+
+// Q4 switch: 1 = the reference's viscous face fluxes (interior half difference, subtracted like
+// the inviscid flux), 0 = the consistent +{Fv}.n (interior average, boundary Fv.n).
+int ref_set_ns_average(void* h, int reference) {
+    if (!h) return -1;
+    static_cast<Ctx*>(h)->ns_average_reference = reference ? 1 : 0;
     return 0;
 }
 

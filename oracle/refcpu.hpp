@@ -44,6 +44,8 @@ void ref_info(void* h, int32_t* out6);
 // Boundary conditions by tag name ("Bottom","Right","Top","Left").  type: 0 FAR_FIELD,
 // 1 SLIP_WALL, 2 NO_SLIP_WALL.  fn may be null -> constant value4.
 int ref_set_euler_bc_points(void* h, int reference);
+int ref_set_ns_average(void* h, int reference);  // Q4: 1 reference viscous face fluxes, 0 consistent
+int ref_set_heat_flux(void* h, int reference);   // Q17: 1 energy flux u.tau + q (reference), 0 u.tau - q
 int ref_set_bc(void* h, const char* tag, int type, const double* value4, ref_bc_fn fn,
                void* user);
 

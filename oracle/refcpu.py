@@ -43,6 +43,8 @@ def lib():
         L.ref_info.argtypes = [C.c_void_p, ip]
         L.ref_set_bc.argtypes = [C.c_void_p, C.c_char_p, C.c_int, dp, BC_FN, C.c_void_p]
         L.ref_set_euler_bc_points.argtypes = [C.c_void_p, C.c_int]
+        L.ref_set_ns_average.argtypes = [C.c_void_p, C.c_int]
+        L.ref_set_heat_flux.argtypes = [C.c_void_p, C.c_int]
         for name in ("ref_residual", "ref_rhs", "ref_project", "ref_eval_volume"):
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
         L.ref_apply_minv.argtypes = [C.c_void_p, dp]
@@ -194,7 +196,8 @@ class Oracle:
 
     def __init__(self, verts, tris, order, basis=None, vol_rule=0, periodic=3,
                  physics="euler", gamma=1.4, mu=1e-3, prandtl=0.72, sigma0=5.0,
-                 ns_face_grad="reference", euler_bc_points="reference", legendre_endpoint="reference"):
+                 ns_face_grad="reference", euler_bc_points="reference", legendre_endpoint="reference",
+                 ns_average="reference", heat_flux="reference"):
         L = lib()
         self._verts = np.ascontiguousarray(verts, dtype=np.float64)
         self._tris = np.ascontiguousarray(tris, dtype=np.int32)
@@ -218,6 +221,12 @@ class Oracle:
         # Q5: Euler boundary faces evaluate the condition at the reference's transposed read of
         # the face points ("reference") or at the points ("correct")
         L.ref_set_euler_bc_points(self._h, {"reference": 1, "correct": 0}[euler_bc_points])
+        # Q4: viscous face fluxes as the reference forms them (interior half difference, entered
+        # with the inviscid flux's sign) or the consistent +{Fv}.n
+        L.ref_set_ns_average(self._h, {"reference": 1, "consistent": 0}[ns_average])
+        # Q17: the energy viscous flux u.tau + q (q = -kappa grad T) as the reference forms it,
+        # or u.tau - q (heat conducted down the temperature gradient)
+        L.ref_set_heat_flux(self._h, {"reference": 1, "conduction": 0}[heat_flux])
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -150,7 +150,7 @@ def test_shear_flow_volume_residual_matches_analytic():
         assert np.allclose(got[:, 1:], exp[:, 1:], rtol=0, atol=1e-10)
 
 
-def _interior_expected(o, verts, tris, u, mu, pr, read):
+def _interior_expected(o, verts, tris, u, mu, pr, read, average="reference"):
     e_l, f_l, e_r, f_r, rev = o.interior_faces()[0]
     s, wg = R.gauss_legendre(2)
     VL, AL, _, gL = elem_geom(verts, tris, e_l)
@@ -171,7 +171,9 @@ def _interior_expected(o, verts, tris, u, mu, pr, read):
         gUR = np.einsum("id,iv->vd", GR[qR], u[e_r])
         FvL, GvL = viscous_flux(UL, gUL, mu, pr)
         FvR, GvR = viscous_flux(UR, gUR, mu, pr)
-        Fn = 0.5 * (FvL * n[0] + GvL * n[1]) + 0.5 * (FvR * (-n[0]) + GvR * (-n[1])) - sigma * (UR - UL)
+        # the reference: each side along its own normal (a half difference); consistent: -{Fv}.n
+        sL = 1.0 if average == "reference" else -1.0
+        Fn = 0.5 * sL * (FvL * n[0] + GvL * n[1]) + 0.5 * (FvR * (-n[0]) + GvR * (-n[1])) - sigma * (UR - UL)
         RL -= wg[q] * L / 2 * np.outer(phL, Fn)
         RR += wg[q] * L / 2 * np.outer(phR, Fn)
     return RL, RR
@@ -304,3 +306,47 @@ def test_euler_boundary_position_dependent_far_field(points):
             exp -= wg[q] * L / 2 * np.outer(ph, H)
         assert np.allclose(o.boundary_face_residual(k, u[e]), exp, rtol=0, atol=1e-12)
     assert seen_difference == (points == "reference")
+
+
+def test_consistent_viscous_face_flux_gives_the_physical_flux():
+    # with true face gradients and the consistent flux, a continuous linear shear (u = y,
+    # rho = p = 1: Fv = (0, 0, mu, 0), Gv = (0, mu, 0, y mu)) adds the physical flux Fv.n on the
+    # interior face to M dU/dt, where the reference's half difference gives zero
+    mu, pr = 5e-2, 0.72
+    verts, tris = mesh2()
+    o = R.Oracle(verts, tris, 1, basis="monomial", periodic=0, physics="ns", gamma=GAMMA, mu=mu,
+                 prandtl=pr, sigma0=5.0, ns_face_grad="correct", ns_average="consistent")
+    u = shear_coeffs(verts, tris)
+    RL, RR = o.interior_face_residual(0, u[0], u[1], viscous=True)
+    eL, eR = _interior_expected(o, verts, tris, u, mu, pr, "correct", average="consistent")
+    assert np.allclose(RL, eL, rtol=0, atol=1e-14) and np.allclose(RR, eR, rtol=0, atol=1e-14)
+    # momentum: the closed form Fn = (0, mu n_y, mu n_x, .) (the P1 energy carries a small heat
+    # flux, so its component is left to the re-derived formula above)
+    e_l, f_l, e_r, _, _ = o.interior_faces()[0]
+    VL = verts[tris[e_l]]
+    n, L = outward_normal(VL, f_l)
+    s, wg = R.gauss_legendre(2)
+    mom = np.zeros((3, 2))
+    for q, x in enumerate(face_points(VL, f_l, s)):
+        a, b = ref_point(VL, x)
+        mom += wg[q] * L / 2 * np.outer([1, b, a], [mu * n[1], mu * n[0]])
+    assert np.allclose(RL[:, 1:3], mom, rtol=0, atol=1e-14)
+    assert np.abs(RL).max() > 1e-3
+
+
+@pytest.mark.parametrize("heat", ["reference", "conduction"])
+def test_heat_flux_sign(heat):
+    # rest state with p = 1 + a x, rho = 1: grad T = (a, 0), q = -kappa grad T.  The reference's
+    # energy flux is u.tau + q (navier_stokes_weak_formulation.cpp:76-77,91,95): heat up the
+    # gradient; conduction is u.tau - q = kappa grad T
+    mu, pr, a = 0.03, 0.72, 0.5
+    verts, tris = mesh2()
+    o = R.Oracle(verts, tris, 1, basis="monomial", periodic=0, physics="ns", gamma=GAMMA, mu=mu,
+                 prandtl=pr, sigma0=5.0, heat_flux=heat)
+    U = prim2cons([1.0, 0.0, 0.0, 1.0])
+    g = np.zeros((4, 2))
+    g[3] = [a / (GAMMA - 1), 0.0]  # dE/dx = dp/dx / (gamma - 1)
+    Fv, Gv = o.viscous_flux(U, g)
+    kappa = mu * GAMMA / (pr * (GAMMA - 1))
+    assert Fv[3] == pytest.approx(-kappa * a if heat == "reference" else kappa * a, rel=1e-14)
+    assert np.allclose(Fv[:3], 0, atol=1e-15) and np.allclose(Gv, 0, atol=1e-15)
