@@ -945,8 +945,8 @@ dgx_status dgx_l2_error(dgx_ctx* c, const double* exact, int32_t exact_is_primit
 dgx_status dgx_write_vtk(dgx_ctx* c, const double* u_or_null, const char* filename, double gamma, int32_t resolution) {
     return guarded([&] {
         if (!c || !filename) throw std::invalid_argument("null argument");
-        if (c->nf == 4) throw Fail(DGX_E_UNSUPPORTED, "the VTK writer covers triangles");
         if (resolution < 1) throw std::invalid_argument("resolution factor must be at least 1");
+        const bool quad = c->nf == 4;
         const int K = c->K, np = c->np, r = resolution, n1 = r + 1;
         std::vector<double> u;
         if (!u_or_null) {
@@ -954,11 +954,14 @@ dgx_status dgx_write_vtk(dgx_ctx* c, const double* u_or_null, const char* filena
             if (dgx_get_state(c, u.data()) != DGX_OK) throw std::runtime_error(g_error);
             u_or_null = u.data();
         }
-        std::vector<double> rp;  // reference subdivision points
+        // reference subdivision points and sub-cells (get_reference_subdivision_points /
+        // _connectivity, S/utils/vtk_writer.cpp:326-424): triangles of the unit triangle, or quads
+        // of [-1,1]^2, j outer
+        std::vector<double> rp;
         for (int j = 0; j < n1; ++j)
-            for (int i = 0; i < n1 - j; ++i) {
-                rp.push_back(double(i) / r);
-                rp.push_back(double(j) / r);
+            for (int i = 0; i < (quad ? n1 : n1 - j); ++i) {
+                rp.push_back(quad ? -1.0 + 2.0 * i / r : double(i) / r);
+                rp.push_back(quad ? -1.0 + 2.0 * j / r : double(j) / r);
             }
         const int npe = (int)rp.size() / 2;
         auto tri_index = [&](int i, int j) {
@@ -966,25 +969,41 @@ dgx_status dgx_write_vtk(dgx_ctx* c, const double* u_or_null, const char* filena
             for (int jj = 0; jj < j; ++jj) idx += n1 - jj;
             return idx + i;
         };
-        std::vector<int> sub;  // reference sub-triangles
+        std::vector<int> sub;
+        const int nvc = quad ? 4 : 3;  // vertices per sub-cell
         for (int j = 0; j < r; ++j)
-            for (int i = 0; i < r - j; ++i) {
+            for (int i = 0; i < (quad ? r : r - j); ++i) {
+                if (quad) {
+                    sub.insert(sub.end(), {j * n1 + i, j * n1 + i + 1, (j + 1) * n1 + i + 1, (j + 1) * n1 + i});
+                    continue;
+                }
                 sub.insert(sub.end(), {tri_index(i, j), tri_index(i + 1, j), tri_index(i, j + 1)});
                 if (i < r - j - 1) sub.insert(sub.end(), {tri_index(i + 1, j), tri_index(i + 1, j + 1), tri_index(i, j + 1)});
             }
-        const int nse = (int)sub.size() / 3;
-        TriBasis B(c->basis, c->p);
+        const int nse = (int)sub.size() / nvc;
         std::vector<double> phi((size_t)npe * np);
-        for (int q = 0; q < npe; ++q) B.values(rp[2 * q], rp[2 * q + 1], &phi[(size_t)q * np]);
+        if (quad) {
+            const QuadBasis B(c->p, true);
+            for (int q = 0; q < npe; ++q) B.values(rp[2 * q], rp[2 * q + 1], &phi[(size_t)q * np]);
+        } else {
+            const TriBasis B(c->basis, c->p);
+            for (int q = 0; q < npe; ++q) B.values(rp[2 * q], rp[2 * q + 1], &phi[(size_t)q * np]);
+        }
         const size_t npts = (size_t)K * npe;
         std::vector<double> xy(npts * 2), U(npts * 4, 0.0);
         for (int e = 0; e < K; ++e) {
-            const double* V = &c->h_verts[(size_t)e * 6];
+            const double* V = &c->h_verts[(size_t)e * 2 * c->nf];
             for (int q = 0; q < npe; ++q) {
                 const double xi = rp[2 * q], eta = rp[2 * q + 1], n0 = 1.0 - xi - eta;
                 const size_t k = (size_t)e * npe + q;
-                xy[2 * k] = n0 * V[0] + xi * V[2] + eta * V[4];
-                xy[2 * k + 1] = n0 * V[1] + xi * V[3] + eta * V[5];
+                if (quad) {
+                    const QuadMap qm = quad_map(V, xi, eta);
+                    xy[2 * k] = qm.x;
+                    xy[2 * k + 1] = qm.y;
+                } else {
+                    xy[2 * k] = n0 * V[0] + xi * V[2] + eta * V[4];
+                    xy[2 * k + 1] = n0 * V[1] + xi * V[3] + eta * V[5];
+                }
                 for (int i = 0; i < np; ++i)
                     for (int v = 0; v < 4; ++v) U[4 * k + v] += u_or_null[((size_t)e * np + i) * 4 + v] * phi[(size_t)q * np + i];
             }
@@ -995,14 +1014,16 @@ dgx_status dgx_write_vtk(dgx_ctx* c, const double* u_or_null, const char* filena
         f << "POINTS " << npts << " double\n" << std::scientific << std::setprecision(10);
         for (size_t k = 0; k < npts; ++k) f << xy[2 * k] << " " << xy[2 * k + 1] << " 0.0\n";
         const size_t ncells = (size_t)K * nse;
-        f << "CELLS " << ncells << " " << ncells * 4 << "\n";
+        f << "CELLS " << ncells << " " << ncells * (1 + nvc) << "\n";
         for (int e = 0; e < K; ++e)
             for (int t = 0; t < nse; ++t) {
                 const size_t o = (size_t)e * npe;
-                f << 3 << " " << o + sub[3 * t] << " " << o + sub[3 * t + 1] << " " << o + sub[3 * t + 2] << "\n";
+                f << nvc;
+                for (int k = 0; k < nvc; ++k) f << " " << o + sub[nvc * t + k];
+                f << "\n";
             }
         f << "CELL_TYPES " << ncells << "\n";
-        for (size_t t = 0; t < ncells; ++t) f << 5 << "\n";
+        for (size_t t = 0; t < ncells; ++t) f << (quad ? 9 : 5) << "\n";  // VTK_QUAD, VTK_TRIANGLE
         f << "POINT_DATA " << npts << "\n" << std::scientific << std::setprecision(10);
         auto field = [&](const char* name, auto fn) {
             f << "SCALARS " << name << " double 1\nLOOKUP_TABLE default\n";

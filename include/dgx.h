@@ -209,7 +209,8 @@ dgx_status dgx_l2_error(dgx_ctx* ctx, const double* exact, int32_t exact_is_prim
  * (S/core/mesh.cpp:366-401, S/boundary/conditions.cpp:47-60). */
 dgx_status dgx_set_bc_function(dgx_ctx* ctx, int32_t bc_index, dgx_bc_fn fn, void* user);
 /* Legacy-VTK file `filename`.vtk of an Euler state (host layout [K][Np][4], or the context's
- * current state when u is NULL): each element subdivided at `resolution_factor`, fields
+ * current state when u is NULL): each element subdivided at `resolution_factor` (triangles
+ * into VTK_TRIANGLE, quads into VTK_QUAD cells), fields
  * density, u_velocity, v_velocity, velocity_magnitude, pressure, temperature, mach, rho,
  * rho_u, rho_v, E at the subdivision points.  Replaces VTKWriter::write_euler_solution
  * (S/utils/vtk_writer.cpp:72-176; defaults gamma 1.4, resolution 3 in

@@ -36,7 +36,7 @@ def _reference_subdivision(r):
     return np.array(pts), np.array(tris)
 
 
-def _parse(path):
+def _parse(path, nvc=3, cell_type="5"):
     lines = open(path).read().split("\n")
     assert lines[:4] == ["# vtk DataFile Version 3.0", "DGFEM Euler Solution", "ASCII", "DATASET UNSTRUCTURED_GRID"]
     k = 4
@@ -47,10 +47,10 @@ def _parse(path):
     k += 1 + n
     _, nc, size = lines[k].split()
     nc = int(nc)
-    assert int(size) == 4 * nc
+    assert int(size) == (1 + nvc) * nc
     cells = np.array([[int(t) for t in lines[k + 1 + i].split()] for i in range(nc)])
     k += 1 + nc
-    assert lines[k] == f"CELL_TYPES {nc}" and all(lines[k + 1 + i] == "5" for i in range(nc))
+    assert lines[k] == f"CELL_TYPES {nc}" and all(lines[k + 1 + i] == cell_type for i in range(nc))
     k += 1 + nc
     assert lines[k] == f"POINT_DATA {n}"
     k += 1
@@ -94,6 +94,36 @@ def test_vtk_output_of_an_exact_quadratic_state(tmp_path, basis, r):
     for name in FIELDS:
         err = np.abs(data[name] - want[name]).max() / np.abs(want[name]).max()
         assert err < 1e-9, (name, err)
+
+
+@pytest.mark.parametrize("r", [1, 2, 4])
+def test_vtk_output_of_quads(tmp_path, r):
+    # rectangles (an affine map) at p = 2: the tensor basis holds the quadratic state exactly;
+    # points (-1 + 2i/r, -1 + 2j/r) j outer, sub-quads (i,j),(i+1,j),(i+1,j+1),(i,j+1), VTK_QUAD
+    verts, quads = dgx.rect_quad_mesh(0.0, 1.0, 0.0, 0.5, 0.25)
+    ctx = dgx.Context(verts, quads, 2, precision="fp64", periodic=0)
+    xy = ctx.volume_points()
+    ctx.project(_poly(xy[..., 0], xy[..., 1]))
+    ctx.write_vtk(str(tmp_path / "q"), resolution_factor=r)
+    xyz, cells, data = _parse(tmp_path / "q.vtk", nvc=4, cell_type="9")
+    n1 = r + 1
+    npe = n1 * n1
+    sub = np.array([[j * n1 + i, j * n1 + i + 1, (j + 1) * n1 + i + 1, (j + 1) * n1 + i]
+                    for j in range(r) for i in range(r)])
+    assert xyz.shape == (ctx.K * npe, 3) and cells.shape == (ctx.K * r * r, 5)
+    assert np.array_equal(cells[:, 0], np.full(len(cells), 4))
+    assert np.array_equal(cells[:, 1:], np.concatenate([sub + e * npe for e in range(ctx.K)]))
+    V = verts[quads]  # [K][4][2], rectangles: x = x0 + (xi + 1)/2 (x1 - x0)
+    s = -1.0 + 2.0 * np.arange(n1) / r
+    xi, eta = np.meshgrid(s, s, indexing="xy")
+    xi, eta = xi.ravel(), eta.ravel()
+    phys = np.stack([V[:, None, 0, 0] + 0.5 * (xi + 1)[None] * (V[:, None, 1, 0] - V[:, None, 0, 0]),
+                     V[:, None, 0, 1] + 0.5 * (eta + 1)[None] * (V[:, None, 3, 1] - V[:, None, 0, 1])],
+                    axis=-1).reshape(-1, 2)
+    assert np.abs(xyz[:, :2] - phys).max() < 1e-9
+    U = _poly(phys[:, 0], phys[:, 1])
+    assert np.abs(data["E"] - U[:, 3]).max() < 1e-9 * np.abs(U[:, 3]).max()
+    assert np.abs(data["rho"] - U[:, 0]).max() < 1e-9 * np.abs(U[:, 0]).max()
 
 
 def test_vtk_output_of_an_explicit_state_and_errors(tmp_path):
