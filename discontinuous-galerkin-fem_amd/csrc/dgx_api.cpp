@@ -121,6 +121,32 @@ void build_geometry(dgx_ctx& c, const dgx_mesh_desc& m) {
     if (err & 2) throw std::invalid_argument("Degenerate edge detected");
 }
 
+// The boundary-condition table of the mesh descriptor (one extra "no condition" row): types,
+// constant values, and no functions yet.
+void init_bc_types(dgx_ctx& c, const dgx_mesh_desc& m) {
+    c.h_bctype.assign(m.n_bcs + 1, -1);
+    c.h_bcval.assign((size_t)(m.n_bcs + 1) * 4, 0.0);
+    c.h_bcfn.assign(m.n_bcs + 1, {nullptr, nullptr});
+    for (int b = 0; b < m.n_bcs; ++b) {
+        const int ty = m.bc_types ? m.bc_types[b] : -1;
+        if (ty == DGX_BC_PERIODIC)
+            throw std::invalid_argument("PERIODIC BC encountered on a boundary face - periodic boundaries should be paired");
+        if (ty < -1 || ty > DGX_BC_NO_SLIP_WALL) throw std::invalid_argument("unknown boundary condition type");
+        c.h_bctype[b] = ty;
+        for (int v = 0; v < 4; ++v) c.h_bcval[b * 4 + v] = m.bc_values ? m.bc_values[b * 4 + v] : 0.0;
+    }
+}
+
+// The context's device, its non-blocking stream and timing events.
+void open_device(dgx_ctx& c) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Fail(DGX_E_DEVICE, "no HIP device available");
+    hip_check(hipGetDevice(&c.device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&c.ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&c.ev1), "hipEventCreate");
+}
+
 // Inverse of the n x n matrix M (row-major) by Gauss-Jordan elimination with partial pivoting.
 std::vector<double> invert_gj(std::vector<double> M, int n) {
     std::vector<double> I(n * n, 0.0);
@@ -352,25 +378,10 @@ void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc
     for (int e = K; e < Kp; ++e)
         for (int i = 0; i < np; ++i) minve[(size_t)(i * np + i) * Kp + e] = 1.0;
     c.n_bfaces = nbf;
-    c.h_bctype.assign(m.n_bcs + 1, -1);
-    c.h_bcval.assign((size_t)(m.n_bcs + 1) * 4, 0.0);
-    c.h_bcfn.assign(m.n_bcs + 1, {nullptr, nullptr});
-    for (int b = 0; b < m.n_bcs; ++b) {
-        const int ty = m.bc_types ? m.bc_types[b] : -1;
-        if (ty == DGX_BC_PERIODIC)
-            throw std::invalid_argument("PERIODIC BC encountered on a boundary face - periodic boundaries should be paired");
-        if (ty < -1 || ty > DGX_BC_NO_SLIP_WALL) throw std::invalid_argument("unknown boundary condition type");
-        c.h_bctype[b] = ty;
-        for (int v = 0; v < 4; ++v) c.h_bcval[b * 4 + v] = m.bc_values ? m.bc_values[b * 4 + v] : 0.0;
-    }
+    init_bc_types(c, m);
 
     // ---------------- device ----------------
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Fail(DGX_E_DEVICE, "no HIP device available");
-    hip_check(hipGetDevice(&c.device), "hipGetDevice");
-    hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
-    hip_check(hipEventCreate(&c.ev0), "hipEventCreate");
-    hip_check(hipEventCreate(&c.ev1), "hipEventCreate");
+    open_device(c);
     c.upload(c.geo, geo.data(), geo.size() * sizeof(double));
     c.upload(c.hmin, hmin.data(), hmin.size() * sizeof(double));
     c.upload(c.code, code.data(), code.size() * sizeof(uint32_t));
@@ -586,25 +597,10 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         }
     }
     c.n_bfaces = nbf;
-    c.h_bctype.assign(m.n_bcs + 1, -1);
-    c.h_bcval.assign((size_t)(m.n_bcs + 1) * 4, 0.0);
-    c.h_bcfn.assign(m.n_bcs + 1, {nullptr, nullptr});
-    for (int b = 0; b < m.n_bcs; ++b) {
-        int t = m.bc_types ? m.bc_types[b] : -1;
-        if (t == DGX_BC_PERIODIC)
-            throw std::invalid_argument("PERIODIC BC encountered on a boundary face - periodic boundaries should be paired");
-        if (t < -1 || t > DGX_BC_NO_SLIP_WALL) throw std::invalid_argument("unknown boundary condition type");
-        c.h_bctype[b] = t;
-        for (int v = 0; v < 4; ++v) c.h_bcval[b * 4 + v] = m.bc_values ? m.bc_values[b * 4 + v] : 0.0;
-    }
+    init_bc_types(c, m);
 
     // ---------------- device ----------------
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Fail(DGX_E_DEVICE, "no HIP device available");
-    hip_check(hipGetDevice(&c.device), "hipGetDevice");
-    hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
-    hip_check(hipEventCreate(&c.ev0), "hipEventCreate");
-    hip_check(hipEventCreate(&c.ev1), "hipEventCreate");
+    open_device(c);
     c.upload(c.nbr, nbr.data(), nbr.size() * sizeof(int32_t));
     build_geometry(c, m);
     upload_bc_table(c);
