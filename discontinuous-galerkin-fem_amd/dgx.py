@@ -37,7 +37,8 @@ BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_voi
 
 # exported symbols, in the order include/dgx.h declares them
 EXPORTS = [
-    "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_mesh_connect_elements", "dgx_mesh_rect_quads", "dgx_create", "dgx_destroy", "dgx_last_error",
+    "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_mesh_connect_elements", "dgx_mesh_rect_quads",
+    "dgx_mesh_read_gmsh", "dgx_create", "dgx_destroy", "dgx_last_error",
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
     "dgx_eval_volume", "dgx_get_geometry", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk",
     "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
@@ -105,6 +106,8 @@ def load():
     L.dgx_mesh_connect.argtypes = [C.c_int32, dp, C.c_int32, ip, C.c_int32, ip]
     L.dgx_mesh_connect_elements.argtypes = [C.c_int32, dp, C.c_int32, ip, C.c_int32, C.c_int32, ip]
     L.dgx_mesh_rect_quads.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, dp, ip]
+    i32p = C.POINTER(C.c_int32)
+    L.dgx_mesh_read_gmsh.argtypes = [C.c_char_p, i32p, i32p, i32p, i32p, dp, ip, ip, C.c_char_p]
     L.dgx_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(SpaceDesc), C.POINTER(Physics), C.c_int32,
                              C.POINTER(vp)]
     L.dgx_destroy.argtypes = [vp]
@@ -181,6 +184,23 @@ def rect_quad_mesh(xmin, xmax, ymin, ymax, dx):
     return verts, quads
 
 
+def read_gmsh(path):
+    """(vertices, elements, face neighbours, boundary tag names) of a GMSH ASCII mesh; boundary
+    faces carry the index of their physical group in the names (pass tags=names to Context)."""
+    L = load()
+    n = [C.c_int32() for _ in range(4)]
+    p = str(path).encode()
+    _check(L.dgx_mesh_read_gmsh(p, *[C.byref(x) for x in n], None, None, None, None))
+    nv, K, nvpe, nt = (x.value for x in n)
+    verts = np.zeros((nv, 2))
+    elems = np.zeros((K, nvpe), dtype=np.int32)
+    nbr = np.zeros((K, nvpe, 2), dtype=np.int32)
+    names = C.create_string_buffer(64 * max(nt, 1))
+    _check(L.dgx_mesh_read_gmsh(p, *[C.byref(x) for x in n], _dp(verts), _ip(elems), _ip(nbr), names))
+    tags = [names.raw[64 * k:64 * (k + 1)].split(b"\0")[0].decode() for k in range(nt)]
+    return verts, elems, nbr, tags
+
+
 def connect(verts, elems, periodic=3):
     """Face neighbours [K][nv][2] of triangles (nv = 3) or quadrilaterals (nv = 4)."""
     verts = np.ascontiguousarray(verts, dtype=np.float64)
@@ -250,10 +270,11 @@ class Context:
 
     def __init__(self, verts, tris, order, weak_form=None, basis=None, precision="fp32",
                  rule=0, periodic=3, neighbors=None, bcs=None, euler_bc_points="reference",
-                 legendre_endpoint="reference"):
+                 legendre_endpoint="reference", tags=None):
         """tris: [K][3] triangles or [K][4] quadrilaterals (counter-clockwise).  basis: dubiner
         (default) or monomial on triangles, legendre on quadrilaterals.
-        bcs: {tag: (type, value)} with tag in Bottom/Right/Top/Left and value a constant state
+        bcs: {tag: (type, value)} with tag in `tags` (default Bottom/Right/Top/Left, the boundary
+        indices of `connect`; read_gmsh returns a mesh's own names) and value a constant state
         (FAR_FIELD conserved, NO_SLIP primitive) or a function (x, y) -> state.
         euler_bc_points: where Euler boundary faces evaluate a position-dependent condition:
         "reference" (the reference's transposed read of the face points) or "correct".
@@ -273,9 +294,9 @@ class Context:
                      else np.ascontiguousarray(neighbors, dtype=np.int32))
         # boundary-condition table indexed by tag 0..3 = Bottom, Right, Top, Left
         bcs = bcs or {}
-        tags = ["Bottom", "Right", "Top", "Left"]
-        bc_t = np.full(4, -1, dtype=np.int32)
-        bc_v = np.zeros((4, 4))
+        tags = list(tags) if tags is not None else ["Bottom", "Right", "Top", "Left"]
+        bc_t = np.full(max(len(tags), 1), -1, dtype=np.int32)
+        bc_v = np.zeros((max(len(tags), 1), 4))
         bc_fns = {}
         for name, (typ, val) in bcs.items():
             k = tags.index(name)
@@ -287,7 +308,7 @@ class Context:
         self._bc_t, self._bc_v = bc_t, bc_v
         nvpe = self._tris.shape[1]
         mesh = MeshDesc(len(self._verts), _dp(self._verts), len(self._tris), _ip(self._tris),
-                        _ip(self._nbr), 4, _ip(bc_t), _dp(bc_v), nvpe)
+                        _ip(self._nbr), len(tags), _ip(bc_t), _dp(bc_v), nvpe)
         basis = basis or ("legendre" if nvpe == 4 else "dubiner")
         space = SpaceDesc(order, BASIS[basis], rule)
         quirks = (wf.quirks | (QUIRK_EULER_BC_POINTS_REFERENCE if euler_bc_points == "reference" else 0)

@@ -85,11 +85,26 @@ public:
         check(dgx_mesh_connect_elements(n_vertices(), vertices_.data(), n_elements(), elements_.data(), nv_,
                                         periodic_, neighbors_.data()));
     }
+    // A GMSH ASCII mesh (MSH 2.2 / 4.1) with the names of its 1-D physical groups as the
+    // boundary tags (MeshCreator::create_dg_mesh_from_gmsh, S/utils/mesh_creation.cpp:195-330).
+    static DGMesh read_gmsh(const std::string& path) {
+        int32_t nv = 0, K = 0, nvpe = 0, nt = 0;
+        check(dgx_mesh_read_gmsh(path.c_str(), &nv, &K, &nvpe, &nt, nullptr, nullptr, nullptr, nullptr));
+        std::vector<double> v(2 * size_t(nv));
+        std::vector<int32_t> e(size_t(K) * nvpe), nb(size_t(K) * nvpe * 2);
+        std::vector<char> names(64 * size_t(std::max(nt, 1)));
+        check(dgx_mesh_read_gmsh(path.c_str(), &nv, &K, &nvpe, &nt, v.data(), e.data(), nb.data(), names.data()));
+        DGMesh m(std::move(v), std::move(e), nvpe);
+        m.neighbors_ = std::move(nb);
+        m.tags_.clear();
+        for (int k = 0; k < nt; ++k) m.tags_.push_back(std::string(&names[64 * size_t(k)]));
+        m.resize_tags();
+        return m;
+    }
     // DGMesh::set_boundary_condition_euler(tag, bc)
     void set_boundary_condition_euler(const std::string& tag, const BoundaryConditionEuler& bc) {
-        static const char* names[4] = {"Bottom", "Right", "Top", "Left"};
-        for (int k = 0; k < 4; ++k)
-            if (tag == names[k]) {
+        for (size_t k = 0; k < tags_.size(); ++k)
+            if (tag == tags_[k]) {
                 if (bc.type == DGX_BC_PERIODIC)
                     throw std::runtime_error("PERIODIC BC: use set_periodic_boundaries");
                 bc_types_[k] = bc.type;
@@ -99,6 +114,7 @@ public:
             }
         throw std::invalid_argument("Unknown boundary tag: " + tag);
     }
+    const std::vector<std::string>& boundary_tags() const { return tags_; }
     int n_vertices() const { return int(vertices_.size() / 2); }
     int n_elements() const { return int(elements_.size() / nv_); }
     int vertices_per_element() const { return nv_; }
@@ -113,7 +129,7 @@ public:
         d.n_elements = n_elements();
         d.elements = elements_.data();
         d.face_neighbors = neighbors_.data();
-        d.n_bcs = 4;
+        d.n_bcs = int32_t(tags_.size());
         d.bc_types = bc_types_.data();
         d.bc_values = bc_values_.data();
         d.vertices_per_element = nv_;
@@ -126,9 +142,17 @@ private:
     std::vector<int32_t> neighbors_;
     int nv_ = 3;
     int periodic_ = 0;
-    std::array<int32_t, 4> bc_types_{-1, -1, -1, -1};
-    std::array<double, 16> bc_values_{};
-    std::array<BoundaryConditionEuler::Function, 4> bc_functions_{};
+    void resize_tags() {
+        bc_types_.assign(tags_.size(), -1);
+        bc_values_.assign(4 * tags_.size(), 0.0);
+        bc_functions_.assign(tags_.size(), BoundaryConditionEuler::Function());
+    }
+    // boundary tags: the positional Bottom/Right/Top/Left of dgx_mesh_connect_elements, or a
+    // GMSH file's physical group names
+    std::vector<std::string> tags_{"Bottom", "Right", "Top", "Left"};
+    std::vector<int32_t> bc_types_ = std::vector<int32_t>(4, -1);
+    std::vector<double> bc_values_ = std::vector<double>(16, 0.0);
+    std::vector<BoundaryConditionEuler::Function> bc_functions_ = std::vector<BoundaryConditionEuler::Function>(4);
 };
 
 // --------------------------------------------------------------------------------------------
@@ -207,7 +231,8 @@ public:
         check(dgx_info(c, &info_));
         // function-valued conditions: the library calls back while it tabulates the boundary
         // face points; the functions live on the heap so a moved context keeps their addresses
-        for (int k = 0; k < 4; ++k)
+        fns_->resize(mesh_.boundary_tags().size());
+        for (int k = 0; k < int(mesh_.boundary_tags().size()); ++k)
             if (mesh_.bc_function(k)) {
                 (*fns_)[k] = mesh_.bc_function(k);
                 check(dgx_set_bc_function(c, k, &call_bc, &(*fns_)[k]));
@@ -224,8 +249,9 @@ private:
         for (int v = 0; v < 4; ++v) out[v] = U[v];
     }
     DGMesh mesh_;
-    std::unique_ptr<std::array<BoundaryConditionEuler::Function, 4>> fns_ =
-        std::make_unique<std::array<BoundaryConditionEuler::Function, 4>>();
+    // heap storage: a moved context keeps the functions' addresses (the library holds them)
+    std::unique_ptr<std::vector<BoundaryConditionEuler::Function>> fns_ =
+        std::make_unique<std::vector<BoundaryConditionEuler::Function>>();
     std::unique_ptr<dgx_ctx, Del> ctx_;
     dgx_info_t info_{};
 };

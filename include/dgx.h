@@ -163,6 +163,18 @@ dgx_status dgx_mesh_connect_elements(int32_t n_vertices, const double* vertices,
 dgx_status dgx_mesh_rect_quads(int32_t nx, int32_t ny, double xmin, double xmax, double ymin, double ymax,
                                double* vertices, int32_t* elements);
 
+/* GMSH ASCII mesh (MSH 2.2 or 4.1) as MeshCreator::extract_mesh_data reads a GMSH model
+ * (S/utils/mesh_creation.cpp:212-330): the nodes in file order, the 2-D elements of the first
+ * 2-D type (3-node triangles or 4-node quadrilaterals), and the boundary faces named by the 1-D
+ * physical groups.  face_neighbors [K][nv][2]: interior (neighbour element, face), boundary
+ * (-1, index of the face's physical group in tag_names, or -1 without one); no periodic
+ * pairing.  Call first with null arrays for the four sizes, then with arrays of those sizes;
+ * tag_names: n_tags names of the 1-D physical groups in tag order, 64 bytes each
+ * (NUL-terminated, at most 63 characters). */
+dgx_status dgx_mesh_read_gmsh(const char* path, int32_t* n_vertices, int32_t* n_elements,
+                              int32_t* vertices_per_element, int32_t* n_tags, double* vertices,
+                              int32_t* elements, int32_t* face_neighbors, char* tag_names);
+
 /* ---- context ------------------------------------------------------------------------------- */
 
 /* Builds the space (DGSpace ctor, S/core/space.cpp:20-67), per-element geometry
