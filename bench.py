@@ -141,9 +141,9 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run only a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="euler",
-                    choices=["euler", "ns", "adjoint", "batch"],
-                    help="euler: the headline vortex case; ns: Taylor-Green Navier-Stokes; "
-                         "adjoint: gradient of the final kinetic energy; batch: 64 mixed instances")
+                    choices=["euler", "ns", "quad", "adjoint", "batch"],
+                    help="euler: the headline vortex case; ns: Taylor-Green Navier-Stokes; quad: the vortex "
+                         "on quadrilaterals; adjoint: gradient of the final kinetic energy; batch: 64 mixed instances")
     args = ap.parse_args()
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
@@ -152,7 +152,7 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     if args.workload != "euler":
-        {"ns": bench_ns, "adjoint": bench_adjoint, "batch": bench_batch}[args.workload](args)
+        {"ns": bench_ns, "quad": bench_quad, "adjoint": bench_adjoint, "batch": bench_batch}[args.workload](args)
         return
 
     n, p = args.n, args.p
@@ -286,6 +286,61 @@ def bench_ns(args):
                     "launch_ms": stage_ms, "alg_bytes_per_launch": bytes_launch},
           # NS p = 4 sits near the ridge: the compute side beside the bandwidth side
           compute=compute_fraction(f"ns_taylor_green_n{n}_p{p}", kname, stage_ms), **extra)
+
+
+def bench_quad(args):
+    """The isentropic vortex on quadrilaterals: 256 x 256 periodic structured quads on [0,10]^2
+    (K = 65536), p = 3 (Legendre tensor basis, Np = 16, 4 x 4 Gauss-Legendre volume rule), FP32,
+    seeded 1% density noise, dt at CFL 0.3, 200 SSP-RK3 steps; the FP64 run beside it and the
+    FP64 CPU oracle on a bounded sample of the same mesh."""
+    n = args.n
+    p = args.p
+    verts, quads = dgx.rect_quad_mesh(0.0, 10.0, 0.0, 10.0, 10.0 / n)
+
+    def make(precision):
+        c = dgx.Context(verts, quads, p, precision=precision)
+        xy = c.volume_points()
+        c.project(vortex_ic(xy, c.K, c.Nq, SEED))
+        return c
+
+    ctx = make("fp32")
+    K, Np = ctx.K, ctx.Np
+    dt = dgx.stable_dt(ctx, 0.3)
+    u0 = ctx.get_state()
+    secs = timed_steps(ctx, dt, args.steps, args.warmup)
+    Uq32 = ctx.eval_volume()
+    ctx.set_state(u0)
+    stage_ms, _ = ctx.rk3_profile(dt, min(args.steps, 50))
+    bytes_launch = ctx.info.bytes_per_stage_alg
+    workload = f"euler_vortex_quads_n{n}_p{p}"
+    kname = "quad_stage_kernel<%d,float,Euler>" % p
+    extra = {}
+    if not args.no_fp64:
+        c64 = make("fp64")
+        secs64 = timed_steps(c64, dt, args.steps, args.warmup)
+        extra["fp64"] = {"value": 12.0 * Np * K * args.steps / secs64, "ms_per_step": secs64 / args.steps * 1e3}
+        extra["l2_err_fp32_vs_fp64"] = dict(zip(["rho", "rhou", "rhov", "E"], l2_rel(Uq32, c64.eval_volume())))
+        c64.close()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import refcpu as R
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ora = R.Oracle(verts, quads, p)
+        ora.rk3_threads(u0, dt, 1, threads)
+        _, s = ora.rk3_threads(u0, dt, 4 * args.cpu_steps, threads)
+        extra["cpu_baseline"] = {"value": 12.0 * Np * K * 4 * args.cpu_steps / s, "unit": "DOF-stage updates/s",
+                                 "cores": threads, "kind": "port",
+                                 "sample": f"{4 * args.cpu_steps} FP64 SSP-RK3 steps of the same {K}-quad p={p} mesh "
+                                           f"on {threads} OpenMP threads ({s:.1f} s, oracle/refcpu.cpp)"}
+    ctx.close()
+    achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
+    _line("Euler RHS DOF-stage updates/s on quadrilaterals (FP32)", 12.0 * Np * K * args.steps / secs,
+          "DOF-stage updates/s", args.steps, args.warmup, secs / args.steps * 1e3, "f32",
+          {"workload": workload, "mesh": f"{n}x{n} structured quads", "elements": K, "order": p, "modes": Np,
+           "basis": "legendre", "volume_rule": f"{p + 1}x{p + 1} Gauss-Legendre", "dt": dt},
+          roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload, kname), "kernel": kname,
+                    "launch_ms": stage_ms, "alg_bytes_per_launch": bytes_launch}, **extra)
 
 
 def bench_adjoint(args):

@@ -247,6 +247,35 @@ void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc
                 t[D.GF + ((f * nfq + q) * np + k) * 2 + 1] = g[2 * k + 1];
             }
         }
+    // one-dimensional factors of the four-lane kernel: P_a(x) and P_a'(x) are the (a, 0) entries
+    // of the tensor basis and its xi-derivative (P_0 = 1)
+    {
+        std::vector<double> ph(np), gr(np * 2);
+        auto P1 = [&](double x, int a) { B.values(x, 0.0, ph.data()); return ph[a]; };
+        auto dP1 = [&](double x, int a) { B.gradients(x, 0.0, gr.data()); return gr[2 * a]; };
+        for (int a = 0; a < nfq; ++a) {
+            double m = 0.0;
+            for (int q = 0; q < nfq; ++q) {
+                t[D.T1 + a * nfq + q] = P1(fr.x[q], a);
+                t[D.D1 + a * nfq + q] = dP1(fr.x[q], a);
+                m += fr.w[q] * (t[D.T1 + a * nfq + q] * t[D.T1 + a * nfq + q]);
+            }
+            t[D.IM1 + a] = 1.0 / m;
+            t[D.X1 + a] = fr.x[a];
+            t[D.W1 + a] = fr.w[a];
+        }
+        for (int f = 0; f < 4; ++f)
+            for (int q = 0; q < nfq; ++q) {
+                double xi, eta;
+                face_point_quad(f, fr.x[q], xi, eta);
+                for (int a = 0; a < nfq; ++a) {
+                    const int k = (f * nfq + q) * nfq + a;
+                    t[D.FX + k] = P1(xi, a);
+                    t[D.FY + k] = P1(eta, a);
+                    t[D.LX + k] = -0.5 * fr.w[q] * t[D.FX + k];
+                }
+            }
+    }
     c.h_qw = wq;
 
     // ---------------- connectivity (host) ----------------
@@ -346,6 +375,8 @@ void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc
         }
         // volume points: dxi/dx and w |J|; face points: dxi/dx; mass matrix
         const double* hv = &c.h_verts[(size_t)e * 8];
+        for (int k = 0; k < 8; ++k) geo[(size_t)(D.gX + k) * Kp + e] = hv[k];
+        if (hv[0] + hv[4] == hv[2] + hv[6] && hv[1] + hv[5] == hv[3] + hv[7]) code[e] |= kQuadAffineBit;
         std::vector<double> M(np * np, 0.0);
         for (int q = 0; q < nq; ++q) {
             const QuadMap qm = quad_map(hv, qxi[q], qeta[q]);
@@ -405,6 +436,9 @@ void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc
     hip_check(hipMemset(c.u2.p, 0, c.state_bytes()), "hipMemset");
     c.stage.alloc((size_t)K * std::max(np, nq) * 4 * sizeof(double));
     c.partial.alloc((size_t)c.red_blocks() * 8 * sizeof(double));
+    // Euler, p <= 3: four lanes per element (DGX_STAGE_KERNEL=lane keeps one element per lane)
+    const char* kv = std::getenv("DGX_STAGE_KERNEL");
+    c.qtensor = c.physics == DGX_EULER && p <= 3 && !(kv && std::string(kv) == "lane");
 }
 
 void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
@@ -812,7 +846,7 @@ dgx_status dgx_info(const dgx_ctx* c, dgx_info_t* o) {
     // per stage: read u_k, (stages 2,3) u^n, write u_{k+1}: 8 state touches per 3 stages,
     // plus 128 B of FP64 geometry + connectivity per element
     o->bytes_per_stage_alg = (double)c->K * (8.0 / 3.0 * 4.0 * c->np * (double)c->rsz + 128.0);
-    o->stage_kernel = c->tabML.p ? 1 : 0;
+    o->stage_kernel = c->tabML.p ? 1 : c->qtensor ? 2 : 0;
     return DGX_OK;
 }
 

@@ -87,6 +87,7 @@ using namespace dgx_detail;
 struct dgx_ctx {
     int K = 0, Kp = 0, p = 1, rule = 0, np = 3, nq = 3, nfq = 2, basis = 1, precision = 0, physics = 0;
     int nf = 3;  // faces (= vertices) per element: 3 triangles, 4 quadrilaterals
+    bool qtensor = false;  // quads: the four-lanes-per-element Euler stage kernel
     double gamma = 1.4, mu = 0, prandtl = 0.72, sigma0 = 5;
     uint32_t quirks = 0;
     int device = 0;
@@ -159,6 +160,7 @@ struct dgx_ctx {
         sp.rec = rec.p;
         sp.bc = bc.as<double>();
         sp.minve = minve.as<double>();
+        sp.tab64 = tab64.as<double>();
         sp.K = K;
         sp.Kp = Kp;
         sp.dt = dt;
@@ -176,7 +178,9 @@ struct dgx_ctx {
             sp.uref[v] = uref.v[v];
             sp.uoff[v] = off.v[v];
         }
-        if (nf == 4)
+        if (nf == 4 && qtensor)
+            hip_check(launch_stage_quad_tensor(precision, p, sp, stream), "stage kernel launch");
+        else if (nf == 4)
             hip_check(launch_stage_quad(precision, physics, p, sp.ns_face_quirk != 0, sp, stream), "stage kernel launch");
         else
             hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");

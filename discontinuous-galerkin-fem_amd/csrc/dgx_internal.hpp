@@ -84,13 +84,28 @@ struct TabLayout {
 //             per face point (f,q) rows gFA+4(f nfq+q)..: dxi/dx
 struct QuadDims {
     int np, nq, nfq;
-    int phiV, GXV, GYV, phiF, LF, GF, total;
-    int gV, gF, gFA, rows;
+    int phiV, GXV, GYV, phiF, LF, GF, T1, D1, X1, W1, IM1, FX, FY, LX, total;
+    int gV, gF, gFA, gX, rows;
     __host__ __device__ constexpr explicit QuadDims(int p)
         : np((p + 1) * (p + 1)), nq((p + 1) * (p + 1)), nfq(p + 1), phiV(0), GXV(nq * np), GYV(2 * nq * np),
           phiF(3 * nq * np), LF(3 * nq * np + 4 * nfq * np), GF(3 * nq * np + 8 * nfq * np),
-          total(3 * nq * np + 16 * nfq * np), gV(0), gF(5 * nq), gFA(5 * nq + 12), rows(5 * nq + 12 + 16 * nfq) {}
+          T1(3 * nq * np + 16 * nfq * np), D1(T1 + nfq * nfq), X1(D1 + nfq * nfq), W1(X1 + nfq), IM1(W1 + nfq),
+          FX(IM1 + nfq), FY(FX + 4 * nfq * nfq), LX(FY + 4 * nfq * nfq), total(LX + 4 * nfq * nfq), gV(0),
+          gF(5 * nq), gFA(5 * nq + 12), gX(5 * nq + 12 + 16 * nfq), rows(5 * nq + 12 + 16 * nfq + 8) {}
 };
+// Factors of the tensor basis for the four-lane quad kernel (offsets above, R = p + 1 GL nodes
+// s_q with weights w_q; P_a the degree-a Legendre polynomial, mode (a, b) = index b R + a):
+//   T1[a R + q] = P_a(s_q), D1[a R + q] = P_a'(s_q), X1[q] = s_q, W1[q] = w_q,
+//   IM1[a] = 1 / sum_q w_q P_a(s_q)^2,
+//   FX[(f R + q) R + a] = P_a(xi) and FY[(f R + q) R + b] = P_b(eta) at face point q of face f,
+//   LX[(f R + q) R + a] = -w_q / 2 P_a(xi) there (the lift's xi factor).
+// Geometry rows gX .. gX + 7: the element's vertices x0, y0, .., x3, y3.
+
+// Quad element code: bits [5f, 5f+5) describe face f (neighbour's local face in 2 bits, reversed
+// point order, canonical (L) side, boundary); bit 20 marks a parallelogram (constant Jacobian,
+// diagonal mass matrix in the Legendre basis).
+__host__ __device__ inline uint32_t quad_code(uint32_t code, int f) { return (code >> (5 * f)) & 31u; }
+constexpr uint32_t kQuadAffineBit = 1u << 20;
 
 // Launch parameters of one fused stage: out = ca*un + cb*(uk + dt*L(uk)) (mode RK),
 // out = L(uk) (mode RHS) or out = M dU/dt (mode RESIDUAL).
@@ -116,6 +131,7 @@ struct StageParams {
     void* rec;              // face-point records [3 Nfq][Kp] (U, Fv.n) of the MFMA Navier-Stokes stage
     const double* bc;       // boundary-face point rows (bc_row)
     const double* minve;    // quads: per-element inverse mass matrices [Np*Np][Kp] (FP64), else null
+    const double* tab64;    // quads: the tables in FP64 (geometry nodes and weights, mass factors)
     int K, Kp;
     double dt, ca, cb;
     double gamma, mu, prandtl, sigma0;
@@ -156,6 +172,8 @@ hipError_t launch_l2_error(const double* Uq, const double* Ex, int ex_prim, cons
                            int K, int Kp, double gamma, double* partial, int nblocks, int per_point, hipStream_t s);
 // Quadrilateral elements (dgx_quad.hip).
 hipError_t launch_stage_quad(int precision, int physics, int p, bool quirk, const StageParams& sp, hipStream_t s);
+// Euler on quads, p <= 3, four lanes per element (dgx_quad_tensor.hip)
+hipError_t launch_stage_quad_tensor(int precision, int p, const StageParams& sp, hipStream_t s);
 hipError_t launch_project_quad(int precision, int np, int nq, const double* Uq, void* u, const double* phiV,
                                const double* geo, const double* minve, int K, int Kp, Off4 off, hipStream_t s);
 hipError_t launch_minv_quad(int np, const double* minve, double* r, int K, int Kp, hipStream_t s);

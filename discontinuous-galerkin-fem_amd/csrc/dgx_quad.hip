@@ -20,10 +20,6 @@
 namespace dgx {
 namespace {
 
-// Code of face f: bits [5f, 5f+5) = neighbour's local face (2 bits), reversed point order,
-// canonical (L) side, boundary.
-__device__ __forceinline__ uint32_t quad_code(uint32_t code, int f) { return (code >> (5 * f)) & 31u; }
-
 // Physical gradient of U at face point q of local face f of element el:
 // g[v*2+c] = sum_i u[i][v] G(q,i,c), G = the reference's read of the stored physical basis
 // gradients (entry (q',i',d) of the interleaved storage, dxi/dx at point q').

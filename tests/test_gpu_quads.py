@@ -217,3 +217,64 @@ def test_quads_outside_the_built_operators_are_rejected(tmp_path):
     tv, tt = dgx.split_quad_mesh(2, 1.0)
     with pytest.raises(dgx.DGXError):
         dgx.Context(tv, tt, 2, basis="legendre", precision="fp64")
+
+
+def _quad_mesh(kind, n, seed):
+    """rectangles: every element a parallelogram (diagonal mass); distorted: none; partial:
+    every third interior vertex moved, so parallelograms and general quads share waves."""
+    if kind == "distorted":
+        return _distorted(n, 1.0, seed)
+    verts, quads = R.rect_quad_mesh(0.0, 1.0, 0.0, 1.0, 1.0 / n)
+    if kind == "partial":
+        verts = verts.copy()
+        inner = np.nonzero((verts[:, 0] > 0) & (verts[:, 0] < 1) & (verts[:, 1] > 0) & (verts[:, 1] < 1))[0][::3]
+        verts[inner] += np.random.default_rng(seed).uniform(-0.15, 0.15, (len(inner), 2)) / n
+    return verts, quads
+
+
+def _lane_context(*args, **kw):
+    import os
+    os.environ["DGX_STAGE_KERNEL"] = "lane"  # read when the context is created
+    try:
+        return dgx.Context(*args, **kw)
+    finally:
+        os.environ.pop("DGX_STAGE_KERNEL", None)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("kind", ["rectangles", "distorted", "partial"])
+@pytest.mark.parametrize("case", ["periodic", "mixed_functions"])
+def test_four_lane_kernel_against_the_oracle_and_the_lane_kernel(p, kind, case):
+    """Euler on quads, p <= 3, runs four lanes per element (stage_kernel 2): residual and RK3
+    steps equal the oracle's (FP64 1e-11, FP32 5e-5 / 2e-5) and the one-element-per-lane
+    kernel's, on parallelograms (diagonal inverse mass), general quads (the element's FP64
+    inverse) and meshes that mix both."""
+    periodic, bcs = CASES[case]
+    verts, quads = _quad_mesh(kind, 6, p)
+    ora = R.Oracle(verts, quads, p, periodic=periodic)
+    for tag, (typ, val) in bcs.items():
+        if callable(val):
+            ora.set_bc(tag, typ, fn=val)
+        else:
+            ora.set_bc(tag, typ, val)
+    xy = ora.volume_points()
+    u0 = ora.project(R.vortex_state(xy, xc=0.45, yc=0.55, beta=1.0, length=1.0))
+    u0 = u0 + 1e-3 * np.random.default_rng(3 + p).standard_normal(u0.shape) * np.abs(u0).max(axis=(0, 1))
+    r_ref = ora.residual(u0)
+    c64 = dgx.Context(verts, quads, p, precision="fp64", periodic=periodic, bcs=bcs)
+    c64.set_state(u0)
+    dt = 0.5 * dgx.stable_dt(c64, 0.3)
+    u_ref, _ = ora.rk3(u0, dt, 4)
+    lane = _lane_context(verts, quads, p, precision="fp64", periodic=periodic, bcs=bcs)
+    assert lane.info.stage_kernel == 0
+    for prec, tr, ts in (("fp64", 1e-11, 1e-11), ("fp32", 5e-5, 2e-5)):
+        c = c64 if prec == "fp64" else dgx.Context(verts, quads, p, precision=prec, periodic=periodic, bcs=bcs)
+        assert c.info.stage_kernel == 2
+        assert _rel(c.residual(u0), r_ref) < tr
+        c.set_state(u0)
+        c.rk3_steps(dt, 4)
+        assert _rel(c.get_state(), u_ref) < ts
+    assert _rel(c64.residual(u0), lane.residual(u0)) < 1e-13
+    lane.set_state(u0)
+    lane.rk3_steps(dt, 4)
+    assert _rel(c64.get_state(), lane.get_state()) < 1e-13
