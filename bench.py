@@ -313,7 +313,8 @@ def bench_quad(args):
     stage_ms, _ = ctx.rk3_profile(dt, min(args.steps, 50))
     bytes_launch = ctx.info.bytes_per_stage_alg
     workload = f"euler_vortex_quads_n{n}_p{p}"
-    kname = "quad_stage_kernel<%d,float,Euler>" % p
+    kname = ("quad_tensor_stage_kernel<%d,float>" % p if ctx.info.stage_kernel == 2
+             else "quad_stage_kernel<%d,float,Euler>" % p)
     extra = {}
     if not args.no_fp64:
         c64 = make("fp64")

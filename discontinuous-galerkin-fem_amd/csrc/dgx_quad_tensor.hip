@@ -11,13 +11,13 @@
 //   * lane f owns face f: its own trace (the other rows by broadcast), the neighbour's trace
 //     (the neighbour's modes streamed row by row from memory), the Rusanov flux; the lift
 //     sends each face flux to the four rows by broadcast;
-//   * geometry is recomputed from the four vertices in FP64 (dxi/dx and w|J| at the points,
-//     the same formulas and operation order as the host-built tables), so per element the
-//     stage reads 64 B of vertices and 96 B of face normals and lengths instead of the
-//     per-point rows;
+//   * geometry is recomputed from the four vertices: the bilinear map's six derivative
+//     coefficients in FP64, dxi/dx and w|J| at the points in the state precision, so per
+//     element the stage reads 64 B of vertices and 96 B of face normals and lengths instead
+//     of the per-point rows;
 //   * the inverse mass: on a parallelogram (code bit kQuadAffineBit) M = |J| diag(m_a m_b), so
-//     M^-1 is applied in place; otherwise the per-element FP64 inverse, with the residual
-//     gathered from the quad by broadcast.
+//     M^-1 is a scaling of each lane's own modes; otherwise the per-element FP64 inverse, with
+//     the residual gathered from the quad by broadcast.
 // Reference behaviour as the one-element-per-lane quad kernel: volume and face terms
 // S/weak_forms/euler_weak_formulation.cpp:72-250, assembly S/solver/assembler.cpp:137-279,
 // M^-1 and SSP-RK3 S/solver/compressible_solver_base.cpp:41-112.
@@ -48,28 +48,6 @@ __device__ __forceinline__ double qb(double x, int src) {
     const uint32_t lo = uint32_t(qb_int(int(uint32_t(b)), src));
     const uint32_t hi = uint32_t(qb_int(int(uint32_t(b >> 32)), src));
     return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
-}
-
-struct PointMap {
-    double a00, a01, a10, a11, det;
-};
-
-// dxi/dx and det J of the bilinear map at (xi, eta): the host's quad_map, term for term.
-__device__ __forceinline__ PointMap point_map(const double (&V)[8], double xi, double eta) {
-#pragma clang fp contract(off)
-    const double dN[4][2] = {{-0.25 * (1.0 - eta), -0.25 * (1.0 - xi)}, {0.25 * (1.0 - eta), -0.25 * (1.0 + xi)},
-                             {0.25 * (1.0 + eta), 0.25 * (1.0 + xi)}, {-0.25 * (1.0 + eta), 0.25 * (1.0 - xi)}};
-    double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-        j00 += dN[n][0] * V[2 * n];
-        j01 += dN[n][0] * V[2 * n + 1];
-        j10 += dN[n][1] * V[2 * n];
-        j11 += dN[n][1] * V[2 * n + 1];
-    }
-    const double det = j00 * j11 - j01 * j10;
-    const double id = 1.0 / det;
-    return {j11 * id, -j01 * id, -j10 * id, j00 * id, det};
 }
 
 template <int P, typename Real>
@@ -126,11 +104,23 @@ __global__ __launch_bounds__(kTBlock) void quad_tensor_stage_kernel(StageParams 
         Real c1[R];  // P_m(s_a): weight of row m's T at this lane's xi node
 #pragma unroll
         for (int m = 0; m < R; ++m) c1[m] = own ? tab[T1 + m * R + a] : Real(0);
-        double V[8];
+        // bilinear map: dx/dxi = ex0 + ex1 eta, dx/deta = fx0 + ex1 xi (y alike), coefficients
+        // from the vertices in FP64, the per-point Jacobians in the state precision
+        Real ex0, ex1, fx0, ey0, ey1, fy0;
+        {
+            double V[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) V[k] = geo[(size_t)(D.gX + k) * Kp + e];
-        inv_det_c = 1.0 / fabs(point_map(V, 0.0, 0.0).det);
-        const double xa = t64[X1 + a], wa = t64[W1 + a];
+            for (int k = 0; k < 8; ++k) V[k] = geo[(size_t)(D.gX + k) * Kp + e];
+            const double dex0 = 0.25 * ((V[2] - V[0]) + (V[4] - V[6])), dex1 = 0.25 * ((V[0] - V[2]) + (V[4] - V[6]));
+            const double dfx0 = 0.25 * ((V[6] - V[0]) + (V[4] - V[2]));
+            const double dey0 = 0.25 * ((V[3] - V[1]) + (V[5] - V[7])), dey1 = 0.25 * ((V[1] - V[3]) + (V[5] - V[7]));
+            const double dfy0 = 0.25 * ((V[7] - V[1]) + (V[5] - V[3]));
+            inv_det_c = 1.0 / fabs(dex0 * dfy0 - dey0 * dfx0);
+            ex0 = Real(dex0), ex1 = Real(dex1), fx0 = Real(dfx0), ey0 = Real(dey0), ey1 = Real(dey1), fy0 = Real(dfy0);
+        }
+        const Real xa = Real(t64[X1 + a]);
+        const double wa = t64[W1 + a];
+        const Real j10 = fx0 + ex1 * xa, j11 = fy0 + ey1 * xa;  // d/deta at this lane's xi node
         Real S[R][4], S2[R][4];  // sum_q P_b(s_q) F^(s_a, s_q), sum_q P_b'(s_q) G^(s_a, s_q)
 #pragma unroll
         for (int b = 0; b < R; ++b)
@@ -146,10 +136,13 @@ __global__ __launch_bounds__(kTBlock) void quad_tensor_stage_kernel(StageParams 
                 for (int m = 0; m < R; ++m) s = fma(c1[m], qb(T[q][v], m), s);
                 U[v] = s;
             }
-            const PointMap G = point_map(V, xa, t64[X1 + q]);
-            const double wd = (wa * t64[W1 + q]) * fabs(G.det);
-            const Real a00 = Real(G.a00), a01 = Real(G.a01), a10 = Real(G.a10), a11 = Real(G.a11);
-            const Real w = own ? Real(wd) : Real(0);
+            const Real eta = Real(t64[X1 + q]);
+            const Real j00 = ex0 + ex1 * eta, j01 = ey0 + ey1 * eta;
+            const Real det = j00 * j11 - j01 * j10;
+            const Real id = rcp(det);
+            const Real a00 = j11 * id, a01 = -j01 * id, a10 = -j10 * id, a11 = j00 * id;
+            const Real wd = Real(wa * t64[W1 + q]) * fabs(det);
+            const Real w = own ? wd : Real(0);
             const Real ri = own ? rcp(U[0]) : Real(0);
             const Real uu = U[1] * ri, vv = U[2] * ri;
             const Real p = (gam - Real(1)) * (U[3] - Real(0.5) * U[0] * (uu * uu + vv * vv));
@@ -364,9 +357,9 @@ __global__ __launch_bounds__(kTBlock) void quad_tensor_stage_kernel(StageParams 
         const double sa = t64[IM1 + a] * inv_det_c;
 #pragma unroll
         for (int b = 0; b < R; ++b) {
-            const double s = sa * t64[IM1 + b];
+            const Real s = Real(sa * t64[IM1 + b]);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) r[b][v] = Real(s * double(acc[b][v]));
+            for (int v = 0; v < 4; ++v) r[b][v] = s * acc[b][v];
         }
     } else {
         const double* __restrict__ mi = sp.minve;
