@@ -50,8 +50,11 @@ __device__ __forceinline__ double qb(double x, int src) {
     return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
+// FP32: two waves per SIMD (256 VGPRs; the few values that do not fit are those held from
+// the start to the RK update); FP64 at the compiler's choice.
 template <int P, typename Real>
-__global__ __launch_bounds__(kTBlock) void quad_tensor_stage_kernel(StageParams sp) {
+__global__ __launch_bounds__(kTBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1)))
+void quad_tensor_stage_kernel(StageParams sp) {
     constexpr QuadDims D(P);
     constexpr int R = P + 1, NP = D.np;
     static_assert(R <= 4, "one mode row per lane");
