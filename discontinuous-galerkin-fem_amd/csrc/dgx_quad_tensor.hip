@@ -50,8 +50,8 @@ __device__ __forceinline__ double qb(double x, int src) {
     return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
-// FP32: two waves per SIMD (256 VGPRs; the few values that do not fit are those held from
-// the start to the RK update); FP64 at the compiler's choice.
+// FP32: two waves per SIMD (256 VGPRs; a few long-lived values spill); FP64 at the compiler's
+// choice.
 template <int P, typename Real>
 __global__ __launch_bounds__(kTBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1)))
 void quad_tensor_stage_kernel(StageParams sp) {
@@ -399,9 +399,11 @@ void quad_tensor_stage_kernel(StageParams sp) {
         for (int v = 0; v < 4; ++v) {
             const int ix = sidx(b * R + a, v, e, Kp);
             Real o;
+            // the own coefficients again from memory (the element's lines are in cache): holding
+            // them in registers through the faces costs more than the reload
             if (sp.mode == MODE_RHS) o = r[b][v];
-            else if (sp.ca == 0.0) o = u[b][v] + dt * r[b][v];
-            else o = ca * un[ix] + cb * (u[b][v] + dt * r[b][v]);
+            else if (sp.ca == 0.0) o = uk[ix] + dt * r[b][v];
+            else o = ca * un[ix] + cb * (uk[ix] + dt * r[b][v]);
             out[ix] = o;
         }
 }
