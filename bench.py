@@ -341,7 +341,8 @@ def bench_quad(args):
            "basis": "legendre", "volume_rule": f"{p + 1}x{p + 1} Gauss-Legendre", "dt": dt},
           roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload, kname), "kernel": kname,
-                    "launch_ms": stage_ms, "alg_bytes_per_launch": bytes_launch}, **extra)
+                    "launch_ms": stage_ms, "alg_bytes_per_launch": bytes_launch},
+          compute=compute_fraction(workload, kname, stage_ms), **extra)
 
 
 def bench_adjoint(args):
