@@ -226,21 +226,21 @@ void quad_tensor_stage_kernel(StageParams sp) {
 
     Real Un[R][4];
     if (!boundary) {
-        Real fx[R][R], fy[R][R];  // the neighbour's face fn at its point order
+        Real fy[R][R];  // the neighbour's face fn at its point order
+        int fxq[R];     // LDS index of its xi factors at point q
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int qq = flip ? (R - 1 - q) : q;
+            fxq[q] = FX + (fn * R + qq) * R;
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                fx[q][k] = tab[FX + (fn * R + qq) * R + k];
-                fy[q][k] = tab[FY + (fn * R + qq) * R + k];
-            }
+            for (int k = 0; k < R; ++k) fy[q][k] = tab[FY + (fn * R + qq) * R + k];
         }
 #pragma unroll
         for (int q = 0; q < R; ++q)
 #pragma unroll
             for (int v = 0; v < 4; ++v) Un[q][v] = ur[v];
-#pragma unroll
+        // the neighbour's mode rows one at a time (a runtime loop: one row's loads in flight)
+#pragma unroll 1
         for (int m = 0; m < R; ++m) {
             Real row[R][4];
 #pragma unroll
@@ -254,7 +254,7 @@ void quad_tensor_stage_kernel(StageParams sp) {
                     Real s = Real(0);
 #pragma unroll
                     for (int b = 0; b < R; ++b) s = fma(fy[q][b], row[b][v], s);
-                    Un[q][v] = fma(fx[q][m], s, Un[q][v]);
+                    Un[q][v] = fma(tab[fxq[q] + m], s, Un[q][v]);
                 }
         }
     } else {
