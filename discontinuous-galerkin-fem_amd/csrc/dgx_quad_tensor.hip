@@ -50,10 +50,11 @@ __device__ __forceinline__ double qb(double x, int src) {
     return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
-// FP32: two waves per SIMD (256 VGPRs; a few long-lived values spill); FP64 at the compiler's
-// choice.
+// FP32: three waves per SIMD (168 VGPRs); FP64 at the compiler's choice.  The volume residual
+// waits in LDS (16 KB per block in FP32) while the faces run, which keeps the face section's
+// traces, fluxes and table factors within the register budget.
 template <int P, typename Real>
-__global__ __launch_bounds__(kTBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1)))
+__global__ __launch_bounds__(kTBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 3 : 1)))
 void quad_tensor_stage_kernel(StageParams sp) {
     constexpr QuadDims D(P);
     constexpr int R = P + 1, NP = D.np;
@@ -179,6 +180,12 @@ void quad_tensor_stage_kernel(StageParams sp) {
             }
     }
 
+    // the volume residual waits in LDS while the faces run
+    __shared__ Real sacc[R * 4][kTBlock];
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sacc[b * 4 + v][threadIdx.x] = acc[b][v];
     // ---------------- faces: lane l evaluates face l ---------------------------------------
     const uint32_t code = sp.code[e];
     const uint32_t c = quad_code(code, l);
@@ -303,6 +310,10 @@ void quad_tensor_stage_kernel(StageParams sp) {
         for (int v = 0; v < 4; ++v) Hs[q][v] = sgn * sfe * H[v];
     }
 
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[b][v] = sacc[b * 4 + v][threadIdx.x];
     // lift: row a gains sum over faces m and points q of -w_q/2 P_a(xi) P_b(eta) Hs_m(q).
     // Faces 0, 2 (eta = -+1): P_b(eta) is the same at every point, so the points are summed
     // first with this row's factor -w_q/2 P_a(xi_q); faces 1, 3 (xi = +-1): P_a(xi) is this
