@@ -50,11 +50,12 @@ __device__ __forceinline__ double qb(double x, int src) {
     return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
-// FP32: three waves per SIMD (168 VGPRs); FP64 at the compiler's choice.  The volume residual
-// waits in LDS (16 KB per block in FP32) while the faces run, which keeps the face section's
-// traces, fluxes and table factors within the register budget.
+// FP32: four waves per SIMD (128 VGPRs, a few values spill; three waves fit without spills and
+// time 3% slower); FP64 at the compiler's choice.  The volume residual waits in LDS (16 KB per
+// block in FP32) while the faces run, which keeps the face section's traces, fluxes and table
+// factors within the register budget.
 template <int P, typename Real>
-__global__ __launch_bounds__(kTBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 3 : 1)))
+__global__ __launch_bounds__(kTBlock) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 4 : 1)))
 void quad_tensor_stage_kernel(StageParams sp) {
     constexpr QuadDims D(P);
     constexpr int R = P + 1, NP = D.np;
