@@ -28,11 +28,12 @@ namespace {
 
 constexpr int kTBlock = 256;  // 64 elements per block
 
-// The value x holds in lane `src` of this lane's quad (DPP quad_perm broadcast).  src is a
-// constant after unrolling; the switch selects the encoded control.
+// The value x holds in lane `src` of this lane's quad (DPP quad_perm broadcast; every lane of a
+// quad is active wherever it is used, so no old value is needed).  src is a constant after
+// unrolling; the switch selects the encoded control.
 template <int SRC>
 __device__ __forceinline__ int qb_i(int x) {
-    return __builtin_amdgcn_update_dpp(0, x, SRC * 0x55, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(x, SRC * 0x55, 0xF, 0xF, true);
 }
 __device__ __forceinline__ int qb_int(int x, int src) {
     switch (src) {
