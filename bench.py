@@ -140,6 +140,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run only a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--physics", default="euler", choices=["euler", "ns"],
+                    help="quad workload only: Euler, or Navier-Stokes (mu = 1e-3, Pr = 0.71, the consistent "
+                         "viscous scheme, dt at CFL 0.05)")
     ap.add_argument("--workload", default="euler",
                     choices=["euler", "ns", "quad", "adjoint", "batch"],
                     help="euler: the headline vortex case; ns: Taylor-Green Navier-Stokes; quad: the vortex "
@@ -295,26 +298,33 @@ def bench_quad(args):
     FP64 CPU oracle on a bounded sample of the same mesh."""
     n = args.n
     p = args.p
+    ns = args.physics == "ns"
     verts, quads = dgx.rect_quad_mesh(0.0, 10.0, 0.0, 10.0, 10.0 / n)
+    # Navier-Stokes: the consistent viscous scheme (true face gradients, +{Fv}.n, heat flux as
+    # conduction; the reference's treatment grows without bound here), and dt at CFL 0.05: at
+    # h = 0.039 the convective CFL 0.3 step exceeds the explicit viscous limit (the FP64 oracle
+    # reaches non-finite values within 20 steps there)
+    wf = (dgx.NavierStokesWeakFormulation(1.4, 1e-3, 0.71, 5.0, face_grad="correct", viscous_average="consistent",
+                                          heat_flux="conduction") if ns else None)
 
     def make(precision):
-        c = dgx.Context(verts, quads, p, precision=precision)
+        c = dgx.Context(verts, quads, p, weak_form=wf, precision=precision)
         xy = c.volume_points()
         c.project(vortex_ic(xy, c.K, c.Nq, SEED))
         return c
 
     ctx = make("fp32")
     K, Np = ctx.K, ctx.Np
-    dt = dgx.stable_dt(ctx, 0.3)
+    dt = dgx.stable_dt(ctx, 0.05 if ns else 0.3)
     u0 = ctx.get_state()
     secs = timed_steps(ctx, dt, args.steps, args.warmup)
     Uq32 = ctx.eval_volume()
     ctx.set_state(u0)
     stage_ms, _ = ctx.rk3_profile(dt, min(args.steps, 50))
     bytes_launch = ctx.info.bytes_per_stage_alg
-    workload = f"euler_vortex_quads_n{n}_p{p}"
+    workload = f"{'ns' if ns else 'euler'}_vortex_quads_n{n}_p{p}"
     kname = ("quad_tensor_stage_kernel<%d,float>" % p if ctx.info.stage_kernel == 2
-             else "quad_stage_kernel<%d,float,Euler>" % p)
+             else "quad_stage_kernel<%d,float,%s>" % (p, "NS" if ns else "Euler"))
     extra = {}
     if not args.no_fp64:
         c64 = make("fp64")
@@ -326,7 +336,8 @@ def bench_quad(args):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import refcpu as R
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ora = R.Oracle(verts, quads, p)
+        ora = (R.Oracle(verts, quads, p, physics="ns", mu=1e-3, prandtl=0.71, sigma0=5.0, ns_face_grad="correct",
+                        ns_average="consistent", heat_flux="conduction") if ns else R.Oracle(verts, quads, p))
         ora.rk3_threads(u0, dt, 1, threads)
         _, s = ora.rk3_threads(u0, dt, 4 * args.cpu_steps, threads)
         extra["cpu_baseline"] = {"value": 12.0 * Np * K * 4 * args.cpu_steps / s, "unit": "DOF-stage updates/s",
@@ -335,7 +346,7 @@ def bench_quad(args):
                                            f"on {threads} OpenMP threads ({s:.1f} s, oracle/refcpu.cpp)"}
     ctx.close()
     achieved = bytes_launch / (stage_ms * 1e-3) / 1e9
-    _line("Euler RHS DOF-stage updates/s on quadrilaterals (FP32)", 12.0 * Np * K * args.steps / secs,
+    _line(("NS" if ns else "Euler") + " RHS DOF-stage updates/s on quadrilaterals (FP32)", 12.0 * Np * K * args.steps / secs,
           "DOF-stage updates/s", args.steps, args.warmup, secs / args.steps * 1e3, "f32",
           {"workload": workload, "mesh": f"{n}x{n} structured quads", "elements": K, "order": p, "modes": Np,
            "basis": "legendre", "volume_rule": f"{p + 1}x{p + 1} Gauss-Legendre", "dt": dt},
