@@ -270,3 +270,71 @@ def test_threaded_rk3_is_bitwise_the_sequential_one(physics):
     a, _ = ora.rk3(u0, 1e-3, 3)
     b, _ = ora.rk3_threads(u0, 1e-3, 3, 4)
     assert np.array_equal(a, b)
+
+
+def _linear_state_coeffs(verts, tris, p, funcs):
+    """Monomial coefficients (1, eta, xi, ...) of a state whose four conserved variables are
+    linear functions of (x, y): the P1 part from the vertex values, zero above."""
+    K = len(tris)
+    Np = (p + 1) * (p + 2) // 2
+    u = np.zeros((K, Np, 4))
+    for e in range(K):
+        V = verts[tris[e]]
+        for v, f in enumerate(funcs):
+            a = [f(*V[k]) for k in range(3)]
+            u[e, :3, v] = [a[0], a[2] - a[0], a[1] - a[0]]
+    return u
+
+
+L_BOX = 10.0
+# rho = 2, m_x = 0.3 + 0.05 x, m_y = -0.2 + 0.08 y, E = 5 + 0.1 x - 0.04 y
+LIN = [lambda x, y: 2.0, lambda x, y: 0.3 + 0.05 * x, lambda x, y: -0.2 + 0.08 * y,
+       lambda x, y: 5.0 + 0.1 * x - 0.04 * y]
+
+
+def _int_lin(a, b, c, L=L_BOX):
+    """int over [0,L]^2 of a + b x + c y."""
+    return a * L * L + (b + c) * L ** 3 / 2
+
+
+def _int_sq(a, b, L=L_BOX):
+    """int over [0,L]^2 of (a + b x)^2."""
+    return L * (a * a * L + a * b * L * L + b * b * L ** 3 / 3)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_conservation_sums_closed_form(p):
+    # int U dA over the periodic [0,10]^2 box for a constant and a linear state: a dropped
+    # |J|, weight or variable moves the sums off the closed form
+    verts, tris = R.split_quad_mesh(4, L_BOX)
+    o = R.Oracle(verts, tris, p, basis="monomial")
+    U = prim(1.3, 0.4, -0.7, 1.1)
+    u = np.zeros((o.K, o.Np, 4))
+    u[:, 0, :] = U
+    assert np.allclose(o.conservation_sums(u), U * L_BOX ** 2, rtol=1e-14, atol=0)
+    u = _linear_state_coeffs(verts, tris, p, LIN)
+    exp = [_int_lin(2.0, 0, 0), _int_lin(0.3, 0.05, 0), _int_lin(-0.2, 0, 0.08), _int_lin(5.0, 0.1, -0.04)]
+    assert np.allclose(o.conservation_sums(u), exp, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("basis", ["monomial", "dubiner"])
+def test_kinetic_energy_closed_form(p, basis):
+    # J = int (m_x^2 + m_y^2) / (2 rho) dA (the adjoint workload's functional).  Constant state:
+    # 1/2 |m|^2 / rho times the area.  Linear momentum at constant density: a quadratic
+    # integrand, integrated exactly by the degree-2p rule.  A missing 1/2, a dropped component
+    # or a weight without |J| fails one of the two.
+    verts, tris = R.split_quad_mesh(4, L_BOX)
+    o = R.Oracle(verts, tris, p, basis=basis)
+    rho, mx, my = 1.3, 0.4, -0.7
+    u = np.zeros((o.K, o.Np, 4))
+    u[:, 0, :] = np.array([rho, mx, my, 3.0]) * (1.0 if basis == "monomial" else 0.5)  # phi_0 = 1 / 2
+    assert o.kinetic_energy(u) == pytest.approx(0.5 * (mx * mx + my * my) / rho * L_BOX ** 2, rel=1e-13)
+    if basis == "monomial":
+        u = _linear_state_coeffs(verts, tris, p, LIN)
+    else:
+        xy = o.volume_points()
+        Uq = np.stack([f(xy[..., 0], xy[..., 1]) + 0 * xy[..., 0] for f in LIN], axis=-1)
+        u = o.project(Uq)
+    exp = (_int_sq(0.3, 0.05) + _int_sq(-0.2, 0.08)) / (2 * 2.0)
+    assert o.kinetic_energy(u) == pytest.approx(exp, rel=1e-13)

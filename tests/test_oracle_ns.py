@@ -251,19 +251,92 @@ def test_viscous_flux_closed_form_shear():
     assert np.allclose(Gv, [0, 0.03, 0, y * 0.03], atol=1e-15)
 
 
+def _wall_state(verts, tris):
+    """P1 monomial coefficients of a flow with non-zero normal and tangential velocity on every
+    side of the unit square: rho = 1 + 0.2 x, u = 0.3 + 0.2 y, v = -0.4 + 0.1 x, p = 1 + 0.3 y
+    (conserved variables taken linear between the vertex values)."""
+    def W(x, y):
+        return prim2cons([1.0 + 0.2 * x, 0.3 + 0.2 * y, -0.4 + 0.1 * x, 1.0 + 0.3 * y])
+    u = np.zeros((2, 3, 4))
+    for e in range(2):
+        V = verts[tris[e]]
+        a = [W(*V[k]) for k in range(3)]
+        u[e] = np.array([a[0], a[2] - a[0], a[1] - a[0]])
+    return u
+
+
+def _cons2prim(U, gam=GAMMA):
+    rho, ru, rv, E = U
+    u, v = ru / rho, rv / rho
+    return np.array([rho, u, v, (gam - 1) * (E - 0.5 * rho * (u * u + v * v))])
+
+
+def _wall_expected(verts, tris, o, u, ghost):
+    """Boundary residual rebuilt from the vertex coordinates: U_L at the GL face points,
+    the ghost state from `ghost(W_L, n)`, Rusanov, R -= w |e|/2 phi H."""
+    s, wg = R.gauss_legendre(2)
+    out = []
+    for k, (e, f, _) in enumerate(o.boundary_faces()):
+        V, _, _, _ = elem_geom(verts, tris, e)
+        n, L = outward_normal(V, f)
+        exp = np.zeros((3, 4))
+        for q, x in enumerate(face_points(V, f, s)):
+            a, b = ref_point(V, x)
+            ph = np.array([1, b, a])
+            UL = ph @ u[e]
+            UR = ghost(_cons2prim(UL), n)
+            FL, sL = _euler_flux(UL, n)
+            FR, sR = _euler_flux(UR, n)
+            H = 0.5 * (FL + FR) - 0.5 * max(sL, sR) * (UR - UL)
+            exp -= wg[q] * L / 2 * np.outer(ph, H)
+        out.append((k, e, n, exp))
+    return out
+
+
 def test_euler_boundary_slip_wall_reflects_normal_velocity():
-    # boundary_face_residual SLIP_WALL (euler_weak_formulation.cpp:214-224): a tangential flow on
-    # a wall has zero mass flux through the wall
+    # boundary_face_residual SLIP_WALL (euler_weak_formulation.cpp:214-224): the ghost state
+    # keeps rho and p and mirrors the normal velocity, u_R = u_t - u_n.  The state has a
+    # non-zero normal velocity on every wall, so a ghost that copied the interior state (or
+    # dropped the reflection) gives a different flux; the reflected state carries no mass
+    # through the wall even though u.n != 0
     verts, tris = mesh2()
     o = R.Oracle(verts, tris, 1, basis="monomial", periodic=0)
     for tag in ("Bottom", "Right", "Top", "Left"):
         o.set_bc(tag, "slip_wall")
-    u = np.zeros((2, 3, 4))
-    u[:, 0, :] = prim2cons([1.0, 0.0, 0.0, 1.0])
-    for k, (e, f, _) in enumerate(o.boundary_faces()):
-        Rb = o.boundary_face_residual(k, u[e])
-        assert np.abs(Rb[:, 0]).max() < 1e-15          # no mass flux
-        assert np.abs(Rb[:, 3]).max() < 1e-15          # no energy flux
+    u = _wall_state(verts, tris)
+
+    def reflect(W, n):
+        un = W[1] * n[0] + W[2] * n[1]
+        return prim2cons([W[0], W[1] - 2 * un * n[0], W[2] - 2 * un * n[1], W[3]])
+
+    def copy(W, n):
+        return prim2cons(W)
+
+    for (k, e, n, exp), (_, _, _, wrong) in zip(_wall_expected(verts, tris, o, u, reflect),
+                                                _wall_expected(verts, tris, o, u, copy)):
+        got = o.boundary_face_residual(k, u[e])
+        assert np.allclose(got, exp, rtol=0, atol=1e-13)
+        assert np.abs(got[:, 0]).max() < 1e-14           # no mass through the wall
+        assert np.abs(wrong[:, 0]).max() > 1e-2          # ... which an unreflected ghost has
+        assert np.abs(got - wrong).max() > 1e-2
+
+
+def test_euler_boundary_no_slip_ghost_is_the_primitive_condition():
+    # boundary_face_residual NO_SLIP_WALL (euler_weak_formulation.cpp:225-227): the ghost state
+    # is primitive_to_conserved(bc(x)), whatever the interior state
+    verts, tris = mesh2()
+    o = R.Oracle(verts, tris, 1, basis="monomial", periodic=0)
+    Wb = [1.2, 0.0, 0.0, 0.9]
+    for tag in ("Bottom", "Right", "Top", "Left"):
+        o.set_bc(tag, "no_slip_wall", Wb)
+    u = _wall_state(verts, tris)
+    for k, e, n, exp in _wall_expected(verts, tris, o, u, lambda W, n: prim2cons(Wb)):
+        got = o.boundary_face_residual(k, u[e])
+        assert np.allclose(got, exp, rtol=0, atol=1e-13)
+    # the same data read as a conserved state (the far-field branch) is a different flux
+    wrong = _wall_expected(verts, tris, o, u, lambda W, n: np.array(Wb))
+    k, e = wrong[0][0], wrong[0][1]
+    assert np.abs(o.boundary_face_residual(k, u[e]) - wrong[0][3]).max() > 1e-2
 
 
 def _euler_flux(U, n, gam=GAMMA):
