@@ -1164,16 +1164,41 @@ dgx_status dgx_synchronize(dgx_ctx* c) {
     });
 }
 
-dgx_status dgx_rk3_steps(dgx_ctx* c, double dt, int32_t nsteps) {
-    dgx_status st = dgx_rk3_steps_async(c, dt, nsteps);
-    if (st != DGX_OK) return st;
+// The reference checks the state for NaN after every step and stops at the first step that
+// has one (run_time_integration, S/solver/compressible_solver_base.cpp:279-301).  Here the steps
+// run back to back and the state is checked once at the end; since a NaN coefficient stays NaN
+// through every later stage (each element's update reads its own coefficients), a clean final
+// state means no step had one.  Otherwise the run is replayed from a copy of its starting
+// state one step at a time with the check after each; the stage kernels are deterministic, so
+// the replay reproduces the same states bit for bit and stops at the reference's step.
+dgx_status dgx_rk3_run(dgx_ctx* c, double dt, int32_t nsteps, int32_t* steps_done) {
+    if (steps_done) *steps_done = 0;
     return guarded([&] {
-        std::vector<double> h = c->reduce(3);
-        double bad = 0.0;
-        for (int b = 0; b < c->red_blocks(); ++b) bad += h[b * 4];
-        if (bad > 0) throw Fail(DGX_E_NONFINITE, "Solution contains NaN values");
+        if (!c) throw std::invalid_argument("null context");
+        if (nsteps < 0 || !std::isfinite(dt)) throw std::invalid_argument("bad step count or dt");
+        if (nsteps == 0) return;
+        const size_t sb = c->state_bytes();
+        if (!c->ub.p) c->ub.alloc(sb);
+        hip_check(hipMemcpyAsync(c->ub.p, c->u.p, sb, hipMemcpyDeviceToDevice, c->stream), "copy");
+        for (int s = 0; s < nsteps; ++s) c->rk3_step(dt);
+        if (c->nan_count() == 0.0) {
+            if (steps_done) *steps_done = nsteps;
+            return;
+        }
+        hip_check(hipMemcpyAsync(c->u.p, c->ub.p, sb, hipMemcpyDeviceToDevice, c->stream), "copy");
+        for (int s = 1; s <= nsteps; ++s) {
+            c->rk3_step(dt);
+            if (c->nan_count() > 0.0) {
+                if (steps_done) *steps_done = s;
+                throw Fail(DGX_E_NONFINITE, "Solution contains NaN values at step " + std::to_string(s) +
+                                                " of " + std::to_string(nsteps));
+            }
+        }
+        throw Fail(DGX_E_DEVICE, "a NaN in the final state did not recur when the steps were replayed");
     });
 }
+
+dgx_status dgx_rk3_steps(dgx_ctx* c, double dt, int32_t nsteps) { return dgx_rk3_run(c, dt, nsteps, nullptr); }
 
 dgx_status dgx_rk3_profile(dgx_ctx* c, double dt, int32_t nsteps, double* stage_ms, double* total_ms) {
     return guarded([&] {

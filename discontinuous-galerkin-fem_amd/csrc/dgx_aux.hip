@@ -79,7 +79,7 @@ constexpr int kMaxOut = 4;
 //   kind 0: conservation sums  sum_e |J| sum_i m_i u_i,v         (4 values, sum)
 //   kind 1: kinetic energy     sum_e sum_q w_q |J| |m|^2/(2 rho)  (1 value, sum)
 //   kind 2: [max (|u|+c)/h_min, rho min, rho max, #bad points]   (max, min, max, sum)
-//   kind 3: [#non-finite state entries]                           (sum)
+//   kind 3: [#NaN state entries, #infinite state entries]         (sum, sum)
 template <typename Real>
 __global__ __launch_bounds__(kRed) void reduce_kernel(int kind, const Real* __restrict__ u,
                                                       const double* __restrict__ tab64,
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(kRed) void reduce_kernel(int kind, const Real* __re
         } else if (kind == 3) {
             for (int k = 0; k < np * 4; ++k) {
                 double x = double(u[sidx(k >> 2, k & 3, e, Kp)]);
-                if (!isfinite(x)) val[0] += 1.0;
+                if (isnan(x)) val[0] += 1.0;
+                else if (isinf(x)) val[1] += 1.0;
             }
         } else {
             double ws = 0.0;

@@ -111,12 +111,13 @@ struct dgx_ctx {
     DevBuf tabVL, tabVR;  // MFMA stage adjoint lane tables (same two variants)
     DevBuf tabNT, rec;    // MFMA Navier-Stokes face-trace tables and face-point records
     DevBuf minve;         // quads: per-element inverse mass matrices [Np*Np][Kp]
+    DevBuf ub;            // the state at the start of a checked run (dgx_rk3_run), allocated on first use
     std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][nf][2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~dgx_ctx() {
         for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage,
-                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec, &minve})
+                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec, &minve, &ub})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -193,6 +194,15 @@ struct dgx_ctx {
         stage_launch(u1.p, u.p, u2.p, MODE_RK, dt, 0.75, 0.25);
         stage_launch(u2.p, u.p, u1.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
         std::swap(u, u1);
+    }
+
+    // NaN entries of the state's coefficients (the reference's per-element hasNaN()); infinite
+    // entries are not counted, as there
+    double nan_count() {
+        std::vector<double> h = reduce(3);
+        double n = 0.0;
+        for (int b = 0; b < red_blocks(); ++b) n += h[b * 4];
+        return n;
     }
 
     std::vector<double> reduce(int kind) {

@@ -61,7 +61,7 @@ __device__ __forceinline__ void rusanov2(const float* A, const float* B, float n
     const float pb = (g - 1.0f) * (B[3] - 0.5f * B[0] * (ub * ub + vb * vb));
     const float una = ua * nx + va * ny, unb = ub * nx + vb * ny;
     const float ca = dsqrt(g * pa * ria), cb = dsqrt(g * pb * rib);
-    const float s = fmaxf(fabsf(una) + ca, fabsf(unb) + cb);
+    const float s = ref_max(fabsf(una) + ca, fabsf(unb) + cb);
     const float fa0 = A[0] * una, fb0 = B[0] * unb;
     const float fa1 = A[1] * una + pa * nx, fb1 = B[1] * unb + pb * nx;
     const float fa2 = A[2] * una + pa * ny, fb2 = B[2] * unb + pb * ny;

@@ -370,7 +370,7 @@ __global__ void quad_minv_kernel(const double* __restrict__ minve, double* __res
 
 // Per-block FP64 reductions with the triangle kernels' output layout:
 //   kind 0: sum_q w_q |J_q| U_q (4), kind 1: kinetic energy, kind 2: [max (|u|+c)/h_min,
-//   rho min, rho max, #bad points], kind 3: [#non-finite state entries].
+//   rho min, rho max, #bad points], kind 3: [#NaN state entries, #infinite state entries].
 template <typename Real>
 __global__ __launch_bounds__(kQRed) void quad_reduce_kernel(int kind, const Real* __restrict__ u,
                                                             const double* __restrict__ phiV,
@@ -384,8 +384,11 @@ __global__ __launch_bounds__(kQRed) void quad_reduce_kernel(int kind, const Real
     if (kind == 2) { val[1] = INFINITY; val[2] = -INFINITY; }
     if (e < K) {
         if (kind == 3) {
-            for (int k = 0; k < np * 4; ++k)
-                if (!isfinite(double(u[sidx(k >> 2, k & 3, e, Kp)]))) val[0] += 1.0;
+            for (int k = 0; k < np * 4; ++k) {
+                const double x = double(u[sidx(k >> 2, k & 3, e, Kp)]);
+                if (isnan(x)) val[0] += 1.0;
+                else if (isinf(x)) val[1] += 1.0;
+            }
         } else {
             double ws = 0.0;
             for (int q = 0; q < nq; ++q) {

@@ -36,6 +36,12 @@ __device__ __forceinline__ Real dsqrt(Real x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }  // v_sqrt_f32, 1 ulp
 
+// std::max(a, b) as the reference evaluates it, (a < b) ? b : a: a NaN first operand is
+// returned, a NaN second operand is dropped (fmax drops either), so a state that goes NaN does
+// so at the same step as in the reference.
+template <typename Real>
+__device__ __forceinline__ Real ref_max(Real a, Real b) { return (a < b) ? b : a; }
+
 // Block -> element-tile remap so that consecutive tiles (mesh neighbours) land on the same
 // XCD and share its L2 (blocks are dealt round-robin over the 8 XCDs).
 __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
@@ -55,7 +61,7 @@ __device__ __forceinline__ void rusanov(const Real* A, const Real* B, Real nx, R
     Real pb = (g - Real(1)) * (B[3] - half * B[0] * (ub * ub + vb * vb));
     Real una = ua * nx + va * ny, unb = ub * nx + vb * ny;
     Real ca = dsqrt(g * pa * ria), cb = dsqrt(g * pb * rib);
-    Real s = fmax(fabs(una) + ca, fabs(unb) + cb);
+    Real s = ref_max(fabs(una) + ca, fabs(unb) + cb);
     Real fa0 = A[0] * una, fb0 = B[0] * unb;
     Real fa1 = A[1] * una + pa * nx, fb1 = B[1] * unb + pb * nx;
     Real fa2 = A[2] * una + pa * ny, fb2 = B[2] * unb + pb * ny;
@@ -72,7 +78,7 @@ template <typename Real>
 __device__ __forceinline__ void viscous_flux(const Real* U, const Real* g, Real gam, Real mu,
                                              Real kappa, Real* Fv, Real* Gv) {
     Real rho = U[0];
-    Real ri_safe = rcp(fmax(rho, Real(1e-12)));
+    Real ri_safe = rcp(ref_max(rho, Real(1e-12)));
     Real ri = rcp(rho);
     Real u = U[1] * ri, v = U[2] * ri;
     Real p = (gam - Real(1)) * (U[3] - Real(0.5) * rho * (u * u + v * v));

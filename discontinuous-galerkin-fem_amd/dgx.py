@@ -41,7 +41,7 @@ EXPORTS = [
     "dgx_mesh_read_gmsh", "dgx_create", "dgx_destroy", "dgx_last_error",
     "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
     "dgx_eval_volume", "dgx_get_geometry", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk",
-    "dgx_residual", "dgx_apply_minv", "dgx_rk3_steps", "dgx_rk3_steps_async",
+    "dgx_residual", "dgx_apply_minv", "dgx_rk3_run", "dgx_rk3_steps", "dgx_rk3_steps_async",
     "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
     "dgx_density_range", "dgx_kinetic_energy", "dgx_residual_vjp", "dgx_rhs_vjp",
     "dgx_kinetic_energy_grad", "dgx_rk3_vjp", "dgx_rk3_vjp_profile",
@@ -122,6 +122,7 @@ def load():
     L.dgx_get_geometry.argtypes = [vp, dp, dp, C.POINTER(C.c_uint32)]
     L.dgx_write_vtk.argtypes = [vp, dp, C.c_char_p, C.c_double, C.c_int32]
     L.dgx_residual.argtypes = [vp, dp, dp]
+    L.dgx_rk3_run.argtypes = [vp, C.c_double, C.c_int32, C.POINTER(C.c_int32)]
     L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]
     L.dgx_rk3_steps_async.argtypes = [vp, C.c_double, C.c_int32]
     L.dgx_synchronize.argtypes = [vp]
@@ -404,6 +405,16 @@ class Context:
     def rk3_steps(self, dt, nsteps):
         _check(load().dgx_rk3_steps(self._h, dt, nsteps))
 
+    def rk3_run(self, dt, nsteps):
+        """Up to nsteps steps with the reference's per-step NaN abort: returns (steps taken,
+        nan), the context holding the state of the last step taken."""
+        done = C.c_int32(0)
+        st = load().dgx_rk3_run(self._h, dt, nsteps, C.byref(done))
+        if st == DGX_E_NONFINITE:
+            return done.value, True
+        _check(st)
+        return done.value, False
+
     def rk3_steps_async(self, dt, nsteps):
         _check(load().dgx_rk3_steps_async(self._h, dt, nsteps))
 
@@ -580,9 +591,8 @@ class DGAssembler:
 class CompressibleDGSolver:
     """EulerDGSolver / NavierStokesDGSolver: solve(ic, T_final, dt, save_every) returns the
     flattened frames (K, 4 Np) with column i*4+v (flatten_frames,
-    S/solver/compressible_solver_base.cpp:340-368), frames at step 0, every save_every steps,
-    and at the end when not aligned.  The step count is ceil(T_final/dt) explicitly instead of
-    the reference's floating accumulation of t."""
+    S/solver/compressible_solver_base.cpp:340-368) under the reference's frame and step-count
+    rules (see solve)."""
 
     def __init__(self, ctx):
         self.ctx = ctx
@@ -610,22 +620,42 @@ class CompressibleDGSolver:
         return tuple(self.ctx.density_range())
 
     def solve(self, ic, T_final, dt, save_every):
+        """run_time_integration (S/solver/compressible_solver_base.cpp:201-338): the step count of
+        its `while (t < T_final) { ...; t += dt; }` loop, a frame at step 0 and at the top of
+        every iteration whose step is a positive multiple of save_every (so only when another
+        step follows), the NaN abort after each step, and a last frame when the final step
+        count is not a multiple of save_every."""
+        if save_every <= 0:
+            raise ValueError("save_every must be positive")
+        n_steps = reference_step_count(T_final, dt)
         self.project_initial_condition(ic)
-        n_steps = int(math.ceil(T_final / dt))
         frames = [self.ctx.get_state()]
         step = 0
         while step < n_steps:
-            chunk = min(save_every - step % save_every, n_steps - step)
-            st = load().dgx_rk3_steps(self.ctx._h, dt, chunk)
-            step += chunk
-            if st == DGX_E_NONFINITE:
-                break
-            _check(st)
-            if step % save_every == 0:
+            if step % save_every == 0 and step > 0:
                 frames.append(self.ctx.get_state())
+            chunk = min(save_every - step % save_every, n_steps - step)
+            done, nan = self.ctx.rk3_run(dt, chunk)
+            step += done
+            if nan:
+                break
         if step % save_every != 0:
             frames.append(self.ctx.get_state())
+        self.steps_taken = step
         return [f.reshape(self.ctx.K, self.ctx.Np * 4) for f in frames]
+
+
+def reference_step_count(T_final, dt):
+    """Iterations of the reference's `while (t < T_final) { ...; t += dt; }` with t starting at
+    0.0 (compressible_solver_base.cpp:242,258,284): FP64 accumulation of t can add a step beyond
+    ceil(T_final / dt), e.g. 11 steps for T_final = 1, dt = 0.1."""
+    if not (dt > 0.0) or not math.isfinite(T_final):
+        raise ValueError("dt must be positive and T_final finite")
+    t, n = 0.0, 0
+    while t < T_final:
+        t += dt
+        n += 1
+    return n
 
 
 EulerDGSolver = CompressibleDGSolver

@@ -49,9 +49,13 @@ int main(int argc, char** argv) {
         double dt = 0.3 / ((2 * order + 1) * lam);
         double s0[4], s1[4];
         check(dgx_conservation_sums(ctx.get(), s0));
+        // T_final = steps dt and save_every = steps, as the reference's driver loop counts them
+        // (its FP64 accumulation of t may add one step; solver.steps_taken() reports the count)
         auto frames = solver.solve(ic, steps * dt, dt, steps);
+        const int taken = solver.steps_taken();
         check(dgx_conservation_sums(ctx.get(), s1));
-        auto [rmin, rmax] = solver.compute_density_range(frames.back());
+        // the final state is the context's (with an aligned step count it is not a frame)
+        auto [rmin, rmax] = solver.compute_density_range(solver.get_state());
 
         // density error against the exact vortex at the volume points
         const auto& I = ctx.info();
@@ -60,11 +64,11 @@ int main(int argc, char** argv) {
         check(dgx_eval_volume(ctx.get(), Uq.data()));
         double num = 0, den = 0;
         for (size_t k = 0; k < xy.size() / 2; ++k) {
-            double r = vortex(xy[2 * k], xy[2 * k + 1], steps * dt, L)[0];
+            double r = vortex(xy[2 * k], xy[2 * k + 1], taken * dt, L)[0];
             num += (Uq[4 * k] - r) * (Uq[4 * k] - r);
             den += r * r;
         }
-        std::printf("elements=%d order=%d dt=%.17g steps=%d frames=%zu", I.n_elements, order, dt, steps, frames.size());
+        std::printf("elements=%d order=%d dt=%.17g steps=%d frames=%zu", I.n_elements, order, dt, taken, frames.size());
         for (int v = 0; v < 4; ++v) std::printf(" sum0_%d=%.17g sum1_%d=%.17g", v, s0[v], v, s1[v]);
         std::printf(" rho_min=%.17g rho_max=%.17g l2_rho=%.6e\n", rmin, rmax, std::sqrt(num / den));
         return 0;

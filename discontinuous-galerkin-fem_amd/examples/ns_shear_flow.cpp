@@ -53,7 +53,7 @@ int main(int argc, char** argv) {
         }
         double err[4];
         check(dgx_l2_error(ctx.get(), W.data(), 1, err));
-        std::printf("elements=%d steps=%d frames=%zu", I.n_elements, int(std::ceil(T_final / dt)), frames.size());
+        std::printf("elements=%d steps=%d frames=%zu", I.n_elements, solver.steps_taken(), frames.size());
         for (int v = 0; v < 4; ++v) std::printf(" err_%d=%.17g", v, err[v]);
         std::printf("\n");
         return 0;

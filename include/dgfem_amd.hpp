@@ -332,30 +332,52 @@ public:
         return u;
     }
 
-    // solve(ic, T_final, dt, save_every): frames at step 0, every save_every steps and at the
-    // end if not aligned, each flattened to (K, 4 Np) with column i*4+v.  The step count is
-    // ceil(T_final/dt); a non-finite state ends the loop early like the reference's NaN abort.
+    // solve(ic, T_final, dt, save_every) as run_time_integration does it
+    // (S/solver/compressible_solver_base.cpp:201-338): the step count of its
+    // `while (t < T_final) { ...; t += dt; }` loop, a frame at step 0 and at the top of every
+    // iteration whose step is a positive multiple of save_every (only when another step
+    // follows), the NaN abort after each step, and a last frame when the final step count is
+    // not a multiple of save_every.  Each frame is the (K, 4 Np) row-major matrix with column
+    // i*4+v, i.e. the [K][Np][4] state itself.  steps_taken() reports the steps run.
     std::vector<std::vector<double>> solve(const InitialCondition& ic, double T_final, double dt, int save_every) {
         if (save_every <= 0) throw std::invalid_argument("save_every must be positive");
+        const int n_steps = reference_step_count(T_final, dt);
         std::vector<std::vector<double>> frames;
         frames.push_back(project_initial_condition(ic));
-        const int n_steps = int(std::ceil(T_final / dt));
         int step = 0;
         while (step < n_steps) {
-            int chunk = std::min(save_every - step % save_every, n_steps - step);
-            dgx_status st = dgx_rk3_steps(ctx_.get(), dt, chunk);
-            step += chunk;
+            if (step % save_every == 0 && step > 0) frames.push_back(get_state());
+            const int chunk = std::min(save_every - step % save_every, n_steps - step);
+            int32_t done = 0;
+            const dgx_status st = dgx_rk3_run(ctx_.get(), dt, chunk, &done);
+            step += done;
             if (st == DGX_E_NONFINITE) break;
             check(st);
-            if (step % save_every == 0) frames.push_back(get_state());
         }
         if (step % save_every != 0) frames.push_back(get_state());
-        return frames;  // [K][Np][4] is already the (K, 4 Np) row-major frame with column i*4+v
+        steps_taken_ = step;
+        return frames;
+    }
+    int steps_taken() const { return steps_taken_; }
+
+    // Iterations of the reference's `while (t < T_final) { ...; t += dt; }` from t = 0
+    // (compressible_solver_base.cpp:242,258,284): the FP64 accumulation can add a step beyond
+    // ceil(T_final / dt).
+    static int reference_step_count(double T_final, double dt) {
+        if (!(dt > 0.0) || !std::isfinite(T_final)) throw std::invalid_argument("dt must be positive and T_final finite");
+        double t = 0.0;
+        int n = 0;
+        while (t < T_final) {
+            t += dt;
+            ++n;
+        }
+        return n;
     }
 
 private:
     DGContext& ctx_;
     DGAssembler assembler_;
+    int steps_taken_ = 0;
 };
 
 using EulerDGSolver = CompressibleDGSolver;

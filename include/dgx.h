@@ -239,10 +239,16 @@ dgx_status dgx_residual(dgx_ctx* ctx, const double* u, double* r);
 /* r <- M^{-1} r in place, [K][Np][4]: CompressibleDGSolverBase::apply_mass_inv
  * (S/solver/compressible_solver_base.cpp:54-61). */
 dgx_status dgx_apply_minv(dgx_ctx* ctx, double* r);
-/* nsteps SSP-RK3 steps of the device state (time_step_ssp_rk3 x nsteps,
- * S/solver/compressible_solver_base.cpp:106-112; I/solver/time_stepping.hpp:39-55).
- * Returns DGX_E_NONFINITE if the state holds NaN/Inf afterwards (the reference's abort check,
- * :290-301); the state is left as computed. */
+/* Up to nsteps SSP-RK3 steps of the device state with the reference's NaN abort: the steps of
+ * run_time_integration's loop (time_step_ssp_rk3, S/solver/compressible_solver_base.cpp:
+ * 106-112,279-301; I/solver/time_stepping.hpp:39-55), which checks every element's coefficients
+ * for NaN (Eigen hasNaN; infinite values do not count) after each step and stops at the first
+ * step that has one.  Returns DGX_E_NONFINITE in that case, with the context holding the state
+ * of that step and *steps_done its number (1-based); otherwise DGX_OK and *steps_done = nsteps.
+ * steps_done may be NULL.  Costs one device copy of the state per call plus one reduction
+ * (and, only when a NaN appears, a replay of the call's steps with a check after each). */
+dgx_status dgx_rk3_run(dgx_ctx* ctx, double dt, int32_t nsteps, int32_t* steps_done);
+/* dgx_rk3_run without the step count. */
 dgx_status dgx_rk3_steps(dgx_ctx* ctx, double dt, int32_t nsteps);
 /* Same, without the host round trip of the non-finite check; for timing loops and capture. */
 dgx_status dgx_rk3_steps_async(dgx_ctx* ctx, double dt, int32_t nsteps);
