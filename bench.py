@@ -17,17 +17,24 @@ Output: one JSON line (rank 0).  Besides the contract keys it carries
   compute       : the compute side for the same kernel: SIMD issue cycles (VALU + FP32 MFMA,
                   which serialise on gfx950) from the committed SQ counters over the live
                   launch time, and the FP32 matrix-product rate vs 157.3 TFLOP/s
-  cpu_baseline  : the FP64 CPU oracle (port of the reference loop) on OMP_NUM_THREADS threads
-                  (and on one thread, single_thread_value), timed on
-                  a bounded sample of the same mesh
   fp64          : the same workload in FP64 (DOF-stage/s)
-  l2_err_fp32_vs_fp64 : per-field relative L2 at volume points after the timed steps
+  accuracy      : per-field relative L2 at the volume points after the workload's 200 steps
+                  (whatever --steps is): FP32 vs the FP64 run, and vs the analytic vortex on
+                  the unperturbed twin
   bf16_volume_variant : the same run with bfloat16 operands in the volume products (FP32
-                  accumulation and state): DOF-stage/s and per-field L2 vs FP64
+                  accumulation and state): DOF-stage/s and per-field L2 vs FP64 at 200 steps
+  adjoint       : the metric's second half at the adjoint workload (K = 32768, 200 steps,
+                  checkpoints every 20): reverse sweep / forward and whole gradient / forward
+  cpu_baseline  : the reference algorithm (oracle/refcpu.cpp) built with the reference's
+                  Release flags on this host, FP64, median of 3 runs on OMP_NUM_THREADS
+                  threads (value) and on one thread, with and without -ffast-math; lscpu model
+  fp64_vs_oracle_full_size : max relative difference of the GPU FP64 state and the CPU
+                  baseline's single-thread state after the same steps at the full size
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -41,6 +48,8 @@ import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 SEED = 20251203
+ACC_STEPS = 200   # the headline workload's step count: its accuracy condition is stated there
+ADJ_STEPS = 200   # the adjoint workload's step count
 
 
 def vortex_ic(xy, K, Nq, seed):
@@ -72,19 +81,53 @@ def l2_rel(a, b):
     return [float(np.linalg.norm(a[..., v] - b[..., v]) / np.linalg.norm(b[..., v])) for v in range(4)]
 
 
-def cpu_baseline(verts, tris, p, u0, dt, steps, threads):
-    """The FP64 CPU oracle (a port of the reference's loop structure) on the host: `steps` steps
-    on one thread, and 4x as many on `threads` OpenMP threads (the same arithmetic, bitwise),
-    each after one warm-up step.  Returns DOF-stage/s and seconds for both."""
+def host_cpu():
+    """(lscpu model name, nproc) of the host running the bench."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=20).stdout.strip())
+    except (OSError, subprocess.SubprocessError, ValueError):
+        nproc = os.cpu_count()
+    return model, nproc
+
+
+def cpu_baseline(verts, tris, p, u0, dt, steps, threads, runs=3):
+    """The reference algorithm on the host's CPU: the oracle's loop structure (oracle/refcpu.cpp)
+    compiled as the reference's Release build (-O3 -march=native, with and without -ffast-math;
+    built here, on the timing host, by `make -B -C oracle timing`), one warm-up step, then the
+    median of `runs` timed runs of `steps` steps on one thread (the reference is
+    single-threaded) and of 4 x `steps` steps on `threads` OpenMP threads.  Also returns the
+    state of the single-thread run of `steps` steps from u0 (the full-size FP64 parity check).
+    Returns ({variant: {...}}, u_steps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcpu as R
-    ora = R.Oracle(verts, tris, p, basis="dubiner")
-    dof = 12.0 * ora.Np * ora.K
-    ora.rk3(u0, dt, 1)
-    _, secs1 = ora.rk3(u0, dt, steps)
-    ora.rk3_threads(u0, dt, 1, threads)
-    _, secsn = ora.rk3_threads(u0, dt, 4 * steps, threads)
-    return dof * steps / secs1, secs1, dof * 4 * steps / secsn, secsn
+    subprocess.run(["make", "-B", "-s", "-C", os.path.join(ROOT, "oracle"), "timing"], check=True,
+                   stdout=subprocess.DEVNULL)
+    out, u_steps = {}, None
+    for variant in ("native", "fastmath"):
+        ora = R.Oracle(verts, tris, p, basis="dubiner", build=variant)
+        dof = 12.0 * ora.Np * ora.K
+        ora.rk3(u0, dt, 1)
+        one = []
+        for _ in range(runs):
+            u, secs = ora.rk3(u0, dt, steps)
+            one.append(secs)
+        if variant == "native":
+            u_steps = u
+        ora.rk3_threads(u0, dt, 1, threads)
+        many = [ora.rk3_threads(u0, dt, 4 * steps, threads)[1] for _ in range(runs)]
+        s1, sn = float(np.median(one)), float(np.median(many))
+        out[variant] = {"single_thread": dof * steps / s1, "single_thread_s": one,
+                        "all_threads": dof * 4 * steps / sn, "all_threads_s": many}
+        del ora
+    return out, u_steps
 
 
 def load_counters(workload, kernel=None):
@@ -142,7 +185,7 @@ def main():
                     help="run only a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--physics", default="euler", choices=["euler", "ns"],
                     help="quad workload only: Euler, or Navier-Stokes (mu = 1e-3, Pr = 0.71, the consistent "
-                         "viscous scheme, dt at CFL 0.05)")
+                         "viscous scheme, dt capped by the explicit viscous limit)")
     ap.add_argument("--workload", default="euler",
                     choices=["euler", "ns", "quad", "adjoint", "batch"],
                     help="euler: the headline vortex case; ns: Taylor-Green Navier-Stokes; quad: the vortex "
@@ -166,7 +209,6 @@ def main():
     u0 = ctx32.get_state()
 
     secs32 = timed_steps(ctx32, dt, args.steps, args.warmup)
-    Uq32 = ctx32.eval_volume()
     dof_stage = 12.0 * Np * K * args.steps
     value = dof_stage / secs32
 
@@ -200,46 +242,87 @@ def main():
         "compute": compute_fraction(workload, kname, stage_ms),
     }
 
+    FIELDS = ["rho", "rhou", "rhov", "E"]
+    ctx64 = None
     if not args.no_fp64:
         ctx64, _, _, _ = make_ctx(n, p, "fp64", SEED)
         secs64 = timed_steps(ctx64, dt, args.steps, args.warmup)
-        Uq64 = ctx64.eval_volume()
         result["fp64"] = {"value": dof_stage / secs64, "ms_per_step": secs64 / args.steps * 1e3}
-        err = l2_rel(Uq32, Uq64)
-        result["l2_err_fp32_vs_fp64"] = dict(zip(["rho", "rhou", "rhov", "E"], err))
-        # variant: bfloat16 operands in the volume products, FP32 accumulation and state
+        # the workload's accuracy condition at its own 200 steps, whatever --steps is: FP32
+        # (and the bfloat16 volume variant) against the FP64 run from the same state and dt
+        acc = {"steps": ACC_STEPS}
+        ctx64.set_state(u0)
+        ctx64.rk3_steps(dt, ACC_STEPS)
+        Uq64 = ctx64.eval_volume()
+        ctx32.set_state(u0)
+        ctx32.rk3_steps(dt, ACC_STEPS)
+        acc["l2_err_fp32_vs_fp64"] = dict(zip(FIELDS, l2_rel(ctx32.eval_volume(), Uq64)))
         if p in (3, 4):
             ctxb, _, _, _ = make_ctx(n, p, "fp32_bf16vol", SEED)
             secsb = timed_steps(ctxb, dt, args.steps, args.warmup)
-            Uqb = ctxb.eval_volume()
             ctxb.set_state(u0)
             stage_b, _ = ctxb.rk3_profile(dt, min(args.steps, 50))
+            ctxb.set_state(u0)
+            ctxb.rk3_steps(dt, ACC_STEPS)
             result["bf16_volume_variant"] = {
                 "value": dof_stage / secsb, "ms_per_step": secsb / args.steps * 1e3, "launch_ms": stage_b,
                 "kernel": "stage_mfma_bf16_kernel_p%d" % p,
-                "l2_err_vs_fp64": dict(zip(["rho", "rhou", "rhov", "E"], l2_rel(Uqb, Uq64)))}
+                "l2_err_vs_fp64_200_steps": dict(zip(FIELDS, l2_rel(ctxb.eval_volume(), Uq64)))}
             ctxb.close()
-        ctx64.close()
+        # analytic oracle on the unperturbed twin (periodic-truncation floor ~1e-5)
+        ctxa, _, _, xya = make_ctx(n, p, "fp32", None)
+        ctxa.rk3_steps(dt, ACC_STEPS)
+        acc["l2_err_fp32_vs_analytic"] = dict(zip(FIELDS, l2_rel(ctxa.eval_volume(),
+                                                                 vortex_ic_exact(xya, dt * ACC_STEPS))))
+        ctxa.close()
+        result["accuracy"] = acc
 
-    # analytic oracle on the unperturbed twin (periodic-truncation floor ~1e-5)
-    ctxa, _, _, xya = make_ctx(n, p, "fp32", None)
-    ctxa.rk3_steps(dt, args.steps)
-    Uqa = ctxa.eval_volume()
-    exact = vortex_ic_exact(xya, dt * args.steps)
-    result["l2_err_fp32_vs_analytic"] = dict(zip(["rho", "rhou", "rhov", "E"], l2_rel(Uqa, exact)))
-    ctxa.close()
+    # the other half of the metric: adjoint / forward time at the adjoint workload (128 x 128
+    # split quads, K = 32768, p = 3, FP32, 200 steps, checkpoints every 20), whatever --steps is
+    result["adjoint"] = adjoint_ratios(128, p, ADJ_STEPS, 20)
 
     if not args.no_cpu:
         # the box's CPU share is what OMP_NUM_THREADS says (not os.cpu_count())
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        v1, s1, vn, sn = cpu_baseline(verts, tris, p, u0, dt, args.cpu_steps, threads)
-        result["cpu_baseline"] = {"value": vn, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
-                                  "sample": f"{4 * args.cpu_steps} FP64 SSP-RK3 steps of the same {K}-element p={p} "
-                                            f"mesh on {threads} OpenMP threads ({sn:.1f} s, oracle/refcpu.cpp); "
-                                            f"{args.cpu_steps} steps on one thread ({s1:.1f} s)",
-                                  "single_thread_value": v1, "host_cpus": os.cpu_count()}
+        model, nproc = host_cpu()
+        var, u_cpu = cpu_baseline(verts, tris, p, u0, dt, args.cpu_steps, threads)
+        fm = var["fastmath"]
+        result["cpu_baseline"] = {
+            "value": fm["all_threads"], "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+            "sample": f"median of 3 runs of {4 * args.cpu_steps} FP64 SSP-RK3 steps of the same {K}-element p={p} "
+                      f"mesh from the same state and dt on {threads} OpenMP threads, reference algorithm restated "
+                      f"(oracle/refcpu.cpp) and built as the reference's Release build (-O3 -march=native "
+                      f"-ffast-math); single thread: median of 3 runs of {args.cpu_steps} steps",
+            "single_thread_value": fm["single_thread"],
+            "variants": {"O3_march_native_ffast_math": fm, "O3_march_native": var["native"]},
+            "cpu_model": model, "nproc": nproc}
+        if ctx64 is not None:
+            # full-size FP64 parity: the GPU's FP64 state after the same steps from the same state
+            ctx64.set_state(u0)
+            ctx64.rk3_steps(dt, args.cpu_steps)
+            g = ctx64.get_state()
+            result["fp64_vs_oracle_full_size"] = {
+                "steps": args.cpu_steps, "elements": K,
+                "max_rel_diff": float(np.abs(g - u_cpu).max() / np.abs(u_cpu).max())}
+    if ctx64 is not None:
+        ctx64.close()
     ctx32.close()
     print(json.dumps(result))
+
+
+def adjoint_ratios(n, p, steps, every):
+    """Reverse sweep and whole gradient (checkpointed forward + reverse) over a plain forward
+    run of the same steps, gradient of the final kinetic energy w.r.t. the initial state."""
+    ctx, _, _, _ = make_ctx(n, p, "fp32", SEED)
+    dt = dgx.stable_dt(ctx, 0.3)
+    ctx.rk3_vjp_profile(dt, 2 * every, every)  # warm-up: checkpoint buffers, code paths
+    fwd, ckpt, rev = ctx.rk3_vjp_profile(dt, steps, every)
+    out = {"workload": f"euler_vortex_adjoint_n{n}_p{p}", "elements": ctx.K, "steps": steps,
+           "checkpoint_every": every, "forward_ms": fwd, "checkpointed_forward_ms": ckpt,
+           "reverse_sweep_ms": rev, "reverse_over_forward": rev / fwd,
+           "gradient_over_forward": (ckpt + rev) / fwd}
+    ctx.close()
+    return out
 
 
 def _line(metric, value, unit, steps, warmup, ms_per_step, dtype, config, **extra):
@@ -301,9 +384,9 @@ def bench_quad(args):
     ns = args.physics == "ns"
     verts, quads = dgx.rect_quad_mesh(0.0, 10.0, 0.0, 10.0, 10.0 / n)
     # Navier-Stokes: the consistent viscous scheme (true face gradients, +{Fv}.n, heat flux as
-    # conduction; the reference's treatment grows without bound here), and dt at CFL 0.05: at
-    # h = 0.039 the convective CFL 0.3 step exceeds the explicit viscous limit (the FP64 oracle
-    # reaches non-finite values within 20 steps there)
+    # conduction; the reference's treatment grows without bound here), and dt capped by the
+    # explicit viscous limit: at h = 0.039 the convective CFL 0.3 step exceeds it (the FP64
+    # oracle reaches non-finite values within 20 steps there)
     wf = (dgx.NavierStokesWeakFormulation(1.4, 1e-3, 0.71, 5.0, face_grad="correct", viscous_average="consistent",
                                           heat_flux="conduction") if ns else None)
 
@@ -315,9 +398,20 @@ def bench_quad(args):
 
     ctx = make("fp32")
     K, Np = ctx.K, ctx.Np
-    dt = dgx.stable_dt(ctx, 0.05 if ns else 0.3)
+    dt = dgx.stable_dt(ctx, 0.3)
+    if ns:
+        # explicit viscous limit ~ h^2 / (mu (p+1)^4) (penalty and face gradients scale with
+        # (p+1)^2 / h each); the factor 0.02 keeps the n = 256 run finite (the FP64 oracle goes
+        # non-finite at the convective CFL 0.3 step there) and leaves coarser meshes at CFL 0.3
+        h = 10.0 / n
+        dt = min(dt, 0.02 * h * h / (1e-3 * (p + 1) ** 4))
     u0 = ctx.get_state()
     secs = timed_steps(ctx, dt, args.steps, args.warmup)
+    lo, hi = ctx.density_range()
+    if not (np.isfinite(lo) and np.isfinite(hi) and lo > 0):
+        print(f"bench.py: the stepped state is not physical (density range [{lo}, {hi}]); no line printed",
+              file=sys.stderr)
+        sys.exit(3)
     Uq32 = ctx.eval_volume()
     ctx.set_state(u0)
     stage_ms, _ = ctx.rk3_profile(dt, min(args.steps, 50))

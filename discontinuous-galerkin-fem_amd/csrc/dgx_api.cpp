@@ -202,6 +202,14 @@ QuadMap quad_map(const double* V, double xi, double eta) {
 
 // The quad space (DGSpace for element type "quad", S/core/space.cpp:47-55): tables, per-point
 // geometry and per-element inverse mass matrices on the host, then uploaded.
+// Largest element count whose state indices (i Kp + e) 4 + v <= 4 Np Kp - 1, with Kp = K rounded up to 256,
+// stay below 2^31 (sidx, the int index of the per-lane, quad, pack and reduction kernels).
+int max_elements(int order, int nvpe) {
+    const long long np = nvpe == 4 ? (long long)(order + 1) * (order + 1) : (long long)(order + 1) * (order + 2) / 2;
+    const long long kp = (1LL << 31) / (4 * np) / 256 * 256;  // largest multiple of 256 with 4 np kp <= 2^31
+    return (int)std::min<long long>(kp, INT32_MAX);
+}
+
 void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
     if (s.basis != DGX_BASIS_LEGENDRE) throw std::invalid_argument("quadrilateral elements use the Legendre basis");
     if (s.rule != DGX_RULE_2P) throw Fail(DGX_E_UNSUPPORTED, "quadrilateral elements use the tensor Gauss-Legendre rule");
@@ -447,6 +455,12 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
     if (s.order < 1 || s.order > 4) throw std::invalid_argument("Element order must be between 1 and 4");
     const int nvpe = m.vertices_per_element == 0 ? 3 : m.vertices_per_element;
     if (nvpe != 3 && nvpe != 4) throw std::invalid_argument("elements must be triangles or quadrilaterals");
+    // every stage, pack and reduction kernel forms state indices in 32 bits: reject larger
+    // meshes before any host or device allocation, for both element types
+    if (m.n_elements > max_elements(s.order, nvpe))
+        throw Fail(DGX_E_UNSUPPORTED, "mesh too large: state indices exceed 32 bits (at most " +
+                                          std::to_string(max_elements(s.order, nvpe)) + " elements of order " +
+                                          std::to_string(s.order) + ")");
     if (nvpe == 3 && s.basis == DGX_BASIS_LEGENDRE) throw std::invalid_argument("the Legendre basis is for quadrilaterals");
     if (nvpe == 3 && s.basis != DGX_BASIS_MONOMIAL && s.basis != DGX_BASIS_DUBINER) throw std::invalid_argument("unknown basis");
     if (s.rule != DGX_RULE_2P && s.rule != DGX_RULE_2P1) throw std::invalid_argument("unknown volume rule");
@@ -657,8 +671,6 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
     // (state, geometry, or the NS face-point records) must stay below 2 GiB
     const double big = std::max({16.0 * c.np, 8.0 * kGeoRows, c.physics == DGX_NAVIER_STOKES ? 96.0 * c.nfq : 0.0}) * c.Kp;
     const bool fits32 = big < 2147483648.0;
-    if (4.0 * c.np * c.Kp >= 2147483648.0)
-        throw Fail(DGX_E_UNSUPPORTED, "mesh too large: state indices exceed 32 bits");
     if (precision == DGX_FP32_BF16VOL && !fits32)
         throw Fail(DGX_E_UNSUPPORTED, "mesh too large for the BF16-input volume variant (2 GiB buffers)");
     if (precision != DGX_FP64 && c.physics == DGX_EULER && c.rule == 0 && mfma_order(c.p) && !lane_only && fits32) {
@@ -830,6 +842,16 @@ dgx_status dgx_create(const dgx_mesh_desc* mesh, const dgx_space_desc* space, co
 }
 
 void dgx_destroy(dgx_ctx* ctx) { delete ctx; }
+
+dgx_status dgx_max_elements(int32_t order, int32_t vertices_per_element, int32_t* out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null argument");
+        if (order < 1 || order > 4) throw std::invalid_argument("Element order must be between 1 and 4");
+        const int nvpe = vertices_per_element == 0 ? 3 : vertices_per_element;
+        if (nvpe != 3 && nvpe != 4) throw std::invalid_argument("elements must be triangles or quadrilaterals");
+        *out = max_elements(order, nvpe);
+    });
+}
 
 dgx_status dgx_info(const dgx_ctx* c, dgx_info_t* o) {
     if (!c || !o) { g_error = "null argument"; return DGX_E_INVALID; }

@@ -39,7 +39,7 @@ BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_voi
 EXPORTS = [
     "dgx_mesh_split_quads", "dgx_mesh_connect", "dgx_mesh_connect_elements", "dgx_mesh_rect_quads",
     "dgx_mesh_read_gmsh", "dgx_create", "dgx_destroy", "dgx_last_error",
-    "dgx_info", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
+    "dgx_info", "dgx_max_elements", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
     "dgx_eval_volume", "dgx_get_geometry", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk",
     "dgx_residual", "dgx_apply_minv", "dgx_rk3_run", "dgx_rk3_steps", "dgx_rk3_steps_async",
     "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
@@ -122,6 +122,7 @@ def load():
     L.dgx_get_geometry.argtypes = [vp, dp, dp, C.POINTER(C.c_uint32)]
     L.dgx_write_vtk.argtypes = [vp, dp, C.c_char_p, C.c_double, C.c_int32]
     L.dgx_residual.argtypes = [vp, dp, dp]
+    L.dgx_max_elements.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     L.dgx_rk3_run.argtypes = [vp, C.c_double, C.c_int32, C.POINTER(C.c_int32)]
     L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]
     L.dgx_rk3_steps_async.argtypes = [vp, C.c_double, C.c_int32]
@@ -643,6 +644,13 @@ class CompressibleDGSolver:
             frames.append(self.ctx.get_state())
         self.steps_taken = step
         return [f.reshape(self.ctx.K, self.ctx.Np * 4) for f in frames]
+
+
+def max_elements(order, vertices_per_element=3):
+    """Largest element count a context of this order and element type accepts."""
+    n = C.c_int32(0)
+    _check(load().dgx_max_elements(order, vertices_per_element, C.byref(n)))
+    return n.value
 
 
 def reference_step_count(T_final, dt):

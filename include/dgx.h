@@ -34,7 +34,7 @@ enum {
     DGX_E_INVALID = 1,     /* bad argument (the reference's std::invalid_argument)        */
     DGX_E_UNSUPPORTED = 2, /* valid request outside what this build implements            */
     DGX_E_DEVICE = 3,      /* HIP runtime error, or the HIP library/device is unavailable  */
-    DGX_E_NONFINITE = 4,   /* NaN/Inf detected in the state (the reference aborts its loop) */
+    DGX_E_NONFINITE = 4,   /* NaN in the state after a step (the reference aborts its loop) */
     DGX_E_NOMEM = 5
 };
 
@@ -186,6 +186,12 @@ dgx_status dgx_create(const dgx_mesh_desc* mesh, const dgx_space_desc* space,
 void dgx_destroy(dgx_ctx* ctx);
 const char* dgx_last_error(void);
 dgx_status dgx_info(const dgx_ctx* ctx, dgx_info_t* out);
+/* The largest element count dgx_create accepts for elements of this order and vertex count
+ * (3 or 0: triangles, 4: quadrilaterals): the kernels index the state in 32 bits, so
+ * 4 Np Kp <= 2^31 with Kp the element count rounded up to 256.  dgx_create rejects larger meshes
+ * with DGX_E_UNSUPPORTED before allocating anything.  No counterpart in the reference (Eigen
+ * indexes with 64-bit Index). */
+dgx_status dgx_max_elements(int32_t order, int32_t vertices_per_element, int32_t* max_elements);
 
 /* ---- state ---------------------------------------------------------------------------------- */
 

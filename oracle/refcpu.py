@@ -16,22 +16,27 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "librefcpu.so")
-_lib = None
+# the checker build (strict FP64: -ffp-contract=off, portable x86-64-v3) and the two timing
+# builds of the CPU baseline, compiled like the reference's Release build on the host that
+# runs them (-O3 -march=native, with and without -ffast-math; `make -C oracle timing`)
+_LIB_PATHS = {"checker": os.path.join(_HERE, "_build", "librefcpu.so"),
+              "native": os.path.join(_HERE, "_build", "librefcpu_native.so"),
+              "fastmath": os.path.join(_HERE, "_build", "librefcpu_fastmath.so")}
+_libs = {}
 
 BC_FN = C.CFUNCTYPE(None, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 
-def build():
-    subprocess.check_call(["make", "-s", "-C", _HERE])
+def build(target="all"):
+    subprocess.check_call(["make", "-s", "-C", _HERE, target])
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        L = C.CDLL(_LIB_PATH)
+def lib(variant="checker"):
+    if variant not in _libs:
+        path = _LIB_PATHS[variant]
+        if not os.path.exists(path):
+            build("all" if variant == "checker" else "timing")
+        L = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int32)
         L.ref_create.restype = C.c_void_p
@@ -77,8 +82,8 @@ def lib():
         L.ref_viscous_flux.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.ref_penalty.restype = C.c_double
         L.ref_penalty.argtypes = [C.c_void_p, C.c_double]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _dp(a):
@@ -197,8 +202,8 @@ class Oracle:
     def __init__(self, verts, tris, order, basis=None, vol_rule=0, periodic=3,
                  physics="euler", gamma=1.4, mu=1e-3, prandtl=0.72, sigma0=5.0,
                  ns_face_grad="reference", euler_bc_points="reference", legendre_endpoint="reference",
-                 ns_average="reference", heat_flux="reference"):
-        L = lib()
+                 ns_average="reference", heat_flux="reference", build="checker"):
+        L = self._L = lib(build)
         self._verts = np.ascontiguousarray(verts, dtype=np.float64)
         self._tris = np.ascontiguousarray(tris, dtype=np.int32)
         self.gamma = gamma
@@ -230,12 +235,12 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().ref_destroy(self._h)
+            self._L.ref_destroy(self._h)
             self._h = None
 
     def _chk(self, rc):
         if rc != 0:
-            raise RuntimeError(lib().ref_last_error().decode())
+            raise RuntimeError(self._L.ref_last_error().decode())
 
     def set_bc(self, tag, bc_type, value=None, fn=None):
         t = {"far_field": 0, "slip_wall": 1, "no_slip_wall": 2}[bc_type]
@@ -248,7 +253,7 @@ class Oracle:
                     out[k] = float(r[k])
             cb = BC_FN(_cb)
         self._cbs.append((val, cb))
-        lib().ref_set_bc(self._h, tag.encode(), t, _dp(val), cb, None)
+        self._L.ref_set_bc(self._h, tag.encode(), t, _dp(val), cb, None)
 
     def zeros(self):
         return np.zeros((self.K, self.Np, 4))
@@ -256,101 +261,101 @@ class Oracle:
     def residual(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         r = np.zeros_like(u)
-        self._chk(lib().ref_residual(self._h, _dp(u), _dp(r)))
+        self._chk(self._L.ref_residual(self._h, _dp(u), _dp(r)))
         return r
 
     def rhs(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         k = np.zeros_like(u)
-        self._chk(lib().ref_rhs(self._h, _dp(u), _dp(k)))
+        self._chk(self._L.ref_rhs(self._h, _dp(u), _dp(k)))
         return k
 
     def apply_minv(self, r):
         r = np.array(r, dtype=np.float64, order="C")
-        self._chk(lib().ref_apply_minv(self._h, _dp(r)))
+        self._chk(self._L.ref_apply_minv(self._h, _dp(r)))
         return r
 
     def rk3(self, u, dt, nsteps):
         """Returns (u_new, seconds in the stepping loop)."""
         u = np.array(u, dtype=np.float64, order="C")
-        secs = lib().ref_rk3(self._h, _dp(u), dt, nsteps)
+        secs = self._L.ref_rk3(self._h, _dp(u), dt, nsteps)
         return u, secs
 
     def rk3_threads(self, u, dt, nsteps, nthreads):
         """rk3 on `nthreads` OpenMP threads (same result bit for bit): (u_new, seconds)."""
         u = np.array(u, dtype=np.float64, order="C")
-        secs = lib().ref_rk3_threads(self._h, _dp(u), dt, nsteps, nthreads)
+        secs = self._L.ref_rk3_threads(self._h, _dp(u), dt, nsteps, nthreads)
         return u, secs
 
     def volume_points(self):
         xy = np.zeros((self.K, self.Nq, 2))
-        self._chk(lib().ref_volume_points(self._h, _dp(xy)))
+        self._chk(self._L.ref_volume_points(self._h, _dp(xy)))
         return xy
 
     def project(self, Uq):
         Uq = np.ascontiguousarray(Uq, dtype=np.float64)
         u = self.zeros()
-        self._chk(lib().ref_project(self._h, _dp(Uq), _dp(u)))
+        self._chk(self._L.ref_project(self._h, _dp(Uq), _dp(u)))
         return u
 
     def eval_volume(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         Uq = np.zeros((self.K, self.Nq, 4))
-        self._chk(lib().ref_eval_volume(self._h, _dp(u), _dp(Uq)))
+        self._chk(self._L.ref_eval_volume(self._h, _dp(u), _dp(Uq)))
         return Uq
 
     def max_cfl(self, u, dt):
         u = np.ascontiguousarray(u, dtype=np.float64)
-        return lib().ref_max_cfl(self._h, _dp(u), dt)
+        return self._L.ref_max_cfl(self._h, _dp(u), dt)
 
     def density_range(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         o = np.zeros(2)
-        lib().ref_density_range(self._h, _dp(u), _dp(o))
+        self._L.ref_density_range(self._h, _dp(u), _dp(o))
         return o
 
     def conservation_sums(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         o = np.zeros(4)
-        lib().ref_conservation_sums(self._h, _dp(u), _dp(o))
+        self._L.ref_conservation_sums(self._h, _dp(u), _dp(o))
         return o
 
     def kinetic_energy(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
-        return lib().ref_kinetic_energy(self._h, _dp(u))
+        return self._L.ref_kinetic_energy(self._h, _dp(u))
 
     # element / face level
     def elem_data(self, e):
         j = np.zeros(self.Nq)
         d = np.zeros(self.Nq * self.Np * 2)
-        lib().ref_elem_data(self._h, e, _dp(j), _dp(d))
+        self._L.ref_elem_data(self._h, e, _dp(j), _dp(d))
         return j, d.reshape(self.Nq, self.Np, 2)
 
     def face_geom(self, e, f):
         n = np.zeros(2)
         ln = np.zeros(1)
-        lib().ref_face_geom(self._h, e, f, _dp(n), _dp(ln))
+        self._L.ref_face_geom(self._h, e, f, _dp(n), _dp(ln))
         return n, float(ln[0])
 
     def interior_faces(self):
         o = np.zeros((max(self.n_interior, 1), 5), dtype=np.int32)
-        lib().ref_interior_faces(self._h, _ip(o))
+        self._L.ref_interior_faces(self._h, _ip(o))
         return o[: self.n_interior]
 
     def neighbors(self):
         o = np.zeros((self.K, self.nvpe, 2), dtype=np.int32)
-        lib().ref_neighbors(self._h, _ip(o))
+        self._L.ref_neighbors(self._h, _ip(o))
         return o
 
     def boundary_faces(self):
         o = np.zeros((max(self.n_boundary, 1), 3), dtype=np.int32)
-        lib().ref_boundary_faces(self._h, _ip(o))
+        self._L.ref_boundary_faces(self._h, _ip(o))
         return o[: self.n_boundary]
 
     def volume_residual(self, e, ue, viscous=False):
         ue = np.ascontiguousarray(ue, dtype=np.float64)
         R = np.zeros((self.Np, 4))
-        fn = lib().ref_viscous_volume_residual if viscous else lib().ref_volume_residual
+        fn = self._L.ref_viscous_volume_residual if viscous else self._L.ref_volume_residual
         self._chk(fn(self._h, e, _dp(ue), _dp(R)))
         return R
 
@@ -359,14 +364,14 @@ class Oracle:
         uR = np.ascontiguousarray(uR, dtype=np.float64)
         RL = np.zeros((self.Np, 4))
         RR = np.zeros((self.Np, 4))
-        self._chk(lib().ref_interior_face_residual(self._h, k, _dp(uL), _dp(uR), _dp(RL), _dp(RR),
+        self._chk(self._L.ref_interior_face_residual(self._h, k, _dp(uL), _dp(uR), _dp(RL), _dp(RR),
                                                    int(viscous)))
         return RL, RR
 
     def boundary_face_residual(self, k, ue, viscous=False):
         ue = np.ascontiguousarray(ue, dtype=np.float64)
         R = np.zeros((self.Np, 4))
-        self._chk(lib().ref_boundary_face_residual(self._h, k, _dp(ue), _dp(R), int(viscous)))
+        self._chk(self._L.ref_boundary_face_residual(self._h, k, _dp(ue), _dp(R), int(viscous)))
         return R
 
     def viscous_flux(self, U, gradU):
@@ -374,11 +379,11 @@ class Oracle:
         g = np.ascontiguousarray(gradU, dtype=np.float64).reshape(8)
         F = np.zeros(4)
         G = np.zeros(4)
-        lib().ref_viscous_flux(self._h, _dp(U), _dp(g), _dp(F), _dp(G))
+        self._L.ref_viscous_flux(self._h, _dp(U), _dp(g), _dp(F), _dp(G))
         return F, G
 
     def penalty(self, h):
-        return lib().ref_penalty(self._h, h)
+        return self._L.ref_penalty(self._h, h)
 
 
 # ----------------------------------------------------------------------------------------------

@@ -108,3 +108,39 @@ def test_null_arguments_are_rejected():
     assert L.dgx_set_state(None, None) == dgx.DGX_E_INVALID
     assert L.dgx_rk3_steps(None, 0.1, 1) == dgx.DGX_E_INVALID
     assert L.dgx_create(None, None, None, 0, None) == dgx.DGX_E_INVALID
+
+
+@pytest.mark.parametrize("nv", [3, 4])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_max_elements_is_the_32_bit_state_index_boundary(order, nv):
+    # state index (i Kp + e) 4 + v < 2^31 for i < Np, e < Kp (K rounded up to 256): the last
+    # accepted K keeps it, the next multiple of 256 does not
+    np_ = (order + 1) ** 2 if nv == 4 else (order + 1) * (order + 2) // 2
+    kmax = dgx.max_elements(order, nv)
+    kp = -(-kmax // 256) * 256
+    assert kmax % 256 == 0
+    assert 4 * np_ * kp <= 2 ** 31
+    assert 4 * np_ * (kp + 256) > 2 ** 31
+    # quads of order 4 stop near 21.5 M elements, triangles of order 1 near 179 M
+    if (order, nv) == (4, 4):
+        assert 21_000_000 < kmax < 21_500_000
+
+
+@pytest.mark.parametrize("nv", [3, 4])
+def test_create_rejects_meshes_beyond_the_index_range_first(nv):
+    # n_elements one past the limit with tiny arrays: rejected before the arrays are read or
+    # anything is allocated (host or device), for triangles and quadrilaterals alike
+    order = 4
+    kmax = dgx.max_elements(order, nv)
+    v = np.zeros((4, 2))
+    t = np.zeros((1, nv), dtype=np.int32)
+    nb = np.zeros((1, nv, 2), dtype=np.int32)
+    mesh = dgx.MeshDesc(len(v), v.ctypes.data_as(C.POINTER(C.c_double)), kmax + 1,
+                        t.ctypes.data_as(C.POINTER(C.c_int32)), nb.ctypes.data_as(C.POINTER(C.c_int32)),
+                        0, None, None, nv)
+    space = dgx.SpaceDesc(order, 2 if nv == 4 else 1, 0)
+    phys = dgx.Physics(0, 1.4, 0.0, 0.72, 5.0, 0)
+    h = C.c_void_p()
+    st = dgx.load().dgx_create(C.byref(mesh), C.byref(space), C.byref(phys), 1, C.byref(h))
+    assert st == dgx.DGX_E_UNSUPPORTED and not h.value
+    assert b"32 bits" in dgx.load().dgx_last_error()
