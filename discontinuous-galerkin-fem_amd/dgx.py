@@ -42,7 +42,7 @@ EXPORTS = [
     "dgx_info", "dgx_max_elements", "dgx_set_state", "dgx_get_state", "dgx_volume_points", "dgx_project",
     "dgx_eval_volume", "dgx_get_geometry", "dgx_l2_error", "dgx_set_bc_function", "dgx_write_vtk",
     "dgx_residual", "dgx_apply_minv", "dgx_rk3_run", "dgx_rk3_steps", "dgx_rk3_steps_async",
-    "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_max_cfl",
+    "dgx_synchronize", "dgx_rk3_profile", "dgx_conservation_sums", "dgx_conservation_sums_device", "dgx_max_cfl",
     "dgx_density_range", "dgx_kinetic_energy", "dgx_residual_vjp", "dgx_rhs_vjp",
     "dgx_kinetic_energy_grad", "dgx_rk3_vjp", "dgx_rk3_vjp_profile",
     "dgx_batch_create", "dgx_batch_destroy", "dgx_batch_instance", "dgx_batch_set_dt",
@@ -122,6 +122,7 @@ def load():
     L.dgx_get_geometry.argtypes = [vp, dp, dp, C.POINTER(C.c_uint32)]
     L.dgx_write_vtk.argtypes = [vp, dp, C.c_char_p, C.c_double, C.c_int32]
     L.dgx_residual.argtypes = [vp, dp, dp]
+    L.dgx_conservation_sums_device.argtypes = [vp, vp]
     L.dgx_max_elements.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     L.dgx_rk3_run.argtypes = [vp, C.c_double, C.c_int32, C.POINTER(C.c_int32)]
     L.dgx_rk3_steps.argtypes = [vp, C.c_double, C.c_int32]

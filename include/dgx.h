@@ -270,6 +270,15 @@ dgx_status dgx_rk3_profile(dgx_ctx* ctx, double dt, int32_t nsteps, double* stag
 
 /* Integral of each conserved variable over the mesh, FP64 deterministic reduction. */
 dgx_status dgx_conservation_sums(dgx_ctx* ctx, double out[4]);
+/* The conservation sums of dgx_conservation_sums written to device memory: out_dev = 4 FP64
+ * values on the context's device, equal bit for bit to dgx_conservation_sums.  No counterpart in
+ * the reference.  For callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops): it allocates nothing, sets no memory and does not synchronise; it is ordered on the
+ * context's stream, so read out_dev after dgx_synchronize (or stream-ordered work).  Its block
+ * partial sums live in one fixed 1 MiB device array per GPU shared by every call on that GPU:
+ * calls that overlap on one GPU (several contexts on concurrent streams) race.  Meshes of more
+ * than 8388608 elements are rejected with DGX_E_UNSUPPORTED. */
+dgx_status dgx_conservation_sums_device(dgx_ctx* ctx, double* out_dev);
 /* compute_max_cfl / compute_density_range (S/solver/compressible_solver_base.cpp:114-199):
  * CFL = dt * max_q(|u|+c) / h_min(e); returns +inf for rho<=0 or p<=0 like the reference. */
 dgx_status dgx_max_cfl(dgx_ctx* ctx, double dt, double* cfl);
