@@ -98,7 +98,8 @@ void upload_bc_table(dgx_ctx& c) {
 // FP64 geometry, element sizes and face codes on the device from the mesh arrays (validated
 // on the host); a singular mapping or a degenerate edge is reported like the reference.
 void build_geometry(dgx_ctx& c, const dgx_mesh_desc& m) {
-    DevBuf dV, dT, dN, derr;
+    DevBuf dV, dT, dN, dP, derr;
+    if (!c.perm.empty()) c.upload(dP, c.perm.data(), c.perm.size() * sizeof(int32_t));
     dV.alloc((size_t)m.n_vertices * 2 * sizeof(double));
     dT.alloc((size_t)c.K * 3 * sizeof(int32_t));
     dN.alloc((size_t)c.K * 6 * sizeof(int32_t));
@@ -110,8 +111,8 @@ void build_geometry(dgx_ctx& c, const dgx_mesh_desc& m) {
     hip_check(hipMemcpyAsync(dT.p, m.elements, (size_t)c.K * 3 * sizeof(int32_t), hipMemcpyHostToDevice, c.stream), "copy");
     hip_check(hipMemcpyAsync(dN.p, m.face_neighbors, (size_t)c.K * 6 * sizeof(int32_t), hipMemcpyHostToDevice, c.stream), "copy");
     hip_check(hipMemsetAsync(derr.p, 0, sizeof(int), c.stream), "memset");
-    hip_check(launch_geometry(dV.as<double>(), static_cast<const int32_t*>(dT.p), static_cast<const int32_t*>(dN.p), c.K,
-                              c.Kp, c.geo.as<double>(), c.hmin.as<double>(), static_cast<uint32_t*>(c.code.p),
+    hip_check(launch_geometry(dV.as<double>(), static_cast<const int32_t*>(dT.p), static_cast<const int32_t*>(dN.p),
+                              static_cast<const int32_t*>(dP.p), c.K, c.Kp, c.geo.as<double>(), c.hmin.as<double>(), static_cast<uint32_t*>(c.code.p),
                               static_cast<int*>(derr.p), c.stream),
               "geometry");
     int err = 0;
@@ -449,6 +450,62 @@ void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc
     c.qtensor = c.physics == DGX_EULER && p <= 3 && !(kv && std::string(kv) == "lane");
 }
 
+// Morton (Z-order) numbering of triangles by centroid: 16-bit coordinates on one common scale
+// interleaved, ties in caller order.  Sets c.perm / c.iperm and returns true, or returns false
+// (caller order kept) for a mesh whose vertex references are invalid (reported later).
+uint32_t spread16(uint32_t x) {
+    x &= 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+bool morton_order(dgx_ctx& c, const dgx_mesh_desc& m) {
+    const int K = m.n_elements;
+    std::vector<double> cx(K), cy(K);
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int e = 0; e < K; ++e) {
+        double sx = 0.0, sy = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const int id = m.elements[e * 3 + k];
+            if (id < 0 || id >= m.n_vertices) return false;
+            sx += m.vertices[2 * id];
+            sy += m.vertices[2 * id + 1];
+        }
+        cx[e] = sx / 3.0;
+        cy[e] = sy / 3.0;
+        x0 = std::min(x0, cx[e]); x1 = std::max(x1, cx[e]);
+        y0 = std::min(y0, cy[e]); y1 = std::max(y1, cy[e]);
+    }
+    const double ext = std::max(x1 - x0, y1 - y0);
+    if (!(ext > 0.0) || !std::isfinite(ext)) return false;
+    const double scale = 65535.0 / ext;
+    std::vector<uint64_t> key(K);
+    for (int e = 0; e < K; ++e) {
+        const uint32_t qx = uint32_t(std::lround((cx[e] - x0) * scale)), qy = uint32_t(std::lround((cy[e] - y0) * scale));
+        key[e] = (uint64_t(spread16(qx) | (spread16(qy) << 1)) << 32) | uint32_t(e);
+    }
+    std::sort(key.begin(), key.end());
+    c.perm.resize(K);
+    c.iperm.resize(K);
+    bool identity = true;
+    for (int e = 0; e < K; ++e) {
+        c.perm[e] = int32_t(key[e] & 0xffffffffu);
+        c.iperm[c.perm[e]] = e;
+        identity &= c.perm[e] == e;
+    }
+    if (identity) {
+        c.perm.clear();
+        c.iperm.clear();
+    }
+    return !identity;
+}
+
+void build_tri_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph,
+                       int precision);
+
 void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph, int precision) {
     if (!m.vertices || !m.elements || !m.face_neighbors || m.n_elements <= 0 || m.n_vertices <= 0)
         throw std::invalid_argument("mesh descriptor is incomplete");
@@ -495,7 +552,33 @@ void build_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, 
         build_quad_context(c, m, s, ph, precision);
         return;
     }
+    // triangles run in Morton order of their centroids (DGX_ELEMENT_ORDER=given keeps the
+    // caller's): the mesh is renumbered here and every per-element array crossing the ABI is
+    // permuted (dgx_ctx::rows_in / rows_out)
+    const char* eo = std::getenv("DGX_ELEMENT_ORDER");
+    std::vector<int32_t> Tp, Np_;
+    dgx_mesh_desc mp = m;
+    if (!(eo && std::string(eo) == "given") && morton_order(c, m)) {
+        const int K = m.n_elements;
+        Tp.resize((size_t)K * 3);
+        Np_.resize((size_t)K * 6);
+        for (int e = 0; e < K; ++e) {
+            const int o = c.perm[e];
+            for (int k = 0; k < 3; ++k) {
+                Tp[e * 3 + k] = m.elements[o * 3 + k];
+                const int en = m.face_neighbors[(o * 3 + k) * 2];
+                Np_[(e * 3 + k) * 2] = (en >= 0 && en < K) ? c.iperm[en] : en;
+                Np_[(e * 3 + k) * 2 + 1] = m.face_neighbors[(o * 3 + k) * 2 + 1];
+            }
+        }
+        mp.elements = Tp.data();
+        mp.face_neighbors = Np_.data();
+    }
+    build_tri_context(c, mp, s, ph, precision);
+}
 
+void build_tri_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc& s, const dgx_physics& ph,
+                       int precision) {
     // ---------------- reference-element tables (FP64) ----------------
     RuleTri vr = dunavant(2 * c.p + c.rule);
     Rule1D fr = gauss_legendre(c.p + 1);
@@ -882,7 +965,9 @@ dgx_status dgx_set_state(dgx_ctx* c, const double* u) {
             for (int v = 0; v < 4; ++v) U[v] += u[(size_t)e * c->np * 4 + v];
         for (int v = 0; v < 4; ++v) U[v] *= c->phi0 / c->K;
         c->set_offset(U);
-        hip_check(hipMemcpyAsync(c->stage.p, u, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+        std::vector<double> tmp;
+        hip_check(hipMemcpyAsync(c->stage.p, c->rows_in(u, (size_t)c->np * 4, tmp), bytes, hipMemcpyHostToDevice, c->stream),
+                  "copy");
         hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u.p, c->K, c->Kp, c->off, c->stream), "pack");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
@@ -895,6 +980,7 @@ dgx_status dgx_get_state(dgx_ctx* c, double* u) {
         hip_check(launch_unpack(c->precision, c->np, c->u.p, c->stage.as<double>(), c->K, c->Kp, c->off, c->stream), "unpack");
         hip_check(hipMemcpyAsync(u, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
+        c->rows_out(u, (size_t)c->np * 4);
     });
 }
 
@@ -923,6 +1009,7 @@ dgx_status dgx_volume_points(dgx_ctx* c, double* xy) {
                 xy[((size_t)e * c->nq + q) * 2 + 1] = n0 * V[1] + n1 * V[3] + n2 * V[5];
             }
         }
+        c->rows_out(xy, (size_t)c->nq * 2);
     });
 }
 
@@ -937,12 +1024,15 @@ dgx_status dgx_get_geometry(dgx_ctx* c, double* geo, double* hmin, uint32_t* cod
             hip_check(hipStreamSynchronize(c->stream), "sync");
             for (size_t e = 0; e < K; ++e)
                 for (int r = 0; r < kGeoRows; ++r) geo[e * kGeoRows + r] = g[r * Kp + e];
+            c->rows_out(geo, (size_t)kGeoRows);
         }
         if (hmin)
             hip_check(hipMemcpyAsync(hmin, c->hmin.p, K * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
         if (codes)
             hip_check(hipMemcpyAsync(codes, c->code.p, K * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
+        c->rows_out(hmin, 1);
+        c->rows_out(codes, 1);
     });
 }
 
@@ -968,7 +1058,10 @@ dgx_status dgx_l2_error(dgx_ctx* c, const double* exact, int32_t exact_is_primit
         ex.alloc(n * sizeof(double));
         const int nb = c->red_blocks();
         part.alloc((size_t)nb * 8 * sizeof(double));
-        hip_check(hipMemcpyAsync(ex.p, exact, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+        std::vector<double> tmp;
+        hip_check(hipMemcpyAsync(ex.p, c->rows_in(exact, (size_t)c->nq * 4, tmp), n * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream),
+                  "copy");
         hip_check(launch_eval_n(c->precision, c->np, c->nq, c->u.p, c->tab64.as<double>(), c->K, c->Kp,
                                 c->stage.as<double>(), c->uref, c->stream),
                   "eval");
@@ -1044,7 +1137,7 @@ dgx_status dgx_write_vtk(dgx_ctx* c, const double* u_or_null, const char* filena
         const size_t npts = (size_t)K * npe;
         std::vector<double> xy(npts * 2), U(npts * 4, 0.0);
         for (int e = 0; e < K; ++e) {
-            const double* V = &c->h_verts[(size_t)e * 2 * c->nf];
+            const double* V = &c->h_verts[(size_t)c->dev_of(e) * 2 * c->nf];  // u is in the caller's order
             for (int q = 0; q < npe; ++q) {
                 const double xi = rp[2 * q], eta = rp[2 * q + 1], n0 = 1.0 - xi - eta;
                 const size_t k = (size_t)e * npe + q;
@@ -1112,7 +1205,9 @@ dgx_status dgx_project(dgx_ctx* c, const double* Uq) {
             for (int v = 0; v < 4; ++v) U[v] += Uq[k * 4 + v];
         for (int v = 0; v < 4; ++v) U[v] /= double(npts);
         c->set_offset(U);
-        hip_check(hipMemcpyAsync(c->stage.p, Uq, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+        std::vector<double> tmp;
+        hip_check(hipMemcpyAsync(c->stage.p, c->rows_in(Uq, (size_t)c->nq * 4, tmp), bytes, hipMemcpyHostToDevice, c->stream),
+                  "copy");
         if (c->nf == 4)
             hip_check(launch_project_quad(c->precision, c->np, c->nq, c->stage.as<double>(), c->u.p,
                                           c->tab64.as<double>(), c->geo.as<double>(), c->minve.as<double>(), c->K,
@@ -1135,6 +1230,7 @@ dgx_status dgx_eval_volume(dgx_ctx* c, double* Uq) {
                   "eval");
         hip_check(hipMemcpyAsync(Uq, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
+        c->rows_out(Uq, (size_t)c->nq * 4);
     });
 }
 
@@ -1143,8 +1239,10 @@ dgx_status dgx_residual(dgx_ctx* c, const double* u, double* r) {
         if (!c || !r) throw std::invalid_argument("null argument");
         size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
         const void* src = c->u.p;
+        std::vector<double> tmp;
         if (u) {
-            hip_check(hipMemcpyAsync(c->stage.p, u, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+            hip_check(hipMemcpyAsync(c->stage.p, c->rows_in(u, (size_t)c->np * 4, tmp), bytes, hipMemcpyHostToDevice, c->stream),
+                      "copy");
             hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), c->u2.p, c->K, c->Kp, c->off, c->stream), "pack");
             src = c->u2.p;
         }
@@ -1152,6 +1250,7 @@ dgx_status dgx_residual(dgx_ctx* c, const double* u, double* r) {
         hip_check(launch_unpack(c->precision, c->np, c->u1.p, c->stage.as<double>(), c->K, c->Kp, dgx_ctx::kNoOff, c->stream), "unpack");
         hip_check(hipMemcpyAsync(r, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
+        c->rows_out(r, (size_t)c->np * 4);
     });
 }
 
@@ -1159,7 +1258,9 @@ dgx_status dgx_apply_minv(dgx_ctx* c, double* r) {
     return guarded([&] {
         if (!c || !r) throw std::invalid_argument("null argument");
         size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
-        hip_check(hipMemcpyAsync(c->stage.p, r, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+        std::vector<double> tmp;
+        hip_check(hipMemcpyAsync(c->stage.p, c->rows_in(r, (size_t)c->np * 4, tmp), bytes, hipMemcpyHostToDevice, c->stream),
+                  "copy");
         if (c->nf == 4)
             hip_check(launch_minv_quad(c->np, c->minve.as<double>(), c->stage.as<double>(), c->K, c->Kp, c->stream), "minv");
         else
@@ -1168,6 +1269,7 @@ dgx_status dgx_apply_minv(dgx_ctx* c, double* r) {
                       "minv");
         hip_check(hipMemcpyAsync(r, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipStreamSynchronize(c->stream), "sync");
+        c->rows_out(r, (size_t)c->np * 4);
     });
 }
 
@@ -1286,16 +1388,20 @@ void adjoint_supported(const dgx_ctx* c) {
 // host [K][Np][4] -> device SoA in `dst`; states carry the storage offset, adjoints do not
 void to_device(dgx_ctx* c, const double* h, void* dst, bool state) {
     size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
-    hip_check(hipMemcpyAsync(c->stage.p, h, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+    std::vector<double> tmp;
+    hip_check(hipMemcpyAsync(c->stage.p, c->rows_in(h, (size_t)c->np * 4, tmp), bytes, hipMemcpyHostToDevice, c->stream),
+              "copy");
     hip_check(launch_pack(c->precision, c->np, c->stage.as<double>(), dst, c->K, c->Kp,
                           state ? c->off : dgx_ctx::kNoOff, c->stream),
               "pack");
+    hip_check(hipStreamSynchronize(c->stream), "sync");  // the host copy may be a temporary
 }
 void to_host(dgx_ctx* c, const void* src, double* h) {
     size_t bytes = (size_t)c->K * c->np * 4 * sizeof(double);
     hip_check(launch_unpack(c->precision, c->np, src, c->stage.as<double>(), c->K, c->Kp, dgx_ctx::kNoOff, c->stream), "unpack");
     hip_check(hipMemcpyAsync(h, c->stage.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
     hip_check(hipStreamSynchronize(c->stream), "sync");
+    c->rows_out(h, (size_t)c->np * 4);
 }
 
 void vjp(dgx_ctx* c, const void* uk, const void* lam, const void* add, const void* acc_in, void* acc_out, void* out,

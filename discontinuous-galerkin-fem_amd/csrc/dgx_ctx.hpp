@@ -1,6 +1,7 @@
 // Private: the context object behind the dgx C ABI, device buffers and error plumbing shared by
 // the host translation units (dgx_api.cpp, dgx_batch_api.cpp).
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <hip/hip_runtime.h>
 
@@ -112,7 +113,28 @@ struct dgx_ctx {
     DevBuf tabNT, rec;    // MFMA Navier-Stokes face-trace tables and face-point records
     DevBuf minve;         // quads: per-element inverse mass matrices [Np*Np][Kp]
     DevBuf ub;            // the state at the start of a checked run (dgx_rk3_run), allocated on first use
-    std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][nf][2]
+    std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][nf][2] in the context's element order
+    // The context numbers the elements in its own order (triangles: Morton order of the
+    // centroids, so that a 64-element tile is a compact patch of the mesh); every per-element
+    // host array crosses the C ABI in the caller's order.  perm[e] = caller index of element e,
+    // iperm its inverse; both empty when the orders agree.
+    std::vector<int32_t> perm, iperm;
+    int dev_of(int ec) const { return iperm.empty() ? ec : iperm[ec]; }
+    // caller-order rows of width w -> context order (h itself when the orders agree)
+    template <typename T>
+    const T* rows_in(const T* h, size_t w, std::vector<T>& tmp) const {
+        if (perm.empty() || !h) return h;
+        tmp.resize((size_t)K * w);
+        for (int e = 0; e < K; ++e) std::copy(h + (size_t)perm[e] * w, h + (size_t)perm[e] * w + w, tmp.data() + (size_t)e * w);
+        return tmp.data();
+    }
+    // context-order rows of width w, in place -> caller order
+    template <typename T>
+    void rows_out(T* h, size_t w) const {
+        if (perm.empty() || !h) return;
+        std::vector<T> tmp(h, h + (size_t)K * w);
+        for (int e = 0; e < K; ++e) std::copy(tmp.data() + (size_t)e * w, tmp.data() + (size_t)e * w + w, h + (size_t)perm[e] * w);
+    }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~dgx_ctx() {

@@ -50,8 +50,12 @@ __device__ __forceinline__ double dist(double ax, double ay, double bx, double b
     return sqrt(dx * dx + dy * dy);
 }
 
+// caller: the caller's index of each element (null when the context keeps the caller's order);
+// the canonical (L) side of an interior face is the element with the smaller caller index, so
+// the geometry and codes do not depend on the context's element order.
 __global__ __launch_bounds__(256) void geometry_kernel(const double* __restrict__ V, const int32_t* __restrict__ T,
-                                                       const int32_t* __restrict__ N, int K, int Kp,
+                                                       const int32_t* __restrict__ N, const int32_t* __restrict__ caller,
+                                                       int K, int Kp,
                                                        double* __restrict__ geo, double* __restrict__ hmin,
                                                        uint32_t* __restrict__ code, int* __restrict__ err) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,7 +92,8 @@ __global__ __launch_bounds__(256) void geometry_kernel(const double* __restrict_
         if (en < 0) {
             cf = kFaceBoundary | (1u << 3);
         } else {
-            const bool isL = (e < en) || (e == en && f < fn);
+            const int ce = caller ? caller[e] : e, cn = caller ? caller[en] : en;
+            const bool isL = (ce < cn) || (ce == cn && f < fn);
             const Tri tn = load_tri(V, T, en);
             const Tri& tL = isL ? t : tn;
             const Tri& tR = isL ? tn : t;
@@ -128,10 +133,10 @@ __global__ __launch_bounds__(256) void geometry_kernel(const double* __restrict_
 
 }  // namespace
 
-hipError_t launch_geometry(const double* V, const int32_t* T, const int32_t* N, int K, int Kp, double* geo,
-                           double* hmin, uint32_t* code, int* err, hipStream_t s) {
+hipError_t launch_geometry(const double* V, const int32_t* T, const int32_t* N, const int32_t* caller, int K, int Kp,
+                           double* geo, double* hmin, uint32_t* code, int* err, hipStream_t s) {
     const int blocks = (Kp + 255) / 256;
-    hipLaunchKernelGGL(geometry_kernel, dim3(blocks), dim3(256), 0, s, V, T, N, K, Kp, geo, hmin, code, err);
+    hipLaunchKernelGGL(geometry_kernel, dim3(blocks), dim3(256), 0, s, V, T, N, caller, K, Kp, geo, hmin, code, err);
     return hipGetLastError();
 }
 

@@ -165,8 +165,8 @@ hipError_t launch_reduce(int kind, int precision, int p, int rule, const void* u
                          const double* geo, const double* hmin, const double* qw, int K, int Kp,
                          double gamma, Off4 uref, Off4 off, double* partial, int nblocks, hipStream_t s);
 // Relative-L2 error partial sums (dgx_aux.hip): 8 doubles per block of 256 elements.
-hipError_t launch_geometry(const double* V, const int32_t* T, const int32_t* N, int K, int Kp, double* geo,
-                           double* hmin, uint32_t* code, int* err, hipStream_t s);
+hipError_t launch_geometry(const double* V, const int32_t* T, const int32_t* N, const int32_t* caller, int K, int Kp,
+                           double* geo, double* hmin, uint32_t* code, int* err, hipStream_t s);
 // per_point: the weights are the quad geometry rows w_q |J_q| instead of qw[q] |J|.
 hipError_t launch_l2_error(const double* Uq, const double* Ex, int ex_prim, const double* geo, const double* qw, int nq,
                            int K, int Kp, double gamma, double* partial, int nblocks, int per_point, hipStream_t s);
