@@ -450,6 +450,77 @@ void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc
     c.qtensor = c.physics == DGX_EULER && p <= 3 && !(kv && std::string(kv) == "lane");
 }
 
+// Arrays of the matrix-core Euler tile kernel: the FP32 geometry (both modes, formed on the
+// device from the FP64 geometry), and per 64-element tile the face connections and the list of
+// external faces (neighbour outside the tile first, boundary faces after).  nbr [3][Kp] holds
+// the neighbour element of each face or a boundary face's running index; the face codes
+// (neighbour's local face, reversed order, canonical side) come from the device geometry.
+void build_tile_arrays(dgx_ctx& c, const std::vector<int32_t>& nbr) {
+    const int K = c.K, Kp = c.Kp, tiles = (K + 63) / 64, nfq = c.nfq;
+    const int nrec = 3 * nfq * 64;  // records of a tile's own face points
+    c.geo32L.alloc((size_t)kGeo32Rows * Kp * sizeof(float));
+    c.geo32R.alloc((size_t)kGeo32Rows * Kp * sizeof(float));
+    hip_check(launch_geometry32(c.geo.as<double>(), K, Kp, 0, c.geo32L.as<float>(), c.stream), "geometry32");
+    hip_check(launch_geometry32(c.geo.as<double>(), K, Kp, 1, c.geo32R.as<float>(), c.stream), "geometry32");
+    std::vector<uint32_t> code(Kp);
+    hip_check(hipMemcpyAsync(code.data(), c.code.p, Kp * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream), "copy");
+    hip_check(hipStreamSynchronize(c.stream), "sync");
+    std::vector<uint32_t> conn((size_t)2 * Kp, 0u);
+    std::vector<int32_t> ext, off(tiles + 1), nint(tiles);
+    int cap = 0;
+    for (int t = 0; t < tiles; ++t) {
+        const int lo = 64 * t, hi = std::min(K, lo + 64);
+        off[t] = int(ext.size() / 2);
+        int slots = 0;
+        for (int pass = 0; pass < 2; ++pass) {  // faces with a neighbour outside the tile, then boundary faces
+            for (int e = lo; e < hi; ++e)
+                for (int f = 0; f < 3; ++f) {
+                    const uint32_t cf = (code[e] >> (4 * f)) & 15u;
+                    const uint32_t fn = cf & 3u, flip = (cf >> 2) & 1u, isL = (cf >> 3) & 1u;
+                    const int en = nbr[(size_t)f * Kp + e];
+                    const bool bnd = fn == kFaceBoundary;
+                    const bool inside = !bnd && en >= lo && en < hi;
+                    uint32_t base, step;
+                    if (inside) {
+                        if (pass) continue;
+                        // the neighbour's own record on its face fn, its points reversed when flagged
+                        base = (fn * nfq + (flip ? nfq - 1 : 0)) * 64 + uint32_t(en - lo);
+                        step = flip ? 1u : 0u;
+                    } else {
+                        if (bnd != (pass == 1)) continue;
+                        const int slot = slots++;
+                        base = bnd ? uint32_t(slot) : uint32_t(nrec + slot * nfq);
+                        step = 2u;
+                        ext.push_back(en);
+                        ext.push_back(int32_t(fn | (flip << 2) | (uint32_t(f) << 3) | (uint32_t(e - lo) << 5)));
+                    }
+                    if (base > 2047u) throw std::runtime_error("tile record index exceeds 11 bits");
+                    const uint32_t field = base | (step << 11) | (isL << 13) | (bnd ? kConnBoundary : 0u);
+                    conn[2 * (size_t)e + (f >> 1)] |= field << (16 * (f & 1));
+                }
+            if (pass == 0) nint[t] = slots;
+        }
+        cap = std::max(cap, nint[t]);
+    }
+    off[tiles] = int(ext.size() / 2);
+    // fixed-stride copy of each tile's first 64 external faces with a neighbour, so a kernel can
+    // address them without first reading the tile's list offset
+    std::vector<int32_t> halo((size_t)tiles * 128, 0);
+    for (int t = 0; t < tiles; ++t)
+        for (int k = 0; k < 64; ++k) {
+            const bool has = k < nint[t];
+            halo[((size_t)t * 64 + k) * 2] = has ? ext[(size_t)(off[t] + k) * 2] : -1;
+            halo[((size_t)t * 64 + k) * 2 + 1] = has ? ext[(size_t)(off[t] + k) * 2 + 1] : 0;
+        }
+    if (ext.empty()) ext.assign(2, 0);
+    c.upload(c.halo, halo.data(), halo.size() * sizeof(int32_t));
+    c.upload(c.conn, conn.data(), conn.size() * sizeof(uint32_t));
+    c.upload(c.ext, ext.data(), ext.size() * sizeof(int32_t));
+    c.upload(c.ext_off, off.data(), off.size() * sizeof(int32_t));
+    c.upload(c.ext_nint, nint.data(), nint.size() * sizeof(int32_t));
+    c.halo_cap = cap;
+}
+
 // Morton (Z-order) numbering of triangles by centroid: 16-bit coordinates on one common scale
 // interleaved, ties in caller order.  Sets c.perm / c.iperm and returns true, or returns false
 // (caller order kept) for a mesh whose vertex references are invalid (reported later).
@@ -767,6 +838,7 @@ void build_tri_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc&
         c.upload(c.tabVL, m.data(), m.size() * sizeof(float));
         build_mfma_vjp_tables(c.p, tR, m);
         c.upload(c.tabVR, m.data(), m.size() * sizeof(float));
+        build_tile_arrays(c, nbr);
     }
     if (precision == DGX_FP32 && c.physics == DGX_NAVIER_STOKES && c.rule == 0 && mfma_order(c.p) && !lane_only &&
         fits32) {

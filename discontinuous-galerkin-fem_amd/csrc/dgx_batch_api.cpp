@@ -67,6 +67,8 @@ struct dgx_batch {
         sp.mode = MODE_RK;
         sp.ns_face_quirk = 0;
         for (int v = 0; v < 4; ++v) sp.uref[v] = c->uref.v[v];
+        sp.tabm = c->tabML.p;  // the matrix-core tile kernel's lane tables (null without them)
+        c->set_tile_params(sp, MODE_RK);
         return sp;
     }
 
@@ -115,12 +117,19 @@ struct dgx_batch {
     }
 
     int grid() const { return std::max(1, cus * std::max(1, tune.blocks_per_cu)); }
+    // the largest external-face trace count of a tile among the order-p instances
+    int halo_cap(int p) const {
+        int cap = 0;
+        for (auto* c : inst)
+            if (c->p == p) cap = std::max(cap, c->halo_cap);
+        return cap;
+    }
 
     void launch_window_stage(int w) {
         const StageParams* pr = static_cast<const StageParams*>(params[w].p);
         for (int p = 4; p >= 1; --p)
             if (tabm[p - 1] && n_p[p - 1])
-                hip_check(launch_batch_stage_mfma(p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1], tabm[p - 1], grid(),
+                hip_check(launch_batch_stage_mfma(p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1], halo_cap(p), grid(),
                                                   stream),
                           "batch stage (matrix cores)");
         if (tune.layout == 0) {

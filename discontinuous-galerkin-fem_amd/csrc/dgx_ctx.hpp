@@ -113,6 +113,10 @@ struct dgx_ctx {
     DevBuf tabNT, rec;    // MFMA Navier-Stokes face-trace tables and face-point records
     DevBuf minve;         // quads: per-element inverse mass matrices [Np*Np][Kp]
     DevBuf ub;            // the state at the start of a checked run (dgx_rk3_run), allocated on first use
+    // matrix-core Euler tile kernel: FP32 geometry (RK/RHS and M dU/dt variants), face
+    // connections, external-face lists and the largest per-tile count of interior external faces
+    DevBuf geo32L, geo32R, conn, ext, ext_off, ext_nint, halo;
+    int halo_cap = 0;
     std::vector<double> h_minv, h_qw, h_verts;  // h_verts [K][nf][2] in the context's element order
     // The context numbers the elements in its own order (triangles: Morton order of the
     // centroids, so that a 64-element tile is a compact patch of the mesh); every per-element
@@ -139,7 +143,8 @@ struct dgx_ctx {
 
     ~dgx_ctx() {
         for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage,
-                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec, &minve, &ub})
+                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec, &minve, &ub, &geo32L, &geo32R, &conn,
+                          &ext, &ext_off, &ext_nint, &halo})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -201,12 +206,24 @@ struct dgx_ctx {
             sp.uref[v] = uref.v[v];
             sp.uoff[v] = off.v[v];
         }
+        set_tile_params(sp, mode);
         if (nf == 4 && qtensor)
             hip_check(launch_stage_quad_tensor(precision, p, sp, stream), "stage kernel launch");
         else if (nf == 4)
             hip_check(launch_stage_quad(precision, physics, p, sp.ns_face_quirk != 0, sp, stream), "stage kernel launch");
         else
             hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
+    }
+
+    // the matrix-core Euler tile kernel's arrays (null when the context has none)
+    void set_tile_params(StageParams& sp, int mode) const {
+        sp.geo32 = (mode == MODE_RESIDUAL ? geo32R : geo32L).as<float>();
+        sp.conn = conn.as<uint32_t>();
+        sp.ext = ext.as<int32_t>();
+        sp.ext_off = ext_off.as<int32_t>();
+        sp.ext_nint = ext_nint.as<int32_t>();
+        sp.halo = halo.as<int32_t>();
+        sp.halo_cap = halo_cap;
     }
 
     // One SSP-RK3 step (SSP_RK::step_rk3, I/solver/time_stepping.hpp:39-55):

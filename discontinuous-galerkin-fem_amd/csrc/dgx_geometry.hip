@@ -131,7 +131,38 @@ __global__ __launch_bounds__(256) void geometry_kernel(const double* __restrict_
     if (bad) atomicOr(err, bad);
 }
 
+// The FP32 operands the matrix-core Euler tile kernel consumes, formed in FP64 from the FP64
+// geometry exactly as the per-element kernels form them (dxi_dx scaled by sv = |det J| for M dU/dt
+// or 1, the lift scale len / |det J| with 1 / |det J| in FP64) and rounded once.
+__global__ __launch_bounds__(256) void geometry32_kernel(const double* __restrict__ geo, int K, int Kp, int residual,
+                                                         float* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Kp) return;
+    const size_t sK = (size_t)Kp;
+    if (e >= K) {
+        for (int r = 0; r < kGeo32Rows; ++r) out[r * sK + e] = 0.0f;
+        return;
+    }
+    const double det = geo[G_DET * sK + e];
+    const double rdet = 1.0 / det;
+    const double sv = residual ? det : 1.0;
+    out[G32_A00 * sK + e] = float(sv * geo[G_A00 * sK + e]);
+    out[G32_A01 * sK + e] = float(sv * geo[G_A01 * sK + e]);
+    out[G32_A10 * sK + e] = float(sv * geo[G_A10 * sK + e]);
+    out[G32_A11 * sK + e] = float(sv * geo[G_A11 * sK + e]);
+    for (int k = 0; k < 6; ++k) out[(G32_N0X + k) * sK + e] = float(geo[(G_N0X + k) * sK + e]);
+    for (int f = 0; f < 3; ++f) {
+        const double len = geo[(G_L0 + f) * sK + e];
+        out[(G32_S0 + f) * sK + e] = residual ? float(len) : float(len * rdet);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_geometry32(const double* geo, int K, int Kp, int residual, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(geometry32_kernel, dim3((Kp + 255) / 256), dim3(256), 0, s, geo, K, Kp, residual, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_geometry(const double* V, const int32_t* T, const int32_t* N, const int32_t* caller, int K, int Kp,
                            double* geo, double* hmin, uint32_t* code, int* err, hipStream_t s) {

@@ -107,6 +107,31 @@ struct QuadDims {
 __host__ __device__ inline uint32_t quad_code(uint32_t code, int f) { return (code >> (5 * f)) & 31u; }
 constexpr uint32_t kQuadAffineBit = 1u << 20;
 
+// FP32 per-element geometry of the matrix-core Euler tile kernel, rows [kGeo32Rows][Kp]: the
+// FP32 roundings of the FP64 geometry as the kernel consumes them, formed in FP64 on the device
+// (dgx_geometry.hip): dxi_dx (times |det J| for M dU/dt), the three outward normals, and the
+// three lift scales len_f / |det J| (len_f for M dU/dt).
+enum Geo32Row : int { G32_A00 = 0, G32_A01, G32_A10, G32_A11, G32_N0X, G32_N0Y, G32_N1X, G32_N1Y, G32_N2X, G32_N2Y,
+                      G32_S0, G32_S1, G32_S2, kGeo32Rows };
+
+// Face connections of the tile kernel, per element two 32-bit words, 16 bits per face f (word
+// f >> 1, shifted by 16 (f & 1)), addressing the tile's trace records: the records of the
+// tile's own face points at index (face Nfq + point) 64 + element, those of the external faces'
+// neighbour points after them at 3 Nfq 64 + slot Nfq + point (in the reading element's order).
+//   bits 0-10   index of the neighbour's record at point 0 of the face (boundary face: its slot
+//               in the tile's external-face list)
+//   bits 11-12  step of that index per point: 0 = +64, 1 = -64 (reversed order), 2 = +1
+//   bit 13      this element holds the canonical (L) orientation
+//   bit 14      boundary face
+// External faces of tile t are entries ext_off[t] .. ext_off[t+1]-1 of a list of int2 (x: the
+// neighbour element, or the boundary face's running index; y: neighbour's local face | reversed
+// order << 2 | the reading element's local face << 3 | its index in the tile << 5), its
+// ext_nint[t] faces with a neighbour first.
+constexpr uint32_t kConnBoundary = 1u << 14;
+__host__ __device__ constexpr uint32_t conn_field(uint32_t w0, uint32_t w1, int f) {
+    return ((f < 2 ? w0 : w1) >> (16 * (f & 1))) & 0xffffu;
+}
+
 // Launch parameters of one fused stage: out = ca*un + cb*(uk + dt*L(uk)) (mode RK),
 // out = L(uk) (mode RHS) or out = M dU/dt (mode RESIDUAL).
 enum StageMode : int { MODE_RK = 0, MODE_RHS = 1, MODE_RESIDUAL = 2 };
@@ -142,6 +167,17 @@ struct StageParams {
                      // (Pr (gamma - 1)) as the reference, its negative for conduction
     double uref[4];  // point-value storage offset (see Off4)
     double uoff[4];  // the same as a constant-mode coefficient (uref / phi_0)
+    // matrix-core Euler tile kernel (null elsewhere): FP32 geometry of this mode, face
+    // connections, external-face lists, and the largest count of interior external faces of a
+    // tile (sizes the dynamic LDS of their traces)
+    const float* geo32;
+    const uint32_t* conn;
+    const int32_t* ext;       // int2 entries
+    const int32_t* ext_off;   // [tiles + 1]
+    const int32_t* ext_nint;  // [tiles]
+    const int32_t* halo;      // int2 [tiles][64]: the first 64 external faces with a neighbour of each
+                              // tile (the ext entries, x = -1 past the tile's count)
+    int halo_cap;
 };
 
 // Kernel launchers (dgx_kernels.hip / dgx_kernels_ns.hip).  Each returns hipSuccess or the
@@ -222,7 +258,10 @@ hipError_t launch_batch_stage_order(int precision, int p, const StageParams* par
 // Batched FP32 Euler items of order p = 3 or 4 on the matrix cores (dgx_stage_mfma.hip): block
 // work items of whole 64-element tiles, `tab` the order's lane tables.
 hipError_t launch_batch_stage_mfma(int p, const StageParams* params, const BatchItem* items, int n_items,
-                                   const void* tab, int grid, hipStream_t s);
+                                   int halo_cap, int grid, hipStream_t s);
+// FP32 tile-kernel geometry (Geo32Row) from the FP64 geometry, for the RK/RHS tables (residual 0)
+// or for M dU/dt (residual 1)
+hipError_t launch_geometry32(const double* geo, int K, int Kp, int residual, float* out, hipStream_t s);
 // gradient of the kinetic-energy functional with respect to the coefficients (device layout)
 hipError_t launch_ke_grad(int precision, int p, int rule, const void* u, const double* tab64, const double* qw,
                           const double* geo, void* grad, int K, int Kp, Off4 uref, hipStream_t s);

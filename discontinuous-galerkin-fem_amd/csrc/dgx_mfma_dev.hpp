@@ -72,6 +72,22 @@ __device__ __forceinline__ void rusanov2(const float* A, const float* B, float n
     H[3] = (fa3 + fb3) - s * (B[3] - A[3]);
 }
 
+// Trace record of a face point for the Rusanov flux, formed once by the element that owns the
+// point: the normal flux F(U).n along the owner's outward normal and the wave speed |u.n| + c,
+// with the arithmetic of rusanov2 for one side.  The normals of the two sides of a face are
+// exact negatives, so a neighbour's record gives F(U_nb).n = -Fn_nb.
+__device__ __forceinline__ void trace_record(const float* U, float nx, float ny, float g, float* Fn, float& s) {
+    const float ri = rcp(U[0]);
+    const float u = U[1] * ri, v = U[2] * ri;
+    const float p = (g - 1.0f) * (U[3] - 0.5f * U[0] * (u * u + v * v));
+    const float un = u * nx + v * ny;
+    s = fabsf(un) + dsqrt(g * p * ri);
+    Fn[0] = U[0] * un;
+    Fn[1] = U[1] * un + p * nx;
+    Fn[2] = U[2] * un + p * ny;
+    Fn[3] = (U[3] + p) * un;
+}
+
 // Flux of one face point in the canonical (L) orientation, scaled for this element's lift;
 // bcr is the point's boundary-data row on a boundary face.
 __device__ __forceinline__ void face_point_flux(const float* Uo, float* Un, uint32_t c, const double* bcr, float nx,
