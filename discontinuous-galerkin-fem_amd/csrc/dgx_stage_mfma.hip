@@ -20,8 +20,8 @@
 // products leave in each lane.  Every element leaves its own traces in LDS, so a face between
 // two elements of the tile reads the neighbour's trace there (both sides then see the same two
 // matrix-core traces, and both copies of the face flux agree bit for bit).  Only the faces that
-// leave the tile need the neighbour's trace evaluated from its coefficients: one wave per tile
-// (rotating over the tiles) does that for the tile's list of external faces, one face per lane.
+// leave the tile need the neighbour's trace evaluated from its coefficients: every wave does
+// that for the tile's list of external faces, one face per lane and its own points of the face.
 #include "dgx_mfma_dev.hpp"
 
 namespace dgx {
@@ -77,8 +77,6 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     const int wu = __builtin_amdgcn_readfirstlane(w);
     const int xs = sp.ext_off[tile];     // the tile's external faces
     const int nint = sp.ext_nint[tile];  // ... of which the first nint have a neighbour
-    // external faces with a neighbour: wave (tile + c) mod 4 takes entries 64 c .. 64 c + 63
-    const int hk0 = 64 * ((wu - tile) & 3);
     constexpr int NREC = MfmaLds<P, BF16>::NREC;
     const int nrs = NREC + (sp.halo_cap > 1 ? sp.halo_cap : 1) * NFQ;  // records per array
     const TraceRecs H{dyn, dyn + nrs, reinterpret_cast<float*>(dyn + 2 * nrs)};
@@ -127,15 +125,15 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
         const int t = int(threadIdx.x) + k * kBlock;
         tabv[k] = t < NTAB ? tm[t] : 0.0f;
     }
-    // neighbour coefficients of this lane's first external face (all modes of one neighbour);
-    // the tile's first 64 such faces come from the fixed-stride copy, one memory step from here
-    const bool hwave = hk0 < nint;  // wave-uniform
-    int hk = hk0 + lane;
+    // External faces with a neighbour: lane k of every wave takes face k of the tile (64 at a
+    // time), each wave its own points of the face (point w, w + 4), so the four waves share the
+    // work evenly.  Neighbour coefficients of this lane's first face, all modes of one neighbour
+    // (the first 64 faces come from the fixed-stride copy, one memory step from here).
+    int hk = lane;
     uint32_t hcode = 0;
     float cn[NP][4];
-    if (hk0 == 0 || hwave) {
-        const int2 x = hk0 == 0 ? reinterpret_cast<const int2*>(sp.halo)[tile * 64 + lane]
-                                : (hk < nint ? reinterpret_cast<const int2*>(sp.ext)[xs + hk] : int2{-1, 0});
+    {
+        const int2 x = reinterpret_cast<const int2*>(sp.halo)[tile * 64 + lane];
         hcode = uint32_t(x.y);
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
@@ -157,51 +155,45 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     // keep the scheduler from sinking the loads above into the arithmetic below
     __builtin_amdgcn_sched_barrier(0);
 
-    // ---------------- traces of the external faces (the tile's halo waves) --------------------
-    if (hwave) {
-        for (;;) {
-            if (hk < nint) {
-                const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
-                const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
-                float Un[NFQ][4];
+    // ---------------- traces of the external faces ----------------------------------------
+    for (;;) {
+        if (hk < nint) {
+            const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
+            const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
+            // records along the neighbour's outward normal, the negated normal of the reading
+            // element's face
+            const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
+            const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
 #pragma unroll
-                for (int q = 0; q < NFQ; ++q)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) Un[q][v] = ur[v];
-#pragma unroll
-                for (int i = 0; i < NP; ++i)
-#pragma unroll
-                    for (int q = 0; q < NFQ; ++q) {
-                        const float t = T[i * NFQ + q];
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) Un[q][v] = fma(t, cn[i][v], Un[q][v]);
-                    }
-                // records along the neighbour's outward normal, the negated normal of the reading
-                // element's face
-                const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
-                const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
-#pragma unroll
-                for (int q = 0; q < NFQ; ++q) {
-                    float Fn[4], sw;
-                    trace_record(Un[q], hnx, hny, gam, Fn, sw);
-                    const int k = NREC + hk * NFQ + q;
-                    H.U[k] = make_float4(Un[q][0], Un[q][1], Un[q][2], Un[q][3]);
-                    H.F[k] = make_float4(Fn[0], Fn[1], Fn[2], Fn[3]);
-                    H.S[k] = sw;
-                }
-            }
-            // more than 64 such faces for this wave (tiles that are not compact patches)
-            hk += 4 * 64;
-            if (__builtin_amdgcn_readfirstlane(hk - lane) >= nint) break;
-            if (hk < nint) {
-                const int2 x = reinterpret_cast<const int2*>(sp.ext)[xs + hk];
-                hcode = uint32_t(x.y);
+            for (int q0 = 0; q0 < NFQ; q0 += 4) {
+                const int q = q0 + wu;
+                if (q >= NFQ) continue;
+                float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
-                    const v4u y = load_x4(r_uk, (unsigned(i) * uKp + unsigned(x.x)) * 16u);
+                    const float t = T[i * NFQ + q];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
+                    for (int v = 0; v < 4; ++v) Un[v] = fma(t, cn[i][v], Un[v]);
                 }
+                float Fn[4], sw;
+                trace_record(Un, hnx, hny, gam, Fn, sw);
+                const int k = NREC + hk * NFQ + q;
+                H.U[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
+                H.F[k] = make_float4(Fn[0], Fn[1], Fn[2], Fn[3]);
+                H.S[k] = sw;
+            }
+        }
+        // more than 64 such faces (tiles that are not compact patches): the next 64
+        hk += 64;
+        if (hk - lane >= nint) break;
+        if (hk < nint) {
+            const int2 x = reinterpret_cast<const int2*>(sp.ext)[xs + hk];
+            hcode = uint32_t(x.y);
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const v4u y = load_x4(r_uk, (unsigned(i) * uKp + unsigned(x.x)) * 16u);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
             }
         }
     }
