@@ -497,7 +497,8 @@ def bench_batch(args):
 
 def calibrate_copy(mib):
     """A plain device-to-device copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
-    scales the counters of the stage kernel."""
+    scales the counters of the stage kernel, and its rate (HIP events over 20 copies after 3
+    warm-up copies; bytes read + written) is the achievable HBM bandwidth of a streaming kernel."""
     import torch
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -505,7 +506,17 @@ def calibrate_copy(mib):
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    gbs = 2.0 * n * 4 / (ms * 1e-3) / 1e9
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3 + reps,
+                      "ms_per_copy": ms, "gb_per_s": gbs, "frac_of_8tbs": gbs / HBM_PEAK_GBS}))
 
 
 def vortex_ic_exact(xy, t):
