@@ -360,9 +360,8 @@ def bench_ns(args):
     stage_ms, _ = ctx.rk3_profile(dt, 20)
     K, Np = ctx.K, ctx.Np
     bytes_launch = ctx.info.bytes_per_stage_alg
-    # one RK stage = the face-trace launch + the stage launch on the matrix cores
-    kname = ("ns_trace_quirk_kernel_p%d+ns_stage_mfma_kernel_p%d" % (p, p) if ctx.info.stage_kernel == 1
-             else "stage_kernel<%d,float,NS>" % p)
+    # one RK stage = one launch of the matrix-core tile kernel
+    kname = ("ns_tile_quirk_kernel_p%d" % p if ctx.info.stage_kernel == 1 else "stage_kernel<%d,float,NS>" % p)
     _line("NS RHS DOF-stage updates/s (FP32)", 12.0 * Np * K * steps / secs, "DOF-stage updates/s", steps,
           args.warmup, secs / steps * 1e3, "f32",
           {"workload": f"ns_taylor_green_n{n}_p{p}", "elements": K, "order": p, "mu": 1 / 1600, "dt": dt},

@@ -450,16 +450,16 @@ void build_quad_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc
     c.qtensor = c.physics == DGX_EULER && p <= 3 && !(kv && std::string(kv) == "lane");
 }
 
-// Arrays of the matrix-core Euler tile kernel: the FP32 geometry (both modes, formed on the
-// device from the FP64 geometry), and per 64-element tile the face connections and the list of
+// Arrays of the matrix-core tile kernels: the FP32 geometry (both modes, formed on the device
+// from the FP64 geometry), and per 64-element tile the face connections and the list of
 // external faces (neighbour outside the tile first, boundary faces after).  nbr [3][Kp] holds
 // the neighbour element of each face or a boundary face's running index; the face codes
 // (neighbour's local face, reversed order, canonical side) come from the device geometry.
 void build_tile_arrays(dgx_ctx& c, const std::vector<int32_t>& nbr) {
     const int K = c.K, Kp = c.Kp, tiles = (K + 63) / 64, nfq = c.nfq;
     const int nrec = 3 * nfq * 64;  // records of a tile's own face points
-    c.geo32L.alloc((size_t)kGeo32Rows * Kp * sizeof(float));
-    c.geo32R.alloc((size_t)kGeo32Rows * Kp * sizeof(float));
+    c.geo32L.alloc((size_t)kGeo32NsRows * Kp * sizeof(float));
+    c.geo32R.alloc((size_t)kGeo32NsRows * Kp * sizeof(float));
     hip_check(launch_geometry32(c.geo.as<double>(), K, Kp, 0, c.geo32L.as<float>(), c.stream), "geometry32");
     hip_check(launch_geometry32(c.geo.as<double>(), K, Kp, 1, c.geo32R.as<float>(), c.stream), "geometry32");
     std::vector<uint32_t> code(Kp);
@@ -492,7 +492,8 @@ void build_tile_arrays(dgx_ctx& c, const std::vector<int32_t>& nbr) {
                         base = bnd ? uint32_t(slot) : uint32_t(nrec + slot * nfq);
                         step = 2u;
                         ext.push_back(en);
-                        ext.push_back(int32_t(fn | (flip << 2) | (uint32_t(f) << 3) | (uint32_t(e - lo) << 5)));
+                        ext.push_back(
+                            int32_t(fn | (flip << 2) | (uint32_t(f) << 3) | (uint32_t(e - lo) << 5) | (isL << 11)));
                     }
                     if (base > 2047u) throw std::runtime_error("tile record index exceeds 11 bits");
                     const uint32_t field = base | (step << 11) | (isL << 13) | (bnd ? kConnBoundary : 0u);
@@ -822,8 +823,8 @@ void build_tri_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc&
     const char* kv = std::getenv("DGX_STAGE_KERNEL");
     const bool lane_only = kv && std::string(kv) == "lane" && precision == DGX_FP32;
     // the matrix-core kernels address every buffer through 32-bit byte offsets: the largest one
-    // (state, geometry, or the NS face-point records) must stay below 2 GiB
-    const double big = std::max({16.0 * c.np, 8.0 * kGeoRows, c.physics == DGX_NAVIER_STOKES ? 96.0 * c.nfq : 0.0}) * c.Kp;
+    // (state or geometry) must stay below 2 GiB
+    const double big = std::max({16.0 * c.np, 8.0 * kGeoRows, 4.0 * kGeo32NsRows}) * c.Kp;
     const bool fits32 = big < 2147483648.0;
     if (precision == DGX_FP32_BF16VOL && !fits32)
         throw Fail(DGX_E_UNSUPPORTED, "mesh too large for the BF16-input volume variant (2 GiB buffers)");
@@ -843,13 +844,14 @@ void build_tri_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc&
     if (precision == DGX_FP32 && c.physics == DGX_NAVIER_STOKES && c.rule == 0 && mfma_order(c.p) && !lane_only &&
         fits32) {
         std::vector<float> m;
-        build_mfma_ns_tables(c.p, tL, false, m);
+        const bool quirk = (c.quirks & DGX_QUIRK_NS_FACE_GRAD_REFERENCE) != 0;
+        build_mfma_ns_tables(c.p, tL, false, quirk, m);
         c.upload(c.tabML, m.data(), m.size() * sizeof(float));
-        build_mfma_ns_tables(c.p, tR, false, m);
+        build_mfma_ns_tables(c.p, tR, false, quirk, m);
         c.upload(c.tabMR, m.data(), m.size() * sizeof(float));
-        build_mfma_ns_tables(c.p, tL, true, m);
+        build_mfma_ns_tables(c.p, tL, true, quirk, m);
         c.upload(c.tabNT, m.data(), m.size() * sizeof(float));
-        c.rec.alloc((size_t)3 * c.nfq * c.Kp * 32);
+        build_tile_arrays(c, nbr);
     }
     c.u.alloc(c.state_bytes());
     c.u1.alloc(c.state_bytes());

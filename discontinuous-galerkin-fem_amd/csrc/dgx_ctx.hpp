@@ -110,7 +110,7 @@ struct dgx_ctx {
     DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
     DevBuf tabML, tabMR;  // MFMA stage kernel lane tables (RK/RHS and residual variants)
     DevBuf tabVL, tabVR;  // MFMA stage adjoint lane tables (same two variants)
-    DevBuf tabNT, rec;    // MFMA Navier-Stokes face-trace tables and face-point records
+    DevBuf tabNT;         // MFMA Navier-Stokes face-trace and external-face tables
     DevBuf minve;         // quads: per-element inverse mass matrices [Np*Np][Kp]
     DevBuf ub;            // the state at the start of a checked run (dgx_rk3_run), allocated on first use
     // matrix-core Euler tile kernel: FP32 geometry (RK/RHS and M dU/dt variants), face
@@ -143,7 +143,7 @@ struct dgx_ctx {
 
     ~dgx_ctx() {
         for (DevBuf* b : {&u, &u1, &u2, &geo, &nbr, &code, &hmin, &bc, &tabL, &tabR, &tab64, &minv, &pm, &qw, &stage,
-                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &rec, &minve, &ub, &geo32L, &geo32R, &conn,
+                          &partial, &tabML, &tabMR, &tabVL, &tabVR, &tabNT, &minve, &ub, &geo32L, &geo32R, &conn,
                           &ext, &ext_off, &ext_nint, &halo})
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
@@ -185,7 +185,6 @@ struct dgx_ctx {
         sp.tab = (mode == MODE_RESIDUAL) ? tabR.p : tabL.p;
         sp.tabm = (mode == MODE_RESIDUAL) ? tabMR.p : tabML.p;
         sp.tabt = tabNT.p;
-        sp.rec = rec.p;
         sp.bc = bc.as<double>();
         sp.minve = minve.as<double>();
         sp.tab64 = tab64.as<double>();
@@ -215,8 +214,20 @@ struct dgx_ctx {
             hip_check(launch_stage(precision, physics, p, rule, sp, stream), "stage kernel launch");
     }
 
-    // the matrix-core Euler tile kernel's arrays (null when the context has none)
+    // the matrix-core tile kernels' arrays (null when the context has none)
     void set_tile_params(StageParams& sp, int mode) const {
+        sp.f.gamma = float(sp.gamma);
+        sp.f.mu = float(sp.mu);
+        sp.f.lam = (-2.0f * sp.f.mu) / 3.0f;  // as viscous_lambda() forms it on the device
+        sp.f.kappa = float(sp.kappa);
+        sp.f.sig0 = float(sp.sigma0 * sp.mu * (p + 1) * (p + 1));
+        sp.f.dt = float(sp.dt);
+        sp.f.ca = float(sp.ca);
+        sp.f.cb = float(sp.cb);
+        for (int v = 0; v < 4; ++v) {
+            sp.f.uref[v] = float(sp.uref[v]);
+            sp.f.uoff[v] = float(sp.uoff[v]);
+        }
         sp.geo32 = (mode == MODE_RESIDUAL ? geo32R : geo32L).as<float>();
         sp.conn = conn.as<uint32_t>();
         sp.ext = ext.as<int32_t>();

@@ -131,16 +131,17 @@ __global__ __launch_bounds__(256) void geometry_kernel(const double* __restrict_
     if (bad) atomicOr(err, bad);
 }
 
-// The FP32 operands the matrix-core Euler tile kernel consumes, formed in FP64 from the FP64
-// geometry exactly as the per-element kernels form them (dxi_dx scaled by sv = |det J| for M dU/dt
-// or 1, the lift scale len / |det J| with 1 / |det J| in FP64) and rounded once.
+// The FP32 operands the matrix-core tile kernels consume, formed in FP64 from the FP64 geometry
+// exactly as the per-element kernels form them (dxi_dx scaled by sv = |det J| for M dU/dt or 1,
+// the lift scale len / |det J| with 1 / |det J| in FP64; dxi_dx and the face lengths as they
+// are) and rounded once.
 __global__ __launch_bounds__(256) void geometry32_kernel(const double* __restrict__ geo, int K, int Kp, int residual,
                                                          float* __restrict__ out) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= Kp) return;
     const size_t sK = (size_t)Kp;
     if (e >= K) {
-        for (int r = 0; r < kGeo32Rows; ++r) out[r * sK + e] = 0.0f;
+        for (int r = 0; r < kGeo32NsRows; ++r) out[r * sK + e] = 0.0f;
         return;
     }
     const double det = geo[G_DET * sK + e];
@@ -154,7 +155,9 @@ __global__ __launch_bounds__(256) void geometry32_kernel(const double* __restric
     for (int f = 0; f < 3; ++f) {
         const double len = geo[(G_L0 + f) * sK + e];
         out[(G32_S0 + f) * sK + e] = residual ? float(len) : float(len * rdet);
+        out[(G32_L0 + f) * sK + e] = float(len);
     }
+    for (int k = 0; k < 4; ++k) out[(G32_G00 + k) * sK + e] = float(geo[(G_A00 + k) * sK + e]);
 }
 
 }  // namespace

@@ -107,12 +107,15 @@ struct QuadDims {
 __host__ __device__ inline uint32_t quad_code(uint32_t code, int f) { return (code >> (5 * f)) & 31u; }
 constexpr uint32_t kQuadAffineBit = 1u << 20;
 
-// FP32 per-element geometry of the matrix-core Euler tile kernel, rows [kGeo32Rows][Kp]: the
-// FP32 roundings of the FP64 geometry as the kernel consumes them, formed in FP64 on the device
+// FP32 per-element geometry of the matrix-core tile kernels, rows [kGeo32NsRows][Kp]: the FP32
+// roundings of the FP64 geometry as the kernels consume them, formed in FP64 on the device
 // (dgx_geometry.hip): dxi_dx (times |det J| for M dU/dt), the three outward normals, and the
-// three lift scales len_f / |det J| (len_f for M dU/dt).
+// three lift scales len_f / |det J| (len_f for M dU/dt) -- the Euler kernel's kGeo32Rows rows --
+// then, for the Navier-Stokes kernel, dxi_dx itself (the gradients) and the face lengths (the
+// penalty).
 enum Geo32Row : int { G32_A00 = 0, G32_A01, G32_A10, G32_A11, G32_N0X, G32_N0Y, G32_N1X, G32_N1Y, G32_N2X, G32_N2Y,
-                      G32_S0, G32_S1, G32_S2, kGeo32Rows };
+                      G32_S0, G32_S1, G32_S2, kGeo32Rows,
+                      G32_G00 = kGeo32Rows, G32_G01, G32_G10, G32_G11, G32_L0, G32_L1, G32_L2, kGeo32NsRows };
 
 // Face connections of the tile kernel, per element two 32-bit words, 16 bits per face f (word
 // f >> 1, shifted by 16 (f & 1)), addressing the tile's trace records: the records of the
@@ -125,8 +128,8 @@ enum Geo32Row : int { G32_A00 = 0, G32_A01, G32_A10, G32_A11, G32_N0X, G32_N0Y, 
 //   bit 14      boundary face
 // External faces of tile t are entries ext_off[t] .. ext_off[t+1]-1 of a list of int2 (x: the
 // neighbour element, or the boundary face's running index; y: neighbour's local face | reversed
-// order << 2 | the reading element's local face << 3 | its index in the tile << 5), its
-// ext_nint[t] faces with a neighbour first.
+// order << 2 | the reading element's local face << 3 | its index in the tile << 5 | the reading
+// element holds the canonical side << 11), its ext_nint[t] faces with a neighbour first.
 constexpr uint32_t kConnBoundary = 1u << 14;
 __host__ __device__ constexpr uint32_t conn_field(uint32_t w0, uint32_t w1, int f) {
     return ((f < 2 ? w0 : w1) >> (16 * (f & 1))) & 0xffffu;
@@ -152,8 +155,7 @@ struct StageParams {
     const uint32_t* code;
     const void* tab;        // tables in the state precision
     const void* tabm;       // lane-major tables of the MFMA kernel (FP32 Euler / Navier-Stokes), or null
-    const void* tabt;       // face-trace tables of the MFMA Navier-Stokes kernel, or null
-    void* rec;              // face-point records [3 Nfq][Kp] (U, Fv.n) of the MFMA Navier-Stokes stage
+    const void* tabt;       // face-trace and external-face tables of the MFMA Navier-Stokes kernel, or null
     const double* bc;       // boundary-face point rows (bc_row)
     const double* minve;    // quads: per-element inverse mass matrices [Np*Np][Kp] (FP64), else null
     const double* tab64;    // quads: the tables in FP64 (geometry nodes and weights, mass factors)
@@ -167,9 +169,9 @@ struct StageParams {
                      // (Pr (gamma - 1)) as the reference, its negative for conduction
     double uref[4];  // point-value storage offset (see Off4)
     double uoff[4];  // the same as a constant-mode coefficient (uref / phi_0)
-    // matrix-core Euler tile kernel (null elsewhere): FP32 geometry of this mode, face
-    // connections, external-face lists, and the largest count of interior external faces of a
-    // tile (sizes the dynamic LDS of their traces)
+    // matrix-core tile kernels (null elsewhere): FP32 geometry of this mode, face connections,
+    // external-face lists, and the largest count of interior external faces of a tile (sizes
+    // the dynamic LDS of their traces)
     const float* geo32;
     const uint32_t* conn;
     const int32_t* ext;       // int2 entries
@@ -178,6 +180,12 @@ struct StageParams {
     const int32_t* halo;      // int2 [tiles][64]: the first 64 external faces with a neighbour of each
                               // tile (the ext entries, x = -1 past the tile's count)
     int halo_cap;
+    // the FP32 tile kernels' scalars, rounded once on the host so that the kernels take them as
+    // wave-uniform operands: gamma, mu, the viscous lambda -2 mu / 3 (FP32 division), kappa,
+    // the penalty prefactor sigma0 mu (p+1)^2, dt, ca, cb, uref and uoff
+    struct F32 {
+        float gamma, mu, lam, kappa, sig0, dt, ca, cb, uref[4], uoff[4];
+    } f;
 };
 
 // Kernel launchers (dgx_kernels.hip / dgx_kernels_ns.hip).  Each returns hipSuccess or the

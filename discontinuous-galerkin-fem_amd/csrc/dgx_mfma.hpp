@@ -76,18 +76,33 @@ struct MfmaVjpShape {
     static constexpr int TOTAL = T_N * 64 + 2 * NT;
 };
 
-// Lane tables of the matrix-core Navier-Stokes stage (dgx_stage_mfma_ns.hip), 64 floats each.
-// Face-trace kernel: EF[R] (Phi_F) and FG[8][R], the 8 face-gradient slots (c, d, k) of the
-// reference's read (rows: face slots, k: modes).  Stage kernel: EV[R], GX[R], GY[R] (Phi_V and
-// its reference derivatives; rows: volume slots, k: modes), PX[RQ], PY[RQ] (rows: modes, k:
-// volume slots), LF[RF] (rows: modes, k: face slots).
+// Tables of the matrix-core Navier-Stokes tile kernel (dgx_stage_mfma_ns.hip).
+//
+// Face gradient.  Component c of the gradient at face point q is
+//   g_c(q) = sum_i sum_k dxi_dx[d][k] T_ck(q, i) u_i,
+// d = c under the plain read (T_0k = T_1k = dphi_i/dxi_k at q); under the reference's read, a
+// column-major view of the interleaved derivative storage, T_ck(q, i) is the derivative of
+// another mode at another point and d = e(c, q) xor (i mod 2) with the row parity
+// e(c, q) = (q Np + c Nfq Np) mod 2 (the builder checks both forms).
+//
+// Face traces (one buffer): the lane tables EF[R] (Phi_F) and FG[4][R] (T_ck, slot c*2 + k;
+// rows: face slots, k: modes), 64 floats each; then the external-face tables, per point q of
+// face f (index f Nfq + q) HP float4:
+//   [i]       (T_0i0, T_0i1, T_1i0, T_1i1) of mode i at the point
+//   [NP ..]   Phi_F of the point, modes 0 .. NP-1 (padded with zeros to whole float4)
+// Volume (one buffer per mode): EV[R], GX[R], GY[R] (Phi_V and its reference derivatives; rows:
+// volume slots, k: modes), PX[RQ], PY[RQ] (rows: modes, k: volume slots), LF[RF] (rows: modes,
+// k: face slots).
 template <int P>
 struct MfmaNsShape {
     using F = MfmaShape<P>;
-    static constexpr int R = F::R, RQ = F::RQ, RF = F::RF;
+    static constexpr int R = F::R, RQ = F::RQ, RF = F::RF, NP = F::NP, NFQ = F::NFQ;
     static constexpr int T_EF = 0;
     static constexpr int T_FG = T_EF + R;
-    static constexpr int TOTAL_T = (T_FG + 8 * R) * 64;
+    static constexpr int TOTAL_T = (T_FG + 4 * R) * 64;
+    static constexpr int HP = NP + (NP + 3) / 4;     // float4 per external-face table point
+    static constexpr int NHT = 3 * NFQ * HP * 4;     // floats of the external-face tables
+    static constexpr int TOTAL_TH = TOTAL_T + NHT;   // floats of the face-trace buffer
     static constexpr int M_EV = 0;
     static constexpr int M_GX = M_EV + R;
     static constexpr int M_GY = M_GX + R;
@@ -115,7 +130,8 @@ int mfma_table_size(int p);  // floats
 // With bf16 set, the bfloat16 volume tables follow (TOTAL_BF floats in all).
 void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out, bool bf16 = false);
 void build_mfma_vjp_tables(int p, const std::vector<double>& tab, std::vector<float>& out);
-// Navier-Stokes: face-trace tables (trace = true) or stage tables from the L / R tables.
-void build_mfma_ns_tables(int p, const std::vector<double>& tab, bool trace, std::vector<float>& out);
+// Navier-Stokes: face-trace and external-face tables (trace = true) or volume tables from the
+// L / R tables.
+void build_mfma_ns_tables(int p, const std::vector<double>& tab, bool trace, bool quirk, std::vector<float>& out);
 
 }  // namespace dgx

@@ -135,14 +135,33 @@ void build_vjp(const std::vector<double>& t, std::vector<float>& out) {
                 }
 }
 
+// Populated face-gradient slot of (point fp, component c, mode i) and its two values T_ck,
+// checked against the structure the kernel relies on (dgx_mfma.hpp).
+struct FaceGradEntry {
+    double v[2];
+};
 template <int P>
-void build_ns(const std::vector<double>& t, bool trace, std::vector<float>& out) {
+FaceGradEntry face_grad_entry(const std::vector<double>& t, const TabLayout& L, int fp, int c, int i, bool quirk) {
+    using S = MfmaNsShape<P>;
+    const int np = S::NP, nfq = S::NFQ, q = fp % nfq;
+    auto fg = [&](int d, int k) { return t[L.FG + (fp * 8 + (c * 2 + d) * 2 + k) * np + i]; };
+    const int e = (q * np + c * nfq * np) & 1;
+    const int d = quirk ? (e ^ (i & 1)) : c;
+    if (fg(1 - d, 0) != 0.0 || fg(1 - d, 1) != 0.0)
+        throw std::runtime_error("face-gradient table does not have the expected populated slots");
+    if (quirk && S::R % 2 != 0 && e != 0)
+        throw std::runtime_error("face-gradient row parity is not uniform for an odd slot count");
+    return {{fg(d, 0), fg(d, 1)}};
+}
+
+template <int P>
+void build_ns(const std::vector<double>& t, bool trace, bool quirk, std::vector<float>& out) {
     using S = MfmaNsShape<P>;
     using F = MfmaShape<P>;
     const TabLayout L(P, 0);
     const int np = F::NP, nq = F::NQ, nfq = F::NFQ;
     if ((int)t.size() != L.total) throw std::runtime_error("table size mismatch");
-    out.assign(trace ? S::TOTAL_T : S::TOTAL_M, 0.0f);
+    out.assign(trace ? S::TOTAL_TH : S::TOTAL_M, 0.0f);
     auto at = [&](int table, int lane) -> float& { return out[(size_t)table * 64 + lane]; };
     for (int lane = 0; lane < 64; ++lane) {
         const int i = lane % 16, k = lane / 16;
@@ -155,8 +174,10 @@ void build_ns(const std::vector<double>& t, bool trace, std::vector<float>& out)
             if (m < 0) continue;
             if (trace) {
                 at(S::T_EF + s, lane) = float(t[L.phiF + fp * np + m]);
-                for (int slot = 0; slot < 8; ++slot)
-                    at(S::T_FG + slot * S::R + s, lane) = float(t[L.FG + (fp * 8 + slot) * np + m]);
+                for (int c = 0; c < 2; ++c) {
+                    const FaceGradEntry x = face_grad_entry<P>(t, L, fp, c, m, quirk);
+                    for (int k = 0; k < 2; ++k) at(S::T_FG + (c * 2 + k) * S::R + s, lane) = float(x.v[k]);
+                }
             } else {
                 at(S::M_EV + s, lane) = float(t[L.phiV + vp * np + m]);
                 at(S::M_GX + s, lane) = float(t[L.DXV + vp * np + m]);
@@ -175,14 +196,27 @@ void build_ns(const std::vector<double>& t, bool trace, std::vector<float>& out)
             at(S::M_LF + r, lane) = (mrow >= 0 && f >= 0) ? float(t[L.LF + f * np + mrow]) : 0.0f;
         }
     }
+    if (!trace) return;
+    // external-face tables
+    for (int fp = 0; fp < 3 * nfq; ++fp) {
+        float* h = &out[S::TOTAL_T + (size_t)fp * S::HP * 4];
+        for (int i = 0; i < np; ++i) {
+            for (int c = 0; c < 2; ++c) {
+                const FaceGradEntry x = face_grad_entry<P>(t, L, fp, c, i, quirk);
+                h[i * 4 + 2 * c] = float(x.v[0]);
+                h[i * 4 + 2 * c + 1] = float(x.v[1]);
+            }
+            h[np * 4 + i] = float(t[L.phiF + fp * np + i]);
+        }
+    }
 }
 
 }  // namespace
 
-void build_mfma_ns_tables(int p, const std::vector<double>& tab, bool trace, std::vector<float>& out) {
+void build_mfma_ns_tables(int p, const std::vector<double>& tab, bool trace, bool quirk, std::vector<float>& out) {
     switch (p) {
-    case 3: build_ns<3>(tab, trace, out); break;
-    case 4: build_ns<4>(tab, trace, out); break;
+    case 3: build_ns<3>(tab, trace, quirk, out); break;
+    case 4: build_ns<4>(tab, trace, quirk, out); break;
     default: throw std::runtime_error("no MFMA Navier-Stokes tables for this order");
     }
 }

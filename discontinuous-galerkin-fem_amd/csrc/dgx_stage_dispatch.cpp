@@ -18,8 +18,9 @@ hipError_t launch_stage(int precision, int physics, int p, int rule, const Stage
     if (sp.tabm && precision != DGX_FP64 && physics == DGX_EULER && rule == 0 && mfma_order(p))
         return launch_stage_mfma(p, precision == DGX_FP32_BF16VOL, sp, s);
     const bool quirk = sp.ns_face_quirk != 0;
-    // FP32 Navier-Stokes likewise, as a face-trace launch and a stage launch
-    if (sp.tabm && sp.tabt && sp.rec && precision == DGX_FP32 && physics == DGX_NAVIER_STOKES && rule == 0 && mfma_order(p))
+    // FP32 Navier-Stokes likewise (tile kernel with its own face-trace tables)
+    if (sp.tabm && sp.tabt && sp.conn && precision == DGX_FP32 && physics == DGX_NAVIER_STOKES && rule == 0 &&
+        mfma_order(p))
         return launch_stage_mfma_ns(p, quirk, sp, s);
     const bool f64 = precision == DGX_FP64;
     if (physics == DGX_EULER)

@@ -73,13 +73,17 @@ __device__ __forceinline__ void rusanov(const Real* A, const Real* B, Real nx, R
 }
 
 // Viscous fluxes (compute_primitive_gradients + compute_viscous_fluxes,
-// navier_stokes_weak_formulation.cpp:36-98), gas constant 1.  g[v*2+d] = d U_v / d x_d.
+// navier_stokes_weak_formulation.cpp:36-98), gas constant 1.  g[v*2+d] = d U_v / d x_d;
+// lam = -2 mu / 3 (viscous_lambda(mu), formed once by the caller).
 template <typename Real>
-__device__ __forceinline__ void viscous_flux(const Real* U, const Real* g, Real gam, Real mu,
-                                             Real kappa, Real* Fv, Real* Gv) {
+__device__ __forceinline__ Real viscous_lambda(Real mu) { return Real(-2) * mu / Real(3); }
+template <typename Real>
+__device__ __forceinline__ void viscous_flux_l(const Real* U, const Real* g, Real gam, Real mu, Real lam,
+                                               Real kappa, Real* Fv, Real* Gv) {
     Real rho = U[0];
-    Real ri_safe = rcp(ref_max(rho, Real(1e-12)));
     Real ri = rcp(rho);
+    // 1 / max(rho, 1e-12) with the reference's max (a NaN density passes through)
+    Real ri_safe = rho < Real(1e-12) ? rcp(Real(1e-12)) : ri;
     Real u = U[1] * ri, v = U[2] * ri;
     Real p = (gam - Real(1)) * (U[3] - Real(0.5) * rho * (u * u + v * v));
     Real ux = (g[2] - u * g[0]) * ri_safe, uy = (g[3] - u * g[1]) * ri_safe;
@@ -90,13 +94,17 @@ __device__ __forceinline__ void viscous_flux(const Real* U, const Real* g, Real 
     Real rsq_i = rcp(rho * rho);
     Real Tx = (rho * px - p * g[0]) * rsq_i, Ty = (rho * py - p * g[1]) * rsq_i;
     Real div = ux + vy;
-    Real lam = Real(-2) * mu / Real(3);
     Real txx = Real(2) * mu * ux + lam * div;
     Real tyy = Real(2) * mu * vy + lam * div;
     Real txy = mu * (uy + vx);
     Real qx = -kappa * Tx, qy = -kappa * Ty;
     Fv[0] = Real(0); Fv[1] = txx; Fv[2] = txy; Fv[3] = u * txx + v * txy + qx;
     Gv[0] = Real(0); Gv[1] = txy; Gv[2] = tyy; Gv[3] = u * txy + v * tyy + qy;
+}
+template <typename Real>
+__device__ __forceinline__ void viscous_flux(const Real* U, const Real* g, Real gam, Real mu,
+                                             Real kappa, Real* Fv, Real* Gv) {
+    viscous_flux_l(U, g, gam, mu, viscous_lambda(mu), kappa, Fv, Gv);
 }
 
 template <typename Real>

@@ -108,13 +108,16 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     }
     const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
     // geometry rows of the tile: wave w loads rows w + 4k (one 256-byte row per instruction;
-    // rows past the last read past the resource: zeros, no traffic)
+    // rows past the last read at an offset past the resource: zeros, no traffic)
     constexpr int kGeoPerThread = (kGeo32Rows + 3) / 4;
     float geov[kGeoPerThread];
 #pragma unroll
-    for (int k = 0; k < kGeoPerThread; ++k)
+    for (int k = 0; k < kGeoPerThread; ++k) {
+        const int row = wu + 4 * k;
         geov[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-            r_geo, unsigned(eb0 + lane) * 4u, unsigned(min(wu + 4 * k, kGeo32Rows)) * 4u * uKp, 0));
+            r_geo, row < kGeo32Rows ? (unsigned(row) * uKp + unsigned(eb0 + lane)) * 4u : unsigned(kGeo32Rows) * 4u * uKp,
+            0, 0));
+    }
     // the tables, staged in LDS by cooperative loads (so that no lane holds its table entries in
     // registers while the loads are in flight)
     constexpr int NTAB = MfmaLds<P, BF16>::NTAB;

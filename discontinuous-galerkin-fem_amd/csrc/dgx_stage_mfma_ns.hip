@@ -10,263 +10,399 @@
 //   inviscid terms   S/weak_forms/euler_weak_formulation.cpp:72-250
 //   assembly         S/solver/assembler.cpp:137-279
 //
-// Two launches per stage.  The face-trace kernel evaluates, for every element and every point
-// of its three faces, the state and the 8-slot face gradient on the matrix cores, forms the
-// viscous flux along the element's outward normal and writes (U, Fv.n) to a record buffer
-// rec[face point][element].  The stage kernel then needs no neighbour gradients: it reads the
-// two records of each face point (its own and the neighbour's, in the neighbour's point order)
-// and combines them as the reference does, with the canonical (L-side) orientation so both
-// copies of a face flux come from the same arithmetic.  Normals of the two sides are exact
-// negatives (geometry stores the L normal on both), so the neighbour's Fv.n_L is the negative of
-// its record.  Volume: U, dU/dxi and dU/deta from the coefficients, and the projection of the
-// contravariant inviscid-minus-viscous fluxes, on the matrix cores; the pointwise physics on the
-// VALU.  Lane layout as the Euler kernel (dgx_mfma.hpp).
+// One launch per stage.  A block owns one 64-element tile of the context's element order
+// (Morton order of the centroids, so a tile is a compact patch of the mesh), each wave 16 of
+// its elements (lane layout in dgx_mfma.hpp), as in the Euler tile kernel (dgx_stage_mfma.hip).
+//
+// Faces.  The reference's interior face flux in the canonical (L) orientation is
+//   Rusanov(U_L, U_R, n_L) + wL Fv_L.n_L + wR Fv_R.n_L - sigma (U_R - U_L)
+// with (wL, wR) = (1/2, -1/2) under the reference's viscous average and (-1/2, -1/2) otherwise,
+// and on a boundary face Rusanov(U, U_ghost, n) + wB Fv.n - sigma (U_bc - U), wB = 2 wL.  Every
+// face point gets one trace record, formed by the element that owns the point and kept in LDS:
+// its state U, the normal flux along the owner's outward normal G = F(U).n + c Fv(U, grad U).n
+// with the owner's weight c (2 wL on the L side, 2 wR on the R side, 2 wB on a boundary face)
+// and the wave speed s = |u.n| + c_sound.  The normals of the two sides are exact negatives, so
+// twice the face flux in an element's own orientation is
+//   X = (G_own - G_nb) - (max(s_L, s_R) + 2 sigma) (U_nb - U_own),
+// the exact negative of the neighbour's X: both copies of an in-tile face flux agree bit for
+// bit.  The neighbours' records on the faces that leave the tile are evaluated per lane from the
+// neighbour's coefficients, one face point per lane.
+//
+// A tile runs in four phases; the LDS table region holds the face-trace and external-face
+// tables for the first two and the volume tables for the last two:
+//   1. records of the external faces' neighbour points
+//   2. the tile's own face points: U and the face-gradient sums on the matrix cores, combined
+//      with the element's dxi_dx per point (dgx_mfma.hpp), own records
+//   3. volume: U, dU/dxi and dU/deta at the volume points and the projection of the
+//      contravariant inviscid-minus-viscous fluxes, on the matrix cores
+//   4. face fluxes from the records, boundary faces, lift, RK update.
 #include "dgx_mfma_dev.hpp"
 
 namespace dgx {
 namespace {
 
-// record of one face point: U and the viscous normal flux (first component zero)
-struct FaceRecord {
-    float4 u, v;
+constexpr int kTileEl = kBlock / 4;  // elements per block tile: 4 waves x 16
+
+template <int P>
+struct NsTile {
+    using S = MfmaNsShape<P>;
+    static constexpr int NREC = 3 * S::NFQ * kTileEl;  // records of the tile's own face points
+    static constexpr int NT1 = S::TOTAL_TH;             // face-trace + external-face tables (floats)
+    static constexpr int REGION = NT1 > S::TOTAL_M ? NT1 : S::TOTAL_M;
+};
+
+// Trace records of the tile in dynamic LDS: U, G and s of every own face point
+// ([face][point][element]), then those of the external faces' neighbour points ([slot][point],
+// the reading element's point order): NREC + halo_cap Nfq entries per array.
+struct TraceRecs {
+    float4* U;
+    float4* G;
+    float* S;
 };
 
 template <int P, bool QUIRK>
-__device__ __forceinline__ void ns_trace_body(const StageParams& sp) {
+__device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* __restrict__ tab,
+                                        float4* __restrict__ dyn) {
     using F = MfmaShape<P>;
     using S = MfmaNsShape<P>;
-    constexpr int NP = F::NP, NFQ = F::NFQ, R = F::R;
-    constexpr int EB = kBlock / 4;
-    __shared__ float s_tab[S::TOTAL_T];
+    using T = NsTile<P>;
+    constexpr int NP = F::NP, NFQ = F::NFQ, R = F::R, RQ = F::RQ, RF = F::RF, HP = S::HP;
+    constexpr int EB = kTileEl, NREC = T::NREC;
+    const int eb0 = tile * EB;
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
-    const int e = xcd_tile(blockIdx.x, gridDim.x) * EB + 16 * w + j;
+    const int eloc = 16 * w + j;
+    const int e = eb0 + eloc;  // element of this lane in the product layout
     const bool valid = e < sp.K;
-    const int ec = valid ? e : sp.K - 1;
+    const int ec = valid ? e : sp.K - 1;  // lanes past the end compute on a valid element
     const unsigned uKp = unsigned(sp.Kp);
-    const float gam = float(sp.gamma), mu = float(sp.mu);
-    const float kappa = float(sp.kappa);
-    const float ur[4] = {float(sp.uref[0]), float(sp.uref[1]), float(sp.uref[2]), float(sp.uref[3])};
+    const bool residual_mode = sp.mode == MODE_RESIDUAL;
+    // wave-uniform scalars rounded on the host (lam = viscous_lambda(mu))
+    const float gam = sp.f.gamma, mu = sp.f.mu, kappa = sp.f.kappa, lam = sp.f.lam;
+    const float* ur = sp.f.uref;
+    // storage offset of the constant mode: under the reference's face-gradient read that mode
+    // has a populated gradient row, so its operand is the full coefficient
+    const float* uo = sp.f.uoff;
+    // parity e(c, q) of the populated dxi_dx row of face point q under the reference's read
+    auto row_parity = [](int c, int q) { return ((q * NP + c * NFQ * NP) & 1) != 0; };
+    const float* __restrict__ tm = static_cast<const float*>(sp.tabm);
     const float* __restrict__ tt = static_cast<const float*>(sp.tabt);
+    const int xs = sp.ext_off[tile];     // the tile's external faces
+    const int nint = sp.ext_nint[tile];  // ... of which the first nint have a neighbour
+    const int nrs = NREC + (sp.halo_cap > 1 ? sp.halo_cap : 1) * NFQ;  // records per array
+    const TraceRecs H{dyn, dyn + nrs, reinterpret_cast<float*>(dyn + 2 * nrs)};
+    // record weights of the viscous normal flux: L side, R side, boundary face
+    const float cL = sp.ns_avg_ref ? 1.0f : -1.0f, cR = -1.0f, cB = 2.0f * cL;
 
-    constexpr int kTabPerThread = (S::TOTAL_T + kBlock - 1) / kBlock;
-    float tabv[kTabPerThread];
-#pragma unroll
-    for (int k = 0; k < kTabPerThread; ++k) {
-        const int t = int(threadIdx.x) + k * kBlock;
-        tabv[k] = t < S::TOTAL_T ? tt[t] : 0.0f;
-    }
+    // ---------------- loads, issued up front in dependency order ----------------------------
     const unsigned state_bytes = unsigned(NP) * 16u * uKp;
     const auto r_uk = make_rsrc(sp.uk, state_bytes);
-    const auto r_geo = make_rsrc(sp.geo, unsigned(kGeoRows) * 8u * uKp);
+    const auto r_geo = make_rsrc(sp.geo32, unsigned(kGeo32NsRows) * 4u * uKp);
+    const auto r_conn = make_rsrc(sp.conn, 8u * uKp);
+    // geometry entry (row, el), the row a wave-uniform offset (geo_at) or a per-lane one
+    auto geo_at = [&](int row, unsigned el) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_geo, el * 4u, unsigned(row) * 4u * uKp, 0));
+    };
+    auto geo_at_lane = [&](int row, unsigned el) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_geo, (unsigned(row) * uKp + el) * 4u, 0, 0));
+    };
+    const v2u cw = __builtin_bit_cast(v2u, __builtin_amdgcn_raw_buffer_load_b64(r_conn, unsigned(ec) * 8u, 0, 0));
     auto slot_off = [&](int s, unsigned el) -> unsigned {
         const int m = mf_mode(R, NP, g, s);
         return m >= 0 ? (unsigned(m) * uKp + el) * 16u : state_bytes;
     };
-    float b[4][R];
+    float b[4][R];  // coefficients of this lane's mode slots: product operands and RK input
 #pragma unroll
     for (int s = 0; s < R; ++s) {
         const v4u x = load_x4(r_uk, slot_off(s, unsigned(ec)));
 #pragma unroll
         for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
     }
-    const unsigned go = unsigned(ec) * 8u;
-    double gA[4], gN[6];
+    // face-trace and external-face tables, staged in LDS by cooperative loads
+    constexpr int kT1 = (T::NT1 + kBlock - 1) / kBlock;
+    float t1v[kT1];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) gA[k] = load_f64(r_geo, go, unsigned(G_A00 + k) * 8u * uKp);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) gN[k] = load_f64(r_geo, go, unsigned(G_N0X + k) * 8u * uKp);
-#pragma unroll
-    for (int k = 0; k < kTabPerThread; ++k) {
+    for (int k = 0; k < kT1; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        if (t < S::TOTAL_T) s_tab[t] = tabv[k];
+        t1v[k] = t < T::NT1 ? tt[t] : 0.0f;
+    }
+    // External-face points: pair t = (face hk, point hq) of the tile's external faces with a
+    // neighbour, one per thread, dealt from wave (tile mod 4) on so that the tiles sharing a CU
+    // load its four SIMDs evenly.  The first 256 pairs (256 / Nfq < 64 faces) come from the
+    // fixed-stride copy of the tile's first 64 external faces; neighbour coefficients, dxi_dx and
+    // the reading element's normal of the thread's first pair are loaded here.
+    const int npair = nint * NFQ;
+    int hp = (int(threadIdx.x) + 64 * (tile & 3)) & (kBlock - 1);
+    float cn[NP][4];
+    float hA[4], hnx, hny;
+    uint32_t hcode;
+    auto halo_load = [&](int2 x) {
+        hcode = uint32_t(x.y);
+        const bool has = x.x >= 0;
+        const unsigned en = has ? unsigned(x.x) : 0u;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            // no neighbour: an offset past the resource, a zero read without traffic
+            const v4u y = load_x4(r_uk, has ? (unsigned(i) * uKp + en) * 16u : state_bytes);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hA[k] = geo_at(G32_G00 + k, en);
+        // the neighbour's outward normal: the negated normal of the reading element's face
+        const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
+        hnx = -geo_at_lane(G32_N0X + 2 * rf, unsigned(eb0 + rel));
+        hny = -geo_at_lane(G32_N0Y + 2 * rf, unsigned(eb0 + rel));
+    };
+    halo_load(reinterpret_cast<const int2*>(sp.halo)[tile * 64 + hp / NFQ]);
+#pragma unroll
+    for (int k = 0; k < kT1; ++k) {
+        const int t = int(threadIdx.x) + k * kBlock;
+        if (t < T::NT1) tab[t] = t1v[k];
     }
     __syncthreads();
+    // keep the scheduler from sinking the loads above into the arithmetic below
     __builtin_amdgcn_sched_barrier(0);
 
-    const float* lt = &s_tab[lane];
-    // traces of U at the face slots
-    v4f Uf[4];
+    // ---------------- 1. records of the external faces' neighbour points ---------------------
+    for (;;) {
+        if (hp < npair) {
+            const int hk = hp / NFQ, hq = hp - hk * NFQ;
+            const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
+            const bool read_is_l = (hcode >> 11) & 1u;
+            const int qq = flip ? NFQ - 1 - hq : hq;  // the point in the neighbour's order
+            const float4* ht = reinterpret_cast<const float4*>(&tab[S::TOTAL_T]) + (fn * NFQ + qq) * HP;
+            float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int i4 = 0; i4 < (NP + 3) / 4; ++i4) {
+                const float4 ph = ht[NP + i4];
+                const float phv[4] = {ph.x, ph.y, ph.z, ph.w};
 #pragma unroll
-        for (int s = 0; s < R; ++s) c = mfma16x16x4(lt[(S::T_EF + s) * 64], b[v][s], c);
-        Uf[v] = c;
+                for (int c = 0; c < 4; ++c) {
+                    if (4 * i4 + c >= NP) continue;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Un[v] = fma(phv[c], cn[4 * i4 + c][v], Un[v]);
+                }
+            }
+            // gradient: component c = sum_i (dxi_dx[d][0] T_ci0 + dxi_dx[d][1] T_ci1) u_i with the
+            // populated row d = c, or e(c, qq) xor (i mod 2) under the reference's read (the
+            // constant mode then with the storage offset)
+            float Ar[2][2][2];  // [c][i mod 2][k]: the dxi_dx row of each component and mode parity
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int par = 0; par < 2; ++par) {
+                    const bool d = QUIRK ? (row_parity(c, qq) != (par != 0)) : (c != 0);
+                    Ar[c][par][0] = d ? hA[2] : hA[0];
+                    Ar[c][par][1] = d ? hA[3] : hA[1];
+                }
+            float gq[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const float4 a = ht[i];
+                const float w0 = fmaf(Ar[0][i & 1][0], a.x, Ar[0][i & 1][1] * a.y);
+                const float w1 = fmaf(Ar[1][i & 1][0], a.z, Ar[1][i & 1][1] * a.w);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float ui = (QUIRK && i == 0) ? cn[0][v] + uo[v] : cn[i][v];
+                    gq[v * 2] = fmaf(w0, ui, gq[v * 2]);
+                    gq[v * 2 + 1] = fmaf(w1, ui, gq[v * 2 + 1]);
+                }
+            }
+            float Fv[4], Gv[4], Fn[4], sw;
+            viscous_flux_l(Un, gq, gam, mu, lam, kappa, Fv, Gv);
+            trace_record(Un, hnx, hny, gam, Fn, sw);
+            const float cn_w = read_is_l ? cR : cL;  // the neighbour holds the other side
+            const int k = NREC + hk * NFQ + hq;
+            H.U[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
+            H.G[k] = make_float4(Fn[0], Fn[1] + cn_w * (Fv[1] * hnx + Gv[1] * hny),
+                                 Fn[2] + cn_w * (Fv[2] * hnx + Gv[2] * hny), Fn[3] + cn_w * (Fv[3] * hnx + Gv[3] * hny));
+            H.S[k] = sw;
+        }
+        // more than 256 external-face points (tiles that are not compact patches): the thread's
+        // next pair, 256 on
+        hp += kBlock;
+        if (hp >= npair) break;
+        halo_load(reinterpret_cast<const int2*>(sp.ext)[xs + hp / NFQ]);
     }
-    // face gradients: g[v][c] = sum_{d,k} FG[(c*2+d)*2+k] (A[d][k] u)_v -- the element's
-    // dxi_dx entry scales the operand (one column per element), so each (variable, component)
-    // accumulates in a single product chain; without the reference's read only the d = c slots
-    // are populated
-    // Under the reference's read the constant mode has a nonzero gradient row, so its operand
-    // is the full coefficient: the stored value plus the storage offset.
-    float bg[4][R];
+
+    // ---------------- 2. own face points ----------------------------------------------------
+    // this lane's element geometry as each phase needs it: dxi_dx and the normals here (their
+    // latency covered by the trace products below)
+    const float A[2][2] = {{geo_at(G32_G00, unsigned(ec)), geo_at(G32_G01, unsigned(ec))},
+                           {geo_at(G32_G10, unsigned(ec)), geo_at(G32_G11, unsigned(ec))}};
+    float fnx[3], fny[3];
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-        const bool m0 = QUIRK && mf_mode(R, NP, g, s) == 0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) bg[v][s] = m0 ? b[v][s] + float(sp.uoff[v]) : b[v][s];
+    for (int f = 0; f < 3; ++f) {
+        fnx[f] = geo_at(G32_N0X + 2 * f, unsigned(ec));
+        fny[f] = geo_at(G32_N0Y + 2 * f, unsigned(ec));
     }
-    v4f gr[4][2];
+    const float* lt = &tab[lane];  // this lane's entries of the region's lane tables
+    constexpr int kTM = (S::TOTAL_M + kBlock - 1) / kBlock;
+    float tmv[kTM];
+    {
+        v4f Uf[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) gr[v][0] = gr[v][1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int v = 0; v < 4; ++v) {
+            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int dk = 0; dk < 4; ++dk) {
-        const int d = dk >> 1, k = dk & 1;
-        const float wA = float(gA[dk]);
-        float bs[4][R];
+            for (int s = 0; s < R; ++s) c = mfma16x16x4(lt[(S::T_EF + s) * 64], b[v][s], c);
+            Uf[v] = c;
+        }
+        // face gradients g[v][c] = sum_k sum_i dxi_dx[d][k] T_ck u_i (the populated row d as in
+        // dgx_mfma.hpp), the constant mode's operand with the storage offset under the
+        // reference's read
+        float b0q[4];  // slot 0 (mode 0 in lanes g = 0)
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
+        for (int v = 0; v < 4; ++v) b0q[v] = (QUIRK && g == 0) ? b[v][0] + uo[v] : b[v][0];
+        auto bq = [&](int v, int s) { return s == 0 ? b0q[v] : b[v][s]; };
+        auto tfg = [&](int c, int k, int s) { return lt[(S::T_FG + (c * 2 + k) * R + s) * 64]; };
+        float gr[4][2][4];  // [v][c][register]: scalars, so no register tuples need assembling
+        if constexpr (!QUIRK) {
+            // d = c: both components from the two reference derivatives S_k at the point
 #pragma unroll
-            for (int s = 0; s < R; ++s) bs[v][s] = wA * bg[v][s];
+            for (int v = 0; v < 4; ++v) {
+                v4f s0 = {0.0f, 0.0f, 0.0f, 0.0f}, s1 = s0;
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            if (!QUIRK && d != c) continue;
-            const int slot = (c * 2 + d) * 2 + k;
+                for (int s = 0; s < R; ++s) {
+                    s0 = mfma16x16x4(tfg(0, 0, s), bq(v, s), s0);
+                    s1 = mfma16x16x4(tfg(0, 1, s), bq(v, s), s1);
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) gr[v][c][r] = A[c][0] * s0[r] + A[c][1] * s1[r];
+            }
+        } else if constexpr (R % 2 == 0) {
+            // mode R g + s has the parity of s: per (c, k) the even- and odd-mode sums S_e, S_o,
+            // then per row (point q) d = e(c, q) for the even modes and 1 - e(c, q) for the odd;
+            // registers r < 3 hold point g, register 3 point 4 (a row parity known here)
+            float Ae[2][2], Ao[2][2];  // [c][k], rows of point g
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const bool e = row_parity(c, g);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    Ae[c][k] = e ? A[1][k] : A[0][k];
+                    Ao[c][k] = e ? A[0][k] : A[1][k];
+                }
+            }
 #pragma unroll
             for (int v = 0; v < 4; ++v)
 #pragma unroll
-                for (int s = 0; s < R; ++s) gr[v][c] = mfma16x16x4(lt[(S::T_FG + slot * R + s) * 64], bs[v][s], gr[v][c]);
+                for (int c = 0; c < 2; ++c) {
+                    const v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
+                    v4f se0 = mfma16x16x4(tfg(c, 0, 0), bq(v, 0), z), se1 = mfma16x16x4(tfg(c, 1, 0), bq(v, 0), z);
+                    v4f so0 = mfma16x16x4(tfg(c, 0, 1), bq(v, 1), z), so1 = mfma16x16x4(tfg(c, 1, 1), bq(v, 1), z);
+#pragma unroll
+                    for (int s = 2; s < R; s += 2) {
+                        se0 = mfma16x16x4(tfg(c, 0, s), bq(v, s), se0);
+                        se1 = mfma16x16x4(tfg(c, 1, s), bq(v, s), se1);
+                        so0 = mfma16x16x4(tfg(c, 0, s + 1), bq(v, s + 1), so0);
+                        so1 = mfma16x16x4(tfg(c, 1, s + 1), bq(v, s + 1), so1);
+                    }
+                    const int e4c = (4 * NP + c * NFQ * NP) & 1;  // row parity of point 4
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float ae0 = r < 3 ? Ae[c][0] : A[e4c][0], ae1 = r < 3 ? Ae[c][1] : A[e4c][1];
+                        const float ao0 = r < 3 ? Ao[c][0] : A[1 - e4c][0], ao1 = r < 3 ? Ao[c][1] : A[1 - e4c][1];
+                        gr[v][c][r] = ae0 * se0[r] + ae1 * se1[r] + ao0 * so0[r] + ao1 * so1[r];
+                        // formed here: left to the compiler, the sums would sink to the records
+                        // below and keep all four product results of every (v, c) live
+                        asm volatile("" : "+v"(gr[v][c][r]));
+                    }
+                }
+        } else {
+            // odd slot count: the row parity is zero throughout (checked on the host), so d is
+            // the mode's parity and dxi_dx[d][k] goes into the operand
+            v4f gs[4][2];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) gs[v][0] = gs[v][1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                float bs[4][R];
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    const int m = mf_mode(R, NP, g, s);
+                    const float a = (m & 1) ? A[1][k] : A[0][k];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) bs[v][s] = a * bq(v, s);
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+#pragma unroll
+                        for (int s = 0; s < R; ++s) gs[v][c] = mfma16x16x4(tfg(c, k, s), bs[v][s], gs[v][c]);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) gr[v][c][r] = gs[v][c][r];
+        }
+        // volume tables, loaded now and staged once every wave is done with the region
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < kTM; ++k) {
+            const int t = int(threadIdx.x) + k * kBlock;
+            tmv[k] = t < S::TOTAL_M ? tm[t] : 0.0f;
+        }
+        // records of the own face points: slot (g, f) = face f, point g (f < 3); face g, point 4
+        // (f = 3, p = 4)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int fp = mf_fpoint(NFQ, g, f);
+            if (f == 3 && NFQ < 5) continue;
+            const int face = fp < 0 ? 0 : fp / NFQ, q = fp < 0 ? 0 : fp % NFQ;
+            const float U[4] = {Uf[0][f] + ur[0], Uf[1][f] + ur[1], Uf[2][f] + ur[2], Uf[3][f] + ur[3]};
+            float gq[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                gq[v * 2] = gr[v][0][f];
+                gq[v * 2 + 1] = gr[v][1][f];
+            }
+            const uint32_t c = conn_field(cw[0], cw[1], face);
+            const float wv = (c & kConnBoundary) ? cB : ((c >> 13) & 1u) ? cL : cR;
+            const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
+            const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
+            float Fv[4], Gv[4], Fn[4], sw;
+            viscous_flux_l(U, gq, gam, mu, lam, kappa, Fv, Gv);
+            trace_record(U, nx, ny, gam, Fn, sw);
+            if (fp >= 0) {
+                const int k = (face * NFQ + q) * EB + eloc;
+                H.U[k] = make_float4(U[0], U[1], U[2], U[3]);
+                H.G[k] = make_float4(Fn[0], Fn[1] + wv * (Fv[1] * nx + Gv[1] * ny), Fn[2] + wv * (Fv[2] * nx + Gv[2] * ny),
+                                     Fn[3] + wv * (Fv[3] * nx + Gv[3] * ny));
+                H.S[k] = sw;
+            }
         }
     }
-    if (!valid) return;
-    const auto r_rec = make_rsrc(sp.rec, unsigned(3 * NFQ) * 32u * uKp);
+    // the rest of the geometry, in flight over the barrier
+    const float a00 = geo_at(G32_A00, unsigned(ec)), a01 = geo_at(G32_A01, unsigned(ec));
+    const float a10 = geo_at(G32_A10, unsigned(ec)), a11 = geo_at(G32_A11, unsigned(ec));
+    float fsf[3], flen[3];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int fp = mf_fpoint(NFQ, g, r);
-        if (fp < 0) continue;
-        const int face = fp / NFQ;
-        const float U[4] = {Uf[0][r] + ur[0], Uf[1][r] + ur[1], Uf[2][r] + ur[2], Uf[3][r] + ur[3]};
-        float gq[8];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            gq[v * 2] = gr[v][0][r];
-            gq[v * 2 + 1] = gr[v][1][r];
-        }
-
-        float Fv[4], Gv[4];
-        viscous_flux(U, gq, gam, mu, kappa, Fv, Gv);
-        const float nx = float(face == 0 ? gN[0] : face == 1 ? gN[2] : gN[4]);
-        const float ny = float(face == 0 ? gN[1] : face == 1 ? gN[3] : gN[5]);
-        const unsigned ro = (unsigned(fp) * uKp + unsigned(e)) * 32u;
-        const v4u xu = {__float_as_uint(U[0]), __float_as_uint(U[1]), __float_as_uint(U[2]), __float_as_uint(U[3])};
-        const v4u xv = {0u, __float_as_uint(Fv[1] * nx + Gv[1] * ny), __float_as_uint(Fv[2] * nx + Gv[2] * ny),
-                        __float_as_uint(Fv[3] * nx + Gv[3] * ny)};
-        __builtin_amdgcn_raw_buffer_store_b128(xu, r_rec, ro, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b128(xv, r_rec, ro + 16u, 0, 0);
+    for (int f = 0; f < 3; ++f) {
+        fsf[f] = geo_at(G32_S0 + f, unsigned(ec));
+        flen[f] = geo_at(G32_L0 + f, unsigned(ec));
     }
-}
-
-template <int P>
-__device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
-    using F = MfmaShape<P>;
-    using S = MfmaNsShape<P>;
-    constexpr int NP = F::NP, NFQ = F::NFQ, R = F::R, RQ = F::RQ, RF = F::RF;
-    constexpr int EB = kBlock / 4;
-    __shared__ float s_tab[S::TOTAL_M];
-    __shared__ double s_geo[kGeoRows][EB];
-
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int g = lane >> 4, j = lane & 15;
-    const int eb0 = xcd_tile(blockIdx.x, gridDim.x) * EB;
-    const int e = eb0 + 16 * w + j;
-    const bool valid = e < sp.K;
-    const int ec = valid ? e : sp.K - 1;
-    const unsigned uKp = unsigned(sp.Kp);
-    const bool residual_mode = sp.mode == MODE_RESIDUAL;
-    const float gam = float(sp.gamma), mu = float(sp.mu);
-    const float kappa = float(sp.kappa);
-    const float ur[4] = {float(sp.uref[0]), float(sp.uref[1]), float(sp.uref[2]), float(sp.uref[3])};
-    const float* __restrict__ tm = static_cast<const float*>(sp.tabm);
-    const auto r_rec = make_rsrc(sp.rec, unsigned(3 * NFQ) * 32u * uKp);
-    auto load_record = [&](unsigned idx) {
-        FaceRecord r;
-        const v4u a = load_x4(r_rec, idx * 32u), b = load_x4(r_rec, idx * 32u + 16u);
-        r.u = make_float4(__uint_as_float(a[0]), __uint_as_float(a[1]), __uint_as_float(a[2]), __uint_as_float(a[3]));
-        r.v = make_float4(__uint_as_float(b[0]), __uint_as_float(b[1]), __uint_as_float(b[2]), __uint_as_float(b[3]));
-        return r;
-    };
-
-    // ---------------- loads, issued up front ----------------------------------------------
-    constexpr int kTabPerThread = (S::TOTAL_M + kBlock - 1) / kBlock;
-    float tabv[kTabPerThread];
+    __syncthreads();  // every record of the tile complete, the region's tables done with
 #pragma unroll
-    for (int k = 0; k < kTabPerThread; ++k) {
+    for (int k = 0; k < kTM; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        tabv[k] = t < S::TOTAL_M ? tm[t] : 0.0f;
-    }
-    const unsigned state_bytes = unsigned(NP) * 16u * uKp;
-    const auto r_uk = make_rsrc(sp.uk, state_bytes);
-    const auto r_geo = make_rsrc(sp.geo, unsigned(kGeoRows) * 8u * uKp);
-    const auto r_nbr = make_rsrc(sp.nbr, 12u * uKp);
-    const auto r_code = make_rsrc(sp.code, 4u * uKp);
-    const uint32_t code = __builtin_amdgcn_raw_buffer_load_b32(r_code, unsigned(ec) * 4u, 0, 0);
-    int nbf[3];
-#pragma unroll
-    for (int f = 0; f < 3; ++f)
-        nbf[f] = int(__builtin_amdgcn_raw_buffer_load_b32(r_nbr, unsigned(ec) * 4u, unsigned(f) * 4u * uKp, 0));
-    auto slot_off = [&](int s, unsigned el) -> unsigned {
-        const int m = mf_mode(R, NP, g, s);
-        return m >= 0 ? (unsigned(m) * uKp + el) * 16u : state_bytes;
-    };
-    float b[4][R];
-#pragma unroll
-    for (int s = 0; s < R; ++s) {
-        const v4u x = load_x4(r_uk, slot_off(s, unsigned(ec)));
-#pragma unroll
-        for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
-    }
-    constexpr int kGeoPerThread = (kGeoRows * EB + kBlock - 1) / kBlock;
-    double geov[kGeoPerThread];
-#pragma unroll
-    for (int k = 0; k < kGeoPerThread; ++k) {
-        const int t = int(threadIdx.x) + k * kBlock;
-        const int row = t / EB, el = min(eb0 + t % EB, sp.K - 1);
-        geov[k] = t < kGeoRows * EB ? load_f64(r_geo, unsigned(el) * 8u, unsigned(row) * 8u * uKp) : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < kTabPerThread; ++k) {
-        const int t = int(threadIdx.x) + k * kBlock;
-        if (t < S::TOTAL_M) s_tab[t] = tabv[k];
-    }
-#pragma unroll
-    for (int k = 0; k < kGeoPerThread; ++k) {
-        const int t = int(threadIdx.x) + k * kBlock;
-        if (t < kGeoRows * EB) s_geo[t / EB][t % EB] = geov[k];
+        if (t < S::TOTAL_M) tab[t] = tmv[k];
     }
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
 
-    const int eloc = 16 * w + j;
-    double gd[kGeoRows];
-#pragma unroll
-    for (int k = 0; k < kGeoRows; ++k) gd[k] = s_geo[k][eloc];
-    const float* lt = &s_tab[lane];
-    const double det = gd[G_DET];
-    const double rdet = 1.0 / det;
-    // gradients use dxi_dx; the contravariant fluxes carry |J| in residual mode (linear in dxi_dx)
-    const float A00 = float(gd[G_A00]), A01 = float(gd[G_A01]), A10 = float(gd[G_A10]), A11 = float(gd[G_A11]);
-    const double sv = residual_mode ? det : 1.0;
-    const float a00 = float(sv * gd[G_A00]), a01 = float(sv * gd[G_A01]);
-    const float a10 = float(sv * gd[G_A10]), a11 = float(sv * gd[G_A11]);
-
-    // per face: outward normal, lift scale, and the penalty sigma0 mu (p+1)^2 / max(h, 1e-12)
-    // (navier_stokes_weak_formulation.cpp:275-278), the latter as an FP32 reciprocal
-    const float sig0 = float(sp.sigma0 * sp.mu * (P + 1) * (P + 1));
-    float fnx[3], fny[3], fsf[3], fsg[3];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        fnx[f] = float(gd[G_N0X + 2 * f]);
-        fny[f] = float(gd[G_N0X + 2 * f + 1]);
-        fsf[f] = residual_mode ? float(gd[G_L0 + f]) : float(gd[G_L0 + f] * rdet);
-        fsg[f] = sig0 * rcp(fmaxf(float(gd[G_L0 + f]), 1e-12f));
-    }
-
-    // ---------------- volume ----------------------------------------------------------------
+    // ---------------- 3. volume ------------------------------------------------------------
+    // gradients use dxi_dx; the contravariant fluxes carry |J| in residual mode
+    const float A00 = A[0][0], A01 = A[0][1], A10 = A[1][0], A11 = A[1][1];
     // The projection contracts one register slot per product, so slot r's products follow its
-    // fluxes directly: one slot of fluxes is live at a time (same accumulation order per
-    // variable as slot-by-slot after all fluxes).
+    // fluxes directly: one slot of fluxes is live at a time.
     v4f acc[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[v] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
@@ -299,7 +435,7 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
                 gq[v * 2 + 1] = A10 * GX[v][r] + A11 * GY[v][r];
             }
             float Fv[4], Gv[4];
-            viscous_flux(Uq, gq, gam, mu, kappa, Fv, Gv);
+            viscous_flux_l(Uq, gq, gam, mu, lam, kappa, Fv, Gv);
             float fh[4], gh[4];
             fh[0] = Uq[0] * c1 - (a00 * Fv[0] + a10 * Gv[0]);
             gh[0] = Uq[0] * c2 - (a01 * Fv[0] + a11 * Gv[0]);
@@ -316,74 +452,64 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             }
         }
     }
-    // face records of this lane's slots: its own, and the neighbour's at the matching point
-    FaceRecord ro[4], rn[4];
+    // ---------------- 4. faces -------------------------------------------------------------
+    // per face: lift scale and twice the penalty 2 sigma0 mu (p+1)^2 / max(h, 1e-12)
+    // (navier_stokes_weak_formulation.cpp:275-278), the latter from an FP32 reciprocal of the
+    // face length (both sides of a face hold the same length)
+    const float sig0 = sp.f.sig0;
+    float fsg2[3];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        const int fp = mf_fpoint(NFQ, g, f);
-        const int fpc = fp < 0 ? 0 : fp;
-        const int face = fpc / NFQ, q = fpc % NFQ;
-        ro[f] = load_record(unsigned(fpc) * uKp + unsigned(ec));
-        const uint32_t c = (code >> (4 * face)) & 15u;
-        const int fn = int(c & 3u);
-        const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
-        const bool interior = fn != int(kFaceBoundary) && fp >= 0;
-        const int qq = ((c >> 2) & 1u) ? NFQ - 1 - q : q;
-        rn[f] = load_record(unsigned(interior ? fn * NFQ + qq : 0) * uKp + unsigned(interior ? nb : ec));
-    }
-
-    // ---------------- faces --------------------------------------------------------------
+    for (int f = 0; f < 3; ++f) fsg2[f] = 2.0f * (sig0 * rcp(fmaxf(flen[f], 1e-12f)));
+    (void)residual_mode;
     // Interior faces first, for every slot and without branches (a boundary slot computes a
-    // finite value from its own record that the pass after it replaces; a padding slot is
-    // selected away); then the boundary slots, only in waves that have one.
+    // value from its own record that the pass after it replaces; a padding slot is selected
+    // away); then the boundary slots, only in waves that have one.
     v4f Hs[4];
     bool any_boundary = false;
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-        const int fp = mf_fpoint(NFQ, g, f);
-        const bool slot_valid = fp >= 0;
-        const int face = slot_valid ? fp / NFQ : 0;
-        const uint32_t c = (code >> (4 * face)) & 15u;
-        const bool isL = (c >> 3) & 1u;
-        any_boundary |= slot_valid && (c & 3u) == kFaceBoundary;
-        const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
-        const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
-        const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
-        const float sigma = face == 0 ? fsg[0] : face == 1 ? fsg[1] : fsg[2];
-        const float Uo[4] = {ro[f].u.x, ro[f].u.y, ro[f].u.z, ro[f].u.w};
-        const float vo[4] = {ro[f].v.x, ro[f].v.y, ro[f].v.z, ro[f].v.w};
-        const float Un[4] = {rn[f].u.x, rn[f].u.y, rn[f].u.z, rn[f].u.w};
-        const float vn[4] = {rn[f].v.x, rn[f].v.y, rn[f].v.z, rn[f].v.w};
-        // canonical orientation: A side = the L element, normal = the L-side outward normal;
-        // Fn = 1/2 Fv_L.n -+ 1/2 Fv_R.n - sigma (U_R - U_L) (:192-196; the reference's half
-        // difference, or the average) with the own Fv.n_L = sgn vo and the neighbour's =
-        // -sgn vn, so both sides form identical numbers
-        const float sgn = isL ? 1.0f : -1.0f;
-        const float wL = sp.ns_avg_ref ? 0.5f : -0.5f, wR = -0.5f;
-        float Aq[4], Bq[4], H[4], Fn[4];
+        const bool slot_valid = f < 3 ? g < NFQ : (NFQ == 5 && g < 3);
+        if (f == 3 && NFQ < 5) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            Aq[v] = isL ? Uo[v] : Un[v];
-            Bq[v] = isL ? Un[v] : Uo[v];
-            const float fo = sgn * vo[v], fnb = -sgn * vn[v];
-            const float fL = isL ? fo : fnb, fR = isL ? fnb : fo;
-            Fn[v] = wL * fL + wR * fR - sigma * (Bq[v] - Aq[v]);
+            for (int v = 0; v < 4; ++v) Hs[v][f] = 0.0f;
+            continue;
         }
-        rusanov(Aq, Bq, sgn * nx, sgn * ny, gam, H);
-        const float hs = sgn * sf;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Hs[v][f] = slot_valid ? hs * (H[v] + Fn[v]) : 0.0f;
+        const int face = f < 3 ? f : (g < 3 ? g : 0);
+        const int q = f < 3 ? (g < NFQ ? g : 0) : 4;
+        const uint32_t c = conn_field(cw[0], cw[1], face);
+        const bool bnd = (c & kConnBoundary) != 0, isL = (c >> 13) & 1u;
+        any_boundary |= slot_valid && bnd;
+        const int ko = (face * NFQ + q) * EB + eloc;
+        const uint32_t stepc = (c >> 11) & 3u;
+        const int step = stepc == 0u ? EB : stepc == 1u ? -EB : 1;
+        const int kn = bnd ? ko : int(c & 2047u) + q * step;
+        const float4 uo = H.U[ko], go = H.G[ko];
+        const float so = H.S[ko];
+        const float4 un = H.U[kn], gn = H.G[kn];
+        const float sn = H.S[kn];
+        const float sw = ref_max(isL ? so : sn, isL ? sn : so) +
+                         (face == 0 ? fsg2[0] : face == 1 ? fsg2[1] : fsg2[2]);
+        const float hs = 0.5f * (face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2]);
+        const float X0 = (go.x - gn.x) - sw * (un.x - uo.x);
+        const float X1 = (go.y - gn.y) - sw * (un.y - uo.y);
+        const float X2 = (go.z - gn.z) - sw * (un.z - uo.z);
+        const float X3 = (go.w - gn.w) - sw * (un.w - uo.w);
+        Hs[0][f] = slot_valid ? hs * X0 : 0.0f;
+        Hs[1][f] = slot_valid ? hs * X1 : 0.0f;
+        Hs[2][f] = slot_valid ? hs * X2 : 0.0f;
+        Hs[3][f] = slot_valid ? hs * X3 : 0.0f;
     }
     if (__builtin_amdgcn_ballot_w64(any_boundary) != 0) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
-            const int fp = mf_fpoint(NFQ, g, f);
-            if (fp < 0) continue;
-            const int face = fp / NFQ;
-            const uint32_t c = (code >> (4 * face)) & 15u;
-            if ((c & 3u) != kFaceBoundary) continue;
-            const int nb = face == 0 ? nbf[0] : face == 1 ? nbf[1] : nbf[2];
-            const double* bcr = bc_row(sp.bc, nb, NFQ, fp % NFQ);
+            if (f == 3 && NFQ < 5) continue;
+            const bool slot_valid = f < 3 ? g < NFQ : (NFQ == 5 && g < 3);
+            const int face = f < 3 ? f : (g < 3 ? g : 0);
+            const uint32_t c = conn_field(cw[0], cw[1], face);
+            if (!slot_valid || !(c & kConnBoundary)) continue;
+            const int q = f < 3 ? g : 4;
+            const int bf = reinterpret_cast<const int2*>(sp.ext)[xs + int(c & 2047u)].x;
+            const double* bcr = bc_row(sp.bc, bf, NFQ, q);
             if (bcr[0] < 0.0) {  // no condition: the face contributes nothing
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Hs[v][f] = 0.0f;
@@ -391,14 +517,16 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             }
             const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
             const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
-            const float sf = face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2];
-            const float sigma = face == 0 ? fsg[0] : face == 1 ? fsg[1] : fsg[2];
-            const float Uo[4] = {ro[f].u.x, ro[f].u.y, ro[f].u.z, ro[f].u.w};
-            const float vo[4] = {ro[f].v.x, ro[f].v.y, ro[f].v.z, ro[f].v.w};
-            // ghost state and viscous boundary flux Fv.n - sigma (U_bc - U) (:249-264), U_bc the
-            // condition's data at the point itself, taken as conserved except for NO_SLIP
-            float Un[4], Ub[4], H[4];
-            ghost_state(Uo, nx, ny, bcr, gam, Un);
+            const float hs = 0.5f * (face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2]);
+            const float sg2 = face == 0 ? fsg2[0] : face == 1 ? fsg2[1] : fsg2[2];
+            const int ko = (face * NFQ + q) * EB + eloc;
+            const float4 uo = H.U[ko], go = H.G[ko];
+            const float Uo[4] = {uo.x, uo.y, uo.z, uo.w};
+            const float Go[4] = {go.x, go.y, go.z, go.w};
+            // ghost state and the condition's state U_bc at the point (:249-264), taken as
+            // conserved except for NO_SLIP
+            float Ug[4], Ub[4], Fg[4], sg;
+            ghost_state(Uo, nx, ny, bcr, gam, Ug);
             if (int(bcr[0]) == DGX_BC_NO_SLIP_WALL) {
                 const float Wb[4] = {float(bcr[5]), float(bcr[6]), float(bcr[7]), float(bcr[8])};
                 prim2cons(Wb, gam, Ub);
@@ -406,21 +534,22 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Ub[v] = float(bcr[5 + v]);
             }
-            rusanov(Uo, Un, nx, ny, gam, H);
-            const float wB = sp.ns_avg_ref ? 1.0f : -1.0f;
+            trace_record(Ug, nx, ny, gam, Fg, sg);
+            const float s = ref_max(H.S[ko], sg);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Hs[v][f] = sf * (H[v] + (wB * vo[v] - sigma * (Ub[v] - Uo[v])));
+            for (int v = 0; v < 4; ++v) Hs[v][f] = hs * ((Go[v] + Fg[v]) - s * (Ug[v] - Uo[v]) - sg2 * (Ub[v] - Uo[v]));
         }
     }
+    // lift: acc += LF^T H
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int t = 0; t < RF; ++t) acc[v] = mfma16x16x4(lt[(S::M_LF + t) * 64], Hs[v][t], acc[v]);
 
     // ---------------- output ---------------------------------------------------------------
-    if (!valid) return;
+    if (!valid) return;  // (no barrier follows inside the tile)
     const auto r_out = make_rsrc(sp.out, state_bytes);
-    const float dt = float(sp.dt), ca = float(sp.ca), cb = float(sp.cb);
+    const float dt = sp.f.dt, ca = sp.f.ca, cb = sp.f.cb;
     const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
     float unv[4][R];
     if (rk_blend) {
@@ -443,45 +572,56 @@ __device__ __forceinline__ void ns_stage_body(const StageParams& sp) {
             else o = ca * unv[v][r] + cb * (b[v][r] + dt * acc[v][r]);
             x[v] = __float_as_uint(o);
         }
+        // padding slots address past the end and are dropped
         __builtin_amdgcn_raw_buffer_store_b128(x, r_out, slot_off(r, unsigned(e)), 0, 0);
     }
 }
 
-// four waves per SIMD fit the face-trace kernels without spilling
-#define DGX_TRACE_KERNEL(NAME, P, QUIRK)                                                                  \
-    __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void NAME(StageParams sp) { \
-        ns_trace_body<P, QUIRK>(sp);                                                                        \
-    }
-DGX_TRACE_KERNEL(ns_trace_kernel_p3, 3, false)
-DGX_TRACE_KERNEL(ns_trace_kernel_p4, 4, false)
-DGX_TRACE_KERNEL(ns_trace_quirk_kernel_p3, 3, true)
-DGX_TRACE_KERNEL(ns_trace_quirk_kernel_p4, 4, true)
-#undef DGX_TRACE_KERNEL
-// the stage kernels fit three waves per SIMD without spilling when asked to (the default
-// schedule keeps more registers and runs two)
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void ns_stage_mfma_kernel_p3(StageParams sp) {
-    ns_stage_body<3>(sp);
+// dynamic LDS of the block: the trace records
+__device__ __forceinline__ float4* ns_records_lds() {
+    extern __shared__ float4 s_ns_rec[];
+    return s_ns_rec;
 }
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void ns_stage_mfma_kernel_p4(StageParams sp) {
-    ns_stage_body<4>(sp);
+
+template <int P, bool QUIRK>
+__device__ __forceinline__ void ns_tile_body(const StageParams& sp) {
+    __shared__ float4 s_region[NsTile<P>::REGION / 4];
+    ns_tile<P, QUIRK>(sp, xcd_tile(blockIdx.x, gridDim.x), reinterpret_cast<float*>(s_region), ns_records_lds());
+}
+
+// three waves per SIMD (the registers of either order; at p = 4 also three blocks per CU by LDS)
+#define DGX_NS_TILE_KERNEL(NAME, P, QUIRK, WAVES)                                                             \
+    __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void NAME(StageParams sp) { \
+        ns_tile_body<P, QUIRK>(sp);                                                                          \
+    }
+DGX_NS_TILE_KERNEL(ns_tile_kernel_p3, 3, false, 3)
+DGX_NS_TILE_KERNEL(ns_tile_kernel_p4, 4, false, 3)
+DGX_NS_TILE_KERNEL(ns_tile_quirk_kernel_p3, 3, true, 3)
+DGX_NS_TILE_KERNEL(ns_tile_quirk_kernel_p4, 4, true, 3)
+#undef DGX_NS_TILE_KERNEL
+
+// dynamic LDS: U, G (float4) and s (float) of the tile's own face points and of its external
+// faces' neighbour points (at least one face)
+size_t ns_records_bytes(int p, int halo_cap) {
+    const size_t nfq = size_t(p + 1);
+    const size_t n = 3 * nfq * kTileEl + size_t(halo_cap > 1 ? halo_cap : 1) * nfq;
+    return n * (2 * sizeof(float4) + sizeof(float));
 }
 
 }  // namespace
 
 hipError_t launch_stage_mfma_ns(int p, bool quirk, const StageParams& sp, hipStream_t s) {
     if (sp.K <= 0) return hipSuccess;
+    if (!sp.geo32 || !sp.conn || !sp.ext || !sp.ext_off || !sp.ext_nint || !sp.halo) return hipErrorInvalidValue;
     dim3 grid((sp.K + 63) / 64), block(kBlock);
+    const size_t lds = ns_records_bytes(p, sp.halo_cap);
     switch (p * 2 + (quirk ? 1 : 0)) {
-    case 6: hipLaunchKernelGGL(ns_trace_kernel_p3, grid, block, 0, s, sp); break;
-    case 7: hipLaunchKernelGGL(ns_trace_quirk_kernel_p3, grid, block, 0, s, sp); break;
-    case 8: hipLaunchKernelGGL(ns_trace_kernel_p4, grid, block, 0, s, sp); break;
-    case 9: hipLaunchKernelGGL(ns_trace_quirk_kernel_p4, grid, block, 0, s, sp); break;
+    case 6: hipLaunchKernelGGL(ns_tile_kernel_p3, grid, block, lds, s, sp); break;
+    case 7: hipLaunchKernelGGL(ns_tile_quirk_kernel_p3, grid, block, lds, s, sp); break;
+    case 8: hipLaunchKernelGGL(ns_tile_kernel_p4, grid, block, lds, s, sp); break;
+    case 9: hipLaunchKernelGGL(ns_tile_quirk_kernel_p4, grid, block, lds, s, sp); break;
     default: return hipErrorInvalidValue;
     }
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    if (p == 3) hipLaunchKernelGGL(ns_stage_mfma_kernel_p3, grid, block, 0, s, sp);
-    else hipLaunchKernelGGL(ns_stage_mfma_kernel_p4, grid, block, 0, s, sp);
     return hipGetLastError();
 }
 
