@@ -1480,54 +1480,117 @@ void to_host(dgx_ctx* c, const void* src, double* h) {
 
 void vjp(dgx_ctx* c, const void* uk, const void* lam, const void* add, const void* acc_in, void* acc_out, void* out,
          int mode, double dt, double ca, double cb) {
-    hip_check(launch_vjp(c->precision, c->p, c->rule, uk, lam, add, acc_in, acc_out, out, c->geo.as<double>(),
-                         c->nbr.as<int32_t>(), c->code.as<uint32_t>(), mode == MODE_RESIDUAL ? c->tabR.p : c->tabL.p,
-                         c->bc.as<double>(), c->K, c->Kp, dt, ca, cb, c->gamma, mode, c->uref,
-                         mode == MODE_RESIDUAL ? c->tabVR.p : c->tabVL.p, c->stream),
-              "vjp kernel launch");
+    VjpParams vp{};
+    vp.uk = uk;
+    vp.lam = lam;
+    vp.add = add;
+    vp.acc_in = acc_in;
+    vp.acc_out = acc_out;
+    vp.out = out;
+    vp.geo = c->geo.as<double>();
+    vp.nbr = c->nbr.as<int32_t>();
+    vp.code = c->code.as<uint32_t>();
+    vp.tab = mode == MODE_RESIDUAL ? c->tabR.p : c->tabL.p;
+    vp.bc = c->bc.as<double>();
+    vp.K = c->K;
+    vp.Kp = c->Kp;
+    vp.dt = dt;
+    vp.ca = ca;
+    vp.cb = cb;
+    vp.gamma = c->gamma;
+    vp.mode = mode;
+    for (int v = 0; v < 4; ++v) vp.uref[v] = c->uref.v[v];
+    vp.tabm = mode == MODE_RESIDUAL ? c->tabVR.p : c->tabVL.p;
+    c->set_vjp_tile_params(vp, mode);
+    hip_check(launch_vjp(c->precision, c->p, c->rule, vp, c->stream), "vjp kernel launch");
 }
 
 struct AdjointBuffers {
-    std::vector<DevBuf> ckpt, stages;
+    std::vector<DevBuf> ckpt, stages[2];
     DevBuf l0, l1, l2, acc;
+    hipEvent_t ev[5] = {};  // forward done; recomputation of stage set 0 / 1 done; reverse steps on set 0 / 1 done
+    ~AdjointBuffers() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 // Reverse sweep of nsteps SSP-RK3 steps from the state in c->u with checkpoints every
 // `every` steps; `lam` holds lambda_T on entry (device, in b.l0) and lambda_0 on exit.
+// Segment j's stage states are recomputed from its checkpoint into stage set j mod 2 on the side
+// stream while the reverse steps of segment j + 1 run on the context's stream from the other
+// set: the recomputation of one segment overlaps the stage adjoints of the next, which are
+// latency-bound launches that leave most of the GPU idle.  The forward pass keeps the last two
+// segments' stage states in the sets as it goes.
 void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every, hipEvent_t mid = nullptr) {
     const size_t sb = c->state_bytes();
     const int nck = (nsteps + every - 1) / every;
     if (b.ckpt.size() < (size_t)nck) b.ckpt.resize(nck);
     for (int j = 0; j < nck; ++j) if (!b.ckpt[j].p) b.ckpt[j].alloc(sb);
-    if (b.stages.size() < (size_t)(3 * every + 1)) b.stages.resize(3 * every + 1);
-    for (auto& s : b.stages) if (!s.p) s.alloc(sb);
-    // forward with checkpoints
-    for (int s = 0; s < nsteps; ++s) {
-        if (s % every == 0)
-            hip_check(hipMemcpyAsync(b.ckpt[s / every].p, c->u.p, sb, hipMemcpyDeviceToDevice, c->stream), "ckpt");
-        c->rk3_step(dt);
+    for (auto& set : b.stages) {
+        if (set.size() < (size_t)(3 * every + 1)) set.resize(3 * every + 1);
+        for (auto& s : set) if (!s.p) s.alloc(sb);
     }
-    if (mid) hip_check(hipEventRecord(mid, c->stream), "record");
-    // reverse sweep, segment by segment
-    for (int j = nck - 1; j >= 0; --j) {
+    for (hipEvent_t& e : b.ev)
+        if (!e) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    hipStream_t A = c->stream, B = c->side_stream();
+    // forward with checkpoints; the last two segments keep their stage states in the two stage
+    // sets instead (the reverse sweep starts on them without recomputation)
+    const int kept = std::min(nck, 2);
+    for (int j = 0; j < nck; ++j) {
         const int s0 = j * every, len = std::min(every, nsteps - s0);
-        hip_check(hipMemcpyAsync(b.stages[0].p, b.ckpt[j].p, sb, hipMemcpyDeviceToDevice, c->stream), "restore");
+        if (j < nck - kept) {
+            hip_check(hipMemcpyAsync(b.ckpt[j].p, c->u.p, sb, hipMemcpyDeviceToDevice, A), "ckpt");
+            for (int k = 0; k < len; ++k) c->rk3_step(dt);
+            continue;
+        }
+        std::vector<DevBuf>& st = b.stages[j & 1];
+        hip_check(hipMemcpyAsync(st[0].p, c->u.p, sb, hipMemcpyDeviceToDevice, A), "segment start");
         for (int k = 0; k < len; ++k) {
-            void* v0 = b.stages[3 * k].p;
-            void* v1 = b.stages[3 * k + 1].p;
-            void* v2 = b.stages[3 * k + 2].p;
+            void* v0 = st[3 * k].p;
+            void* v1 = st[3 * k + 1].p;
+            void* v2 = st[3 * k + 2].p;
             c->stage_launch(v0, v0, v1, MODE_RK, dt, 0.0, 1.0);
             c->stage_launch(v1, v0, v2, MODE_RK, dt, 0.75, 0.25);
-            if (k + 1 < len) c->stage_launch(v2, v0, b.stages[3 * k + 3].p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
+            c->stage_launch(v2, v0, k + 1 < len ? st[3 * k + 3].p : c->u.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
         }
+    }
+    if (mid) hip_check(hipEventRecord(mid, A), "record");
+    hip_check(hipEventRecord(b.ev[0], A), "record");
+    for (int set = 0; set < 2; ++set) hip_check(hipEventRecord(b.ev[1 + set], A), "record");  // the kept sets
+    hip_check(hipStreamWaitEvent(B, b.ev[0], 0), "wait");
+    // stage states of segment j into its set, on the side stream
+    bool used[2] = {false, false};  // a reverse step has read the set
+    auto recompute = [&](int j) {
+        const int set = j & 1, s0 = j * every, len = std::min(every, nsteps - s0);
+        std::vector<DevBuf>& st = b.stages[set];
+        if (used[set]) hip_check(hipStreamWaitEvent(B, b.ev[3 + set], 0), "wait");
+        hip_check(hipMemcpyAsync(st[0].p, b.ckpt[j].p, sb, hipMemcpyDeviceToDevice, B), "restore");
+        for (int k = 0; k < len; ++k) {
+            void* v0 = st[3 * k].p;
+            void* v1 = st[3 * k + 1].p;
+            void* v2 = st[3 * k + 2].p;
+            c->stage_launch(v0, v0, v1, MODE_RK, dt, 0.0, 1.0, B);
+            c->stage_launch(v1, v0, v2, MODE_RK, dt, 0.75, 0.25, B);
+            if (k + 1 < len) c->stage_launch(v2, v0, st[3 * k + 3].p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0, B);
+        }
+        hip_check(hipEventRecord(b.ev[1 + set], B), "record");
+    };
+    for (int j = nck - 1; j >= 0; --j) {
+        if (j > 0 && j - 1 < nck - kept) recompute(j - 1);
+        const int set = j & 1, s0 = j * every, len = std::min(every, nsteps - s0);
+        std::vector<DevBuf>& st = b.stages[set];
+        hip_check(hipStreamWaitEvent(A, b.ev[1 + set], 0), "wait");
         for (int k = len - 1; k >= 0; --k) {
-            void* v0 = b.stages[3 * k].p;
-            void* v1 = b.stages[3 * k + 1].p;
-            void* v2 = b.stages[3 * k + 2].p;
+            void* v0 = st[3 * k].p;
+            void* v1 = st[3 * k + 1].p;
+            void* v2 = st[3 * k + 2].p;
             vjp(c, v2, b.l0.p, nullptr, nullptr, b.acc.p, b.l1.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
             vjp(c, v1, b.l1.p, nullptr, b.acc.p, b.acc.p, b.l2.p, MODE_RK, dt, 0.75, 0.25);
             vjp(c, v0, b.l2.p, b.acc.p, nullptr, nullptr, b.l0.p, MODE_RK, dt, 0.0, 1.0);
         }
+        hip_check(hipEventRecord(b.ev[3 + set], A), "record");
+        used[set] = true;
     }
 }
 

@@ -107,6 +107,7 @@ struct dgx_ctx {
     static constexpr Off4 kNoOff{};
     size_t rsz = 8;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // side_stream(), created on first use
     DevBuf u, u1, u2, geo, nbr, code, hmin, bc, tabL, tabR, tab64, minv, pm, qw, stage, partial;
     DevBuf tabML, tabMR;  // MFMA stage kernel lane tables (RK/RHS and residual variants)
     DevBuf tabVL, tabVR;  // MFMA stage adjoint lane tables (same two variants)
@@ -148,6 +149,7 @@ struct dgx_ctx {
             b->release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -174,7 +176,15 @@ struct dgx_ctx {
         upload(b, t.data(), t.size() * sizeof(T));
     }
 
-    void stage_launch(const void* uk, const void* un, void* out, int mode, double dt, double ca, double cb) {
+    // a second stream of default priority (the adjoint's recomputation beside its reverse sweep)
+    hipStream_t side_stream() {
+        if (!stream2) hip_check(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "hipStreamCreate");
+        return stream2;
+    }
+
+    void stage_launch(const void* uk, const void* un, void* out, int mode, double dt, double ca, double cb,
+                      hipStream_t on = nullptr) {
+        const hipStream_t stream = on ? on : this->stream;
         StageParams sp{};
         sp.uk = uk;
         sp.un = un;
@@ -235,6 +245,22 @@ struct dgx_ctx {
         sp.ext_nint = ext_nint.as<int32_t>();
         sp.halo = halo.as<int32_t>();
         sp.halo_cap = halo_cap;
+    }
+
+    // the matrix-core stage adjoint's tile arrays and FP32 scalars (null arrays without them)
+    void set_vjp_tile_params(VjpParams& vp, int mode) const {
+        vp.geo32 = (mode == MODE_RESIDUAL ? geo32R : geo32L).as<float>();
+        vp.conn = conn.as<uint32_t>();
+        vp.ext = ext.as<int32_t>();
+        vp.ext_off = ext_off.as<int32_t>();
+        vp.ext_nint = ext_nint.as<int32_t>();
+        vp.halo = halo.as<int32_t>();
+        vp.halo_cap = halo_cap;
+        vp.f.gamma = float(vp.gamma);
+        vp.f.dt = float(vp.dt);
+        vp.f.ca = float(vp.ca);
+        vp.f.cb = float(vp.cb);
+        for (int v = 0; v < 4; ++v) vp.f.uref[v] = float(vp.uref[v]);
     }
 
     // One SSP-RK3 step (SSP_RK::step_rk3, I/solver/time_stepping.hpp:39-55):

@@ -244,16 +244,25 @@ struct VjpParams {
     int mode;
     double uref[4];    // point-value storage offset of the state (zero in FP64)
     const void* tabm;  // lane tables of the matrix-core VJP kernel (FP32 Euler), or null
+    // matrix-core tile kernel (null elsewhere): the forward tile kernel's FP32 geometry of this
+    // mode, face connections and external-face lists (StageParams), and its scalars rounded to
+    // FP32 on the host
+    const float* geo32;
+    const uint32_t* conn;
+    const int32_t* ext;
+    const int32_t* ext_off;
+    const int32_t* ext_nint;
+    const int32_t* halo;
+    int halo_cap;
+    struct F32 {
+        float gamma, dt, ca, cb, uref[4];
+    } f;
 };
 
 // Euler stage adjoint (dgx_vjp.hip): out = cb (lam + dt (dL/du)^T lam) + add,
 // acc_out = acc_in + ca lam (mode RK); out = (dL/du)^T lam (mode RHS) or (dR/du)^T lam
 // (mode RESIDUAL, unfolded tables).
-hipError_t launch_vjp(int precision, int p, int rule, const void* uk, const void* lam, const void* add,
-                      const void* acc_in, void* acc_out, void* out, const double* geo, const int32_t* nbr,
-                      const uint32_t* code, const void* tab, const double* bc, int K, int Kp, double dt,
-                      double ca, double cb, double gamma, int mode, Off4 uref, const void* tabm,
-                      hipStream_t s);
+hipError_t launch_vjp(int precision, int p, int rule, const VjpParams& vp, hipStream_t s);
 // Batched persistent stage (dgx_batch.hip): items of (instance, first element, count, order);
 // params[instance] holds that instance's stage parameters; tabs[p-1] the order-p tables.
 struct BatchItem {

@@ -57,9 +57,9 @@ struct MfmaShape {
 //   EF[R]  Phi_F  (rows: face slots, k: modes)       own traces
 //   LE[R]  LF     (rows: face slots, k: modes)       own lift weights of lambda
 //   FT[RF] Phi_F  (rows: modes, k: face slots)       transposed trace evaluation
-// then NT[fn*2 + flip][i][q] (Phi_F of the neighbour, as for the forward kernel) and
-// NL[fn*2 + flip][i][q] (its LF).  Padding rows of EV / EF repeat point 0 (finite states);
-// every other padding entry is zero.
+// then the external-face tables: per point q of face f (index f Nfq + q) HV float4, entry m
+// holding (Phi_F, LF) of modes 2m and 2m + 1 at the point (zeros past the last mode).  Padding
+// rows of EV / EF repeat point 0 (finite states); every other padding entry is zero.
 template <int P>
 struct MfmaVjpShape {
     using F = MfmaShape<P>;
@@ -72,8 +72,9 @@ struct MfmaVjpShape {
     static constexpr int T_LE = T_EF + R;
     static constexpr int T_FT = T_LE + R;
     static constexpr int T_N = T_FT + RF;
-    static constexpr int NT = F::NT;
-    static constexpr int TOTAL = T_N * 64 + 2 * NT;
+    static constexpr int NP = F::NP, NFQ = F::NFQ;
+    static constexpr int HV = (NP + 1) / 2;        // float4 per external-face table point
+    static constexpr int TOTAL = T_N * 64 + 3 * NFQ * HV * 4;
 };
 
 // Tables of the matrix-core Navier-Stokes tile kernel (dgx_stage_mfma_ns.hip).

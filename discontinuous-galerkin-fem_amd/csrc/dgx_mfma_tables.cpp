@@ -122,17 +122,13 @@ void build_vjp(const std::vector<double>& t, std::vector<float>& out) {
             at(S::T_FT + r, lane) = (mrow >= 0 && f >= 0) ? float(t[L.phiF + f * np + mrow]) : 0.0f;
         }
     }
-    float* nt = &out[(size_t)S::T_N * 64];
-    float* nl = nt + S::NT;
-    for (int fn = 0; fn < 3; ++fn)
-        for (int flip = 0; flip < 2; ++flip)
-            for (int m = 0; m < np; ++m)
-                for (int q = 0; q < nfq; ++q) {
-                    const int qq = flip ? nfq - 1 - q : q;
-                    const size_t o = ((size_t)(fn * 2 + flip) * np + m) * nfq + q;
-                    nt[o] = float(t[L.phiF + (fn * nfq + qq) * np + m]);
-                    nl[o] = float(t[L.LF + (fn * nfq + qq) * np + m]);
-                }
+    float* h = &out[(size_t)S::T_N * 64];
+    for (int fp = 0; fp < 3 * nfq; ++fp)
+        for (int m = 0; m < np; ++m) {
+            float* x = &h[((size_t)fp * S::HV + m / 2) * 4 + 2 * (m & 1)];
+            x[0] = float(t[L.phiF + fp * np + m]);
+            x[1] = float(t[L.LF + fp * np + m]);
+        }
 }
 
 // Populated face-gradient slot of (point fp, component c, mode i) and its two values T_ck,

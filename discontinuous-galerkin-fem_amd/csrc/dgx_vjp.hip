@@ -278,15 +278,9 @@ hipError_t launch_vjp_real(int p, int rule, const VjpParams& vp, hipStream_t s) 
 
 }  // namespace
 
-hipError_t launch_vjp(int precision, int p, int rule, const void* uk, const void* lam, const void* add,
-                      const void* acc_in, void* acc_out, void* out, const double* geo, const int32_t* nbr,
-                      const uint32_t* code, const void* tab, const double* bc, int K, int Kp, double dt,
-                      double ca, double cb, double gamma, int mode, Off4 uref, const void* tabm,
-                      hipStream_t s) {
-    if (K <= 0) return hipSuccess;
-    VjpParams vp{uk, lam, add, acc_in, acc_out, out, geo, nbr, code, tab, bc, K, Kp, dt, ca, cb, gamma, mode,
-                 {uref.v[0], uref.v[1], uref.v[2], uref.v[3]}, tabm};
-    if (tabm && precision == DGX_FP32 && rule == 0 && mfma_order(p)) return launch_vjp_mfma(p, vp, s);
+hipError_t launch_vjp(int precision, int p, int rule, const VjpParams& vp, hipStream_t s) {
+    if (vp.K <= 0) return hipSuccess;
+    if (vp.tabm && vp.conn && precision == DGX_FP32 && rule == 0 && mfma_order(p)) return launch_vjp_mfma(p, vp, s);
     return precision == DGX_FP64 ? launch_vjp_real<double>(p, rule, vp, s) : launch_vjp_real<float>(p, rule, vp, s);
 }
 

@@ -10,6 +10,16 @@ template <typename Real>
 __device__ __forceinline__ Real rcp_(Real x) { return Real(1) / x; }
 template <>
 __device__ __forceinline__ float rcp_<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+// a / b and sqrt(x): correctly rounded in FP64; in FP32 the hardware reciprocal and square root
+// (1 ulp, v_rcp_f32 / v_sqrt_f32), as the forward FP32 kernels evaluate them
+template <typename Real>
+__device__ __forceinline__ Real div_(Real a, Real b) { return a / b; }
+template <>
+__device__ __forceinline__ float div_<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename Real>
+__device__ __forceinline__ Real sqrt_(Real x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float sqrt_<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ __forceinline__ int xcd_tile_(int bid, int nblk) {
     if ((nblk & 7) != 0) return bid;
@@ -33,10 +43,10 @@ __device__ __forceinline__ void flux_dir_adjoint(const Real* U, Real nx, Real ny
     Real rbar = psi[0] * un, mxbar = psi[1] * un, mybar = psi[2] * un, Ebar = psi[3] * un;
     if (sbar != Real(0)) {
         // s = |un| + c, c = sqrt(g p / r)
-        Real c = sqrt(g * p * ri);
+        Real c = sqrt_(g * p * ri);
         unbar += sbar * (un >= Real(0) ? Real(1) : Real(-1));
-        pbar += sbar * g / (Real(2) * c * r);
-        rbar -= sbar * c / (Real(2) * r);
+        pbar += div_(sbar * g, Real(2) * c * r);
+        rbar -= div_(sbar * c, Real(2) * r);
     }
     Real ubar = unbar * nx, vbar = unbar * ny;
     Ebar += gm1 * pbar;
@@ -57,15 +67,15 @@ __device__ __forceinline__ Real wave_speed(const Real* U, Real nx, Real ny, Real
     Real ri = rcp_(U[0]);
     Real u = U[1] * ri, v = U[2] * ri;
     Real p = (g - Real(1)) * (U[3] - Real(0.5) * U[0] * (u * u + v * v));
-    return fabs(u * nx + v * ny) + sqrt(g * p * ri);
+    return fabs(u * nx + v * ny) + sqrt_(g * p * ri);
 }
 
 // Ubar_own = (dH/dU_own)^T psi for H(A, B, n) = 1/2 (Fn(A) + Fn(B)) - 1/2 s (B - A),
 // s = max(sA, sB) (A branch at ties).  own_is_A selects the side.
+// (rusanov_adjoint_s: the two wave speeds given)
 template <typename Real>
-__device__ __forceinline__ void rusanov_adjoint(const Real* A, const Real* B, Real nx, Real ny,
-                                                const Real* psi, bool own_is_A, Real g, Real* Ubar) {
-    Real sA = wave_speed(A, nx, ny, g), sB = wave_speed(B, nx, ny, g);
+__device__ __forceinline__ void rusanov_adjoint_s(const Real* A, const Real* B, Real nx, Real ny, const Real* psi,
+                                                  bool own_is_A, Real sA, Real sB, Real g, Real* Ubar) {
     bool from_A = sA >= sB;
     Real s = from_A ? sA : sB;
     Real jump = psi[0] * (B[0] - A[0]) + psi[1] * (B[1] - A[1]) + psi[2] * (B[2] - A[2]) + psi[3] * (B[3] - A[3]);
@@ -76,6 +86,11 @@ __device__ __forceinline__ void rusanov_adjoint(const Real* A, const Real* B, Re
     Real sgn = own_is_A ? Real(0.5) : Real(-0.5);
 #pragma unroll
     for (int v = 0; v < 4; ++v) Ubar[v] += sgn * s * psi[v];
+}
+template <typename Real>
+__device__ __forceinline__ void rusanov_adjoint(const Real* A, const Real* B, Real nx, Real ny,
+                                                const Real* psi, bool own_is_A, Real g, Real* Ubar) {
+    rusanov_adjoint_s(A, B, nx, ny, psi, own_is_A, wave_speed(A, nx, ny, g), wave_speed(B, nx, ny, g), g, Ubar);
 }
 
 }  // namespace
