@@ -315,7 +315,7 @@ def adjoint_ratios(n, p, steps, every):
     run of the same steps, gradient of the final kinetic energy w.r.t. the initial state."""
     ctx, _, _, _ = make_ctx(n, p, "fp32", SEED)
     dt = dgx.stable_dt(ctx, 0.3)
-    ctx.rk3_vjp_profile(dt, 2 * every, every)  # warm-up: checkpoint buffers, code paths
+    ctx.rk3_vjp_profile(dt, 3 * every, every)  # warm-up: checkpoint buffers, side stream, code paths
     fwd, ckpt, rev = ctx.rk3_vjp_profile(dt, steps, every)
     out = {"workload": f"euler_vortex_adjoint_n{n}_p{p}", "elements": ctx.K, "steps": steps,
            "checkpoint_every": every, "forward_ms": fwd, "checkpointed_forward_ms": ckpt,

@@ -1680,10 +1680,11 @@ dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t ev
         DevBuf u0;
         u0.alloc(c->state_bytes());
         hip_check(hipMemcpyAsync(u0.p, c->u.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
-        // warm-up allocations of the checkpoint and stage stores with a one-segment sweep
+        // warm-up allocations of the checkpoint and stage stores, and the side stream, with a
+        // sweep of three segments (one recomputed)
         hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
                                  c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->uref, c->stream), "ke grad");
-        rk3_adjoint(c, b, dt, std::min(nsteps, every), every);
+        rk3_adjoint(c, b, dt, std::min(nsteps, 3 * every), every);
         hip_check(hipMemcpyAsync(c->u.p, u0.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
         // forward only
         hip_check(hipEventRecord(c->ev0, c->stream), "record");
