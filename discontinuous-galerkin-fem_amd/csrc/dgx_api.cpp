@@ -828,17 +828,20 @@ void build_tri_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc&
     const bool fits32 = big < 2147483648.0;
     if (precision == DGX_FP32_BF16VOL && !fits32)
         throw Fail(DGX_E_UNSUPPORTED, "mesh too large for the BF16-input volume variant (2 GiB buffers)");
-    if (precision != DGX_FP64 && c.physics == DGX_EULER && c.rule == 0 && mfma_order(c.p) && !lane_only && fits32) {
+    if (precision != DGX_FP64 && c.physics == DGX_EULER && c.rule == 0 && mfma_euler_order(c.p) && !lane_only &&
+        fits32) {
         const bool bf16 = precision == DGX_FP32_BF16VOL;
         std::vector<float> m;
         build_mfma_tables(c.p, tL, m, bf16);
         c.upload(c.tabML, m.data(), m.size() * sizeof(float));
         build_mfma_tables(c.p, tR, m, bf16);
         c.upload(c.tabMR, m.data(), m.size() * sizeof(float));
-        build_mfma_vjp_tables(c.p, tL, m);
-        c.upload(c.tabVL, m.data(), m.size() * sizeof(float));
-        build_mfma_vjp_tables(c.p, tR, m);
-        c.upload(c.tabVR, m.data(), m.size() * sizeof(float));
+        if (mfma_order(c.p)) {  // the matrix-core stage adjoint (p = 3, 4)
+            build_mfma_vjp_tables(c.p, tL, m);
+            c.upload(c.tabVL, m.data(), m.size() * sizeof(float));
+            build_mfma_vjp_tables(c.p, tR, m);
+            c.upload(c.tabVR, m.data(), m.size() * sizeof(float));
+        }
         build_tile_arrays(c, nbr);
     }
     if (precision == DGX_FP32 && c.physics == DGX_NAVIER_STOKES && c.rule == 0 && mfma_order(c.p) && !lane_only &&

@@ -124,7 +124,10 @@ __host__ __device__ constexpr int mf_fpoint(int nfq, int g, int r) {
     return r < 3 ? (g < nfq ? r * nfq + g : -1) : ((nfq == 5 && g < 3) ? g * nfq + 4 : -1);
 }
 
-// Orders with an MFMA stage kernel (FP32 Euler, volume rule exact to degree 2p).
+// Orders with a matrix-core FP32 Euler stage (tile) kernel, volume rule exact to degree 2p.
+constexpr bool mfma_euler_order(int p) { return p >= 1 && p <= 4; }
+// Orders with the matrix-core Navier-Stokes stage, the stage adjoint and the bfloat16 volume
+// variant.
 constexpr bool mfma_order(int p) { return p == 3 || p == 4; }
 int mfma_table_size(int p);  // floats
 // Build the lane-major tables of order p from the (L or R) tables in TabLayout order.

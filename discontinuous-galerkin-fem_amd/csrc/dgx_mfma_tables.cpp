@@ -227,6 +227,7 @@ void build_mfma_vjp_tables(int p, const std::vector<double>& tab, std::vector<fl
 
 int mfma_table_size(int p) {
     switch (p) {
+    case 1: return MfmaShape<1>::TOTAL;
     case 2: return MfmaShape<2>::TOTAL;
     case 3: return MfmaShape<3>::TOTAL;
     case 4: return MfmaShape<4>::TOTAL;
@@ -236,6 +237,7 @@ int mfma_table_size(int p) {
 
 void build_mfma_tables(int p, const std::vector<double>& tab, std::vector<float>& out, bool bf16) {
     switch (p) {
+    case 1: build<1>(tab, out, bf16); break;
     case 2: build<2>(tab, out, bf16); break;
     case 3: build<3>(tab, out, bf16); break;
     case 4: build<4>(tab, out, bf16); break;

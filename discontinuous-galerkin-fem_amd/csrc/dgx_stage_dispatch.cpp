@@ -15,7 +15,7 @@ hipError_t launch_stage(int precision, int physics, int p, int rule, const Stage
     if (sp.K <= 0) return hipSuccess;
     // FP32 Euler with the degree-2p volume rule runs on the matrix cores when its lane tables
     // were built for the context
-    if (sp.tabm && precision != DGX_FP64 && physics == DGX_EULER && rule == 0 && mfma_order(p))
+    if (sp.tabm && precision != DGX_FP64 && physics == DGX_EULER && rule == 0 && mfma_euler_order(p))
         return launch_stage_mfma(p, precision == DGX_FP32_BF16VOL, sp, s);
     const bool quirk = sp.ns_face_quirk != 0;
     // FP32 Navier-Stokes likewise (tile kernel with its own face-trace tables)

@@ -1,4 +1,4 @@
-// FP32 Euler stage kernel on the matrix cores (gfx950 v_mfma_f32_16x16x4f32), orders 3 and 4,
+// FP32 Euler stage kernel on the matrix cores (gfx950 v_mfma_f32_16x16x4f32), orders 1 to 4,
 // volume rule exact to degree 2p.  Same operator and output modes as the per-lane stage kernel
 // (dgx_stage_impl.hpp): volume residual + Rusanov faces + inverse mass + SSP-RK3 update.
 //
@@ -458,7 +458,13 @@ __device__ __forceinline__ void batch_stage_mfma_body(const StageParams* __restr
 }
 
 // waves per SIMD as measured best on gfx950: p = 3 is asked for three (it gets four), p = 4
-// for four
+// for four; p = 1, 2 for four
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void stage_mfma_kernel_p1(StageParams sp) {
+    stage_mfma_body<1, false>(sp);
+}
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void stage_mfma_kernel_p2(StageParams sp) {
+    stage_mfma_body<2, false>(sp);
+}
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void stage_mfma_kernel_p3(StageParams sp) {
     stage_mfma_body<3, false>(sp);
 }
@@ -471,6 +477,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
 }
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void stage_mfma_bf16_kernel_p4(StageParams sp) {
     stage_mfma_body<4, true>(sp);
+}
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void batch_stage_mfma_kernel_p1(
+    const StageParams* params, const BatchItem* items, int n_items) {
+    batch_stage_mfma_body<1>(params, items, n_items);
+}
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void batch_stage_mfma_kernel_p2(
+    const StageParams* params, const BatchItem* items, int n_items) {
+    batch_stage_mfma_body<2>(params, items, n_items);
 }
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_stage_mfma_kernel_p3(
     const StageParams* params, const BatchItem* items, int n_items) {
@@ -496,6 +510,8 @@ hipError_t launch_stage_mfma(int p, bool bf16vol, const StageParams& sp, hipStre
     dim3 grid((sp.K + 63) / 64), block(kBlock);
     const size_t lds = halo_bytes(p, sp.halo_cap);
     switch (p * 2 + (bf16vol ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL(stage_mfma_kernel_p1, grid, block, lds, s, sp); break;
+    case 4: hipLaunchKernelGGL(stage_mfma_kernel_p2, grid, block, lds, s, sp); break;
     case 6: hipLaunchKernelGGL(stage_mfma_kernel_p3, grid, block, lds, s, sp); break;
     case 7: hipLaunchKernelGGL(stage_mfma_bf16_kernel_p3, grid, block, lds, s, sp); break;
     case 8: hipLaunchKernelGGL(stage_mfma_kernel_p4, grid, block, lds, s, sp); break;
@@ -510,6 +526,8 @@ hipError_t launch_batch_stage_mfma(int p, const StageParams* params, const Batch
     if (n_items <= 0) return hipSuccess;
     const size_t lds = halo_bytes(p, halo_cap);
     switch (p) {
+    case 1: hipLaunchKernelGGL(batch_stage_mfma_kernel_p1, dim3(grid), dim3(kBlock), lds, s, params, items, n_items); break;
+    case 2: hipLaunchKernelGGL(batch_stage_mfma_kernel_p2, dim3(grid), dim3(kBlock), lds, s, params, items, n_items); break;
     case 3: hipLaunchKernelGGL(batch_stage_mfma_kernel_p3, dim3(grid), dim3(kBlock), lds, s, params, items, n_items); break;
     case 4: hipLaunchKernelGGL(batch_stage_mfma_kernel_p4, dim3(grid), dim3(kBlock), lds, s, params, items, n_items); break;
     default: return hipErrorInvalidValue;

@@ -129,8 +129,8 @@ typedef struct {
     int32_t n_boundary_faces;
     int32_t device;
     double bytes_per_stage_alg; /* algorithmic HBM bytes of one fused stage launch */
-    int32_t stage_kernel;       /* 0: one element per lane; 1: 16-element matrix-core tiles
-                                   (FP32 Euler and Navier-Stokes triangles, p = 3, 4); 2: four
+    int32_t stage_kernel;       /* 0: one element per lane; 1: 64-element matrix-core tiles
+                                   (FP32 triangles: Euler p = 1..4, Navier-Stokes p = 3, 4); 2: four
                                    lanes per element (Euler quadrilaterals, p <= 3);
                                    DGX_STAGE_KERNEL=lane at creation selects 0 */
 } dgx_info_t;

@@ -49,8 +49,9 @@ def test_batched_steps_match_single_context(layout, nsteps):
 @pytest.mark.parametrize("layout", ["fused", "per_order"])
 @pytest.mark.parametrize("nsteps", [4, 5])
 def test_batched_fp32_matches_single_context(layout, nsteps):
-    # FP32: orders 3 and 4 run the matrix-core stage in both the batch (tile work items) and
-    # the single context; 1 and 2 the per-lane stage.  The single context chooses its own
+    # FP32: every order runs the matrix-core stage in both the batch (tile work items, one
+    # launch per order) and the single context; the two layouts differ only for per-lane
+    # orders (FP64 above).  The single context chooses its own
     # storage offset, so the comparison is to FP32 rounding (1e-6 relative after the steps).
     b, meshes, dts = _make(SPECS, "fp32")
     b.set_tuning(layout, chunk=(1, 2, 4, 1), blocks_per_cu=1)
@@ -59,7 +60,7 @@ def test_batched_fp32_matches_single_context(layout, nsteps):
     got = [c.get_state() for c in b.instances]
     for (v, t), s, u, g, dt in zip(meshes, SPECS, u0, got, dts):
         ctx = dgx.Context(v, t, s["p"], precision="fp32")
-        assert ctx.info.stage_kernel == (1 if s["p"] >= 3 else 0)
+        assert ctx.info.stage_kernel == 1
         ctx.set_state(u)
         ctx.rk3_steps(dt, nsteps)
         ref = ctx.get_state()

@@ -1,4 +1,4 @@
-"""The matrix-core stage kernel (FP32 Euler, p = 3 and 4) against the one-element-per-lane
+"""The matrix-core stage kernel (FP32 Euler, p = 1 to 4) against the one-element-per-lane
 stage kernel and the FP64 oracle, on periodic and bounded meshes whose element counts are not
 multiples of the 16-element wave tile or the 64-element block.
 
@@ -64,7 +64,7 @@ def _setup(p, case, n=5):
     return ora, mf, ln, u0
 
 
-@pytest.mark.parametrize("p", [3, 4])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 @pytest.mark.parametrize("case", ["periodic", "far_field", "slip_wall", "no_slip_wall", "mixed"])
 def test_mfma_residual_matches_lane_kernel_and_oracle(p, case):
     ora, mf, ln, u0 = _setup(p, case)
@@ -75,7 +75,7 @@ def test_mfma_residual_matches_lane_kernel_and_oracle(p, case):
     assert e_mf < 2 * e_ln + 1e-6, (e_mf, e_ln)
 
 
-@pytest.mark.parametrize("p", [3, 4])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
 @pytest.mark.parametrize("case", ["periodic", "mixed"])
 def test_mfma_rk3_steps_match_lane_kernel(p, case):
     ora, mf, ln, u0 = _setup(p, case, n=7)
