@@ -15,7 +15,7 @@ struct dgx_batch {
     int order_mask = 0;
     int cus = 256;
     hipStream_t stream = nullptr;
-    dgx_batch_tuning tune{0, {4, 4, 2, 2}, 2};
+    dgx_batch_tuning tune{2, {4, 4, 4, 4}, 4};
     DevBuf params[6];                 // [window stage][instance] StageParams
     DevBuf items_all, items_p[4];     // work lists (fused, per order)
     int n_all = 0, n_p[4] = {0, 0, 0, 0};
@@ -26,6 +26,11 @@ struct dgx_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // layout 2: one stream per order (default priority), forked from and joined back into
+    // `stream` around each run of window stages, so that one order's launch tail overlaps the
+    // next order's launches
+    hipStream_t ostream[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[4] = {nullptr, nullptr, nullptr, nullptr};
     bool dirty = true;
 
     ~dgx_batch() {
@@ -33,6 +38,11 @@ struct dgx_batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (fork) (void)hipEventDestroy(fork);
+        for (int p = 0; p < 4; ++p) {
+            if (join[p]) (void)hipEventDestroy(join[p]);
+            if (ostream[p]) (void)hipStreamDestroy(ostream[p]);
+        }
         for (auto* c : inst) dgx_destroy(c);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -110,7 +120,7 @@ struct dgx_batch {
         if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
         if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        for (int w = 0; w < 6; ++w) launch_window_stage(w);
+        launch_window(0, 6);
         hip_check(hipStreamEndCapture(stream, &graph), "end capture");
         hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
         dirty = false;
@@ -125,13 +135,38 @@ struct dgx_batch {
         return cap;
     }
 
+    // the order-p launch of window stage w on stream s
+    void launch_order_stage(int w, int p, hipStream_t s) {
+        const StageParams* pr = static_cast<const StageParams*>(params[w].p);
+        if (tabm[p - 1])
+            hip_check(launch_batch_stage_mfma(p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1], halo_cap(p), grid(), s),
+                      "batch stage (matrix cores)");
+        else
+            hip_check(launch_batch_stage_order(precision, p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1], tabs[p - 1],
+                                               grid(), s),
+                      "batch stage (order)");
+    }
+
+    // window stages [w0, w1) on `stream` (captured or not); layout 2 forks one stream per order
+    void launch_window(int w0, int w1) {
+        if (tune.layout != 2) {
+            for (int w = w0; w < w1; ++w) launch_window_stage(w);
+            return;
+        }
+        hip_check(hipEventRecord(fork, stream), "fork");
+        for (int p = 4; p >= 1; --p) {
+            if (!n_p[p - 1]) continue;
+            hip_check(hipStreamWaitEvent(ostream[p - 1], fork, 0), "fork wait");
+            for (int w = w0; w < w1; ++w) launch_order_stage(w, p, ostream[p - 1]);
+            hip_check(hipEventRecord(join[p - 1], ostream[p - 1]), "join");
+            hip_check(hipStreamWaitEvent(stream, join[p - 1], 0), "join wait");
+        }
+    }
+
     void launch_window_stage(int w) {
         const StageParams* pr = static_cast<const StageParams*>(params[w].p);
         for (int p = 4; p >= 1; --p)
-            if (tabm[p - 1] && n_p[p - 1])
-                hip_check(launch_batch_stage_mfma(p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1], halo_cap(p), grid(),
-                                                  stream),
-                          "batch stage (matrix cores)");
+            if (tabm[p - 1] && n_p[p - 1]) launch_order_stage(w, p, stream);
         if (tune.layout == 0) {
             int mask = order_mask;
             for (int p = 1; p <= 4; ++p)
@@ -141,10 +176,7 @@ struct dgx_batch {
                           "batch stage");
         } else {
             for (int p = 4; p >= 1; --p)
-                if (!tabm[p - 1] && n_p[p - 1])
-                    hip_check(launch_batch_stage_order(precision, p, pr, items_p[p - 1].as<BatchItem>(), n_p[p - 1],
-                                                       tabs[p - 1], grid(), stream),
-                              "batch stage (order)");
+                if (!tabm[p - 1] && n_p[p - 1]) launch_order_stage(w, p, stream);
         }
     }
 
@@ -154,7 +186,7 @@ struct dgx_batch {
         if (dirty) build();
         for (int s = 0; s + 1 < nsteps; s += 2) hip_check(hipGraphLaunch(exec, stream), "graph launch");
         if (nsteps & 1) {
-            for (int w = 0; w < 3; ++w) launch_window_stage(w);
+            launch_window(0, 3);
             hip_check(hipStreamSynchronize(stream), "sync");
             for (auto* c : inst) std::swap(c->u, c->u1);
             dirty = true;
@@ -252,6 +284,11 @@ dgx_status dgx_batch_create(int32_t n, const dgx_mesh_desc* meshes, const dgx_sp
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreate(&b->ev0), "event");
         hip_check(hipEventCreate(&b->ev1), "event");
+        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "event");
+        for (int p = 0; p < 4; ++p) {
+            hip_check(hipStreamCreateWithFlags(&b->ostream[p], hipStreamNonBlocking), "stream");
+            hip_check(hipEventCreateWithFlags(&b->join[p], hipEventDisableTiming), "event");
+        }
     });
     if (st != DGX_OK) { delete b; return st; }
     *out = b;
@@ -312,7 +349,7 @@ dgx_status dgx_batch_get_tuning(const dgx_batch* b, dgx_batch_tuning* t) {
 dgx_status dgx_batch_set_tuning(dgx_batch* b, const dgx_batch_tuning* t) {
     return guarded([&] {
         if (!b || !t) throw std::invalid_argument("null argument");
-        if (t->layout < 0 || t->layout > 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16)
+        if (t->layout < 0 || t->layout > 2 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16)
             throw std::invalid_argument("bad tuning");
         for (int p = 0; p < 4; ++p)
             if (t->chunk[p] < 1 || t->chunk[p] > 64) throw std::invalid_argument("bad tuning chunk");
@@ -335,7 +372,7 @@ dgx_status dgx_batch_autotune(dgx_batch* b, int32_t trial_steps, dgx_batch_tunin
         dgx_batch_tuning best = b->tune;
         double best_ms = measure();
         const int chunks[] = {1, 2, 4, 8, 16};
-        for (int layout = 0; layout <= 1; ++layout) {
+        for (int layout = 0; layout <= 2; ++layout) {
             dgx_batch_tuning cur = best;
             cur.layout = layout;
             for (int p = 4; p >= 1; --p) {

@@ -176,9 +176,17 @@ struct dgx_ctx {
         upload(b, t.data(), t.size() * sizeof(T));
     }
 
-    // a second stream of default priority (the adjoint's recomputation beside its reverse sweep)
+    // A second stream of default priority (the adjoint's recomputation beside its reverse
+    // sweep).  Streams share the process's few hardware queues once those are all in use, and
+    // two streams on one queue run one after the other; a stream with a CU mask gets a queue of
+    // its own, so the side stream is made with a mask that enables every CU.
     hipStream_t side_stream() {
-        if (!stream2) hip_check(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "hipStreamCreate");
+        if (!stream2) {
+            hipDeviceProp_t prop;
+            hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+            std::vector<uint32_t> mask((std::max(prop.multiProcessorCount, 1) + 31) / 32, 0xffffffffu);
+            hip_check(hipExtStreamCreateWithCUMask(&stream2, (uint32_t)mask.size(), mask.data()), "hipStreamCreate");
+        }
         return stream2;
     }
 

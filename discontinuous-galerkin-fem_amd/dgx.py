@@ -78,7 +78,7 @@ class BatchTuning(C.Structure):
     _fields_ = [("layout", C.c_int32), ("chunk", C.c_int32 * 4), ("blocks_per_cu", C.c_int32)]
 
     def as_dict(self):
-        return {"layout": ["fused", "per_order"][self.layout], "chunk": list(self.chunk),
+        return {"layout": ["fused", "per_order", "concurrent"][self.layout], "chunk": list(self.chunk),
                 "blocks_per_cu": self.blocks_per_cu}
 
 
@@ -558,8 +558,8 @@ class Batch:
         _check(load().dgx_batch_get_tuning(self._h, C.byref(t)))
         return t
 
-    def set_tuning(self, layout="fused", chunk=(4, 4, 2, 2), blocks_per_cu=2):
-        t = BatchTuning({"fused": 0, "per_order": 1}[layout], (C.c_int32 * 4)(*chunk), blocks_per_cu)
+    def set_tuning(self, layout="concurrent", chunk=(4, 4, 4, 4), blocks_per_cu=4):
+        t = BatchTuning({"fused": 0, "per_order": 1, "concurrent": 2}[layout], (C.c_int32 * 4)(*chunk), blocks_per_cu)
         _check(load().dgx_batch_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, trial_steps=4):

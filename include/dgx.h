@@ -320,7 +320,9 @@ dgx_status dgx_rk3_vjp_profile(dgx_ctx* ctx, double dt, int32_t nsteps, int32_t 
 typedef struct dgx_batch dgx_batch;
 
 typedef struct {
-    int32_t layout;      /* 0 = one launch per stage for all orders, 1 = one launch per order */
+    int32_t layout;      /* 0 = one launch per stage for all orders, 1 = one launch per order,
+                            2 = one launch per order, each order's stages on a stream of its own
+                            (the orders' launches overlap) */
     int32_t chunk[4];    /* elements per work item = 64 * chunk[p-1] */
     int32_t blocks_per_cu;
 } dgx_batch_tuning;

@@ -30,7 +30,7 @@ SPECS = [{"n": 4, "p": 1, "beta": 4.0, "xc": 5.0, "yc": 5.0},
          {"n": 16, "p": 2, "beta": 5.0, "xc": 3.0, "yc": 7.0}]
 
 
-@pytest.mark.parametrize("layout", ["fused", "per_order"])
+@pytest.mark.parametrize("layout", ["fused", "per_order", "concurrent"])
 @pytest.mark.parametrize("nsteps", [4, 5])
 def test_batched_steps_match_single_context(layout, nsteps):
     b, meshes, dts = _make(SPECS, "fp64")
@@ -46,12 +46,12 @@ def test_batched_steps_match_single_context(layout, nsteps):
         assert np.abs(g - ref).max() < 1e-13 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("layout", ["fused", "per_order"])
+@pytest.mark.parametrize("layout", ["fused", "per_order", "concurrent"])
 @pytest.mark.parametrize("nsteps", [4, 5])
 def test_batched_fp32_matches_single_context(layout, nsteps):
     # FP32: every order runs the matrix-core stage in both the batch (tile work items, one
-    # launch per order) and the single context; the two layouts differ only for per-lane
-    # orders (FP64 above).  The single context chooses its own
+    # launch per order, on one stream or one stream per order) and the single context; fused
+    # and per-order launches differ only for per-lane orders (FP64 above).  The single context chooses its own
     # storage offset, so the comparison is to FP32 rounding (1e-6 relative after the steps).
     b, meshes, dts = _make(SPECS, "fp32")
     b.set_tuning(layout, chunk=(1, 2, 4, 1), blocks_per_cu=1)
@@ -82,7 +82,7 @@ def test_autotune_keeps_results_and_restores_state():
     b, meshes, dts = _make(specs, "fp32")
     before = [c.get_state() for c in b.instances]
     t = b.autotune(2)
-    assert t.layout in (0, 1) and all(1 <= c <= 64 for c in t.chunk)
+    assert t.layout in (0, 1, 2) and all(1 <= c <= 64 for c in t.chunk)
     after = [c.get_state() for c in b.instances]
     for x, y in zip(before, after):
         assert np.array_equal(x, y)
