@@ -818,10 +818,14 @@ void build_tri_context(dgx_ctx& c, const dgx_mesh_desc& m, const dgx_space_desc&
         c.upload_real<float>(c.tabL, tL);
         c.upload_real<float>(c.tabR, tR);
     }
-    // matrix-core stage kernels: FP32 Euler and Navier-Stokes, degree-2p volume rule, p = 3, 4
-    // (DGX_STAGE_KERNEL=lane keeps the one-element-per-lane kernel for FP32)
+    // matrix-core stage kernels: FP32 Euler (p = 1..4) and Navier-Stokes (p = 3, 4), degree-2p
+    // volume rule (DGX_STAGE_KERNEL=lane keeps the one-element-per-lane kernel for FP32).  At
+    // p = 1 the per-lane kernel is the faster one on large meshes (stage 10.2 against 12.0 us at
+    // K = 131072, 18.1 against 22.9 us at K = 294912; the tiles win below K = 75000), so single
+    // FP32 contexts of more than 98304 elements keep it.
     const char* kv = std::getenv("DGX_STAGE_KERNEL");
-    const bool lane_only = kv && std::string(kv) == "lane" && precision == DGX_FP32;
+    const bool lane_only = precision == DGX_FP32 &&
+                           ((kv && std::string(kv) == "lane") || (c.p == 1 && c.K > 98304));
     // the matrix-core kernels address every buffer through 32-bit byte offsets: the largest one
     // (state or geometry) must stay below 2 GiB
     const double big = std::max({16.0 * c.np, 8.0 * kGeoRows, 4.0 * kGeo32NsRows}) * c.Kp;

@@ -108,6 +108,25 @@ def test_mfma_large_mesh_tail_block():
     assert np.abs(mf.residual(u0)[tail] - r_ref[tail]).max() < 1e-4 * np.abs(r_ref).max()
 
 
+def test_p1_kernel_choice_by_mesh_size():
+    # at p = 1 a single FP32 context keeps the per-lane kernel above 98304 elements (it is the
+    # faster one there); each choice is held against the FP64 oracle like the per-lane kernel
+    got = {}
+    for n in (221, 222):  # K = 97682 (tiles), 98568 (per lane)
+        verts, tris = R.split_quad_mesh(n, 10.0)
+        ora = R.Oracle(verts, tris, 1, basis="dubiner")
+        u0 = ora.project(R.vortex_state(ora.volume_points()))
+        ctx = dgx.Context(verts, tris, 1, basis="dubiner", precision="fp32")
+        ln = _context(True, verts, tris, 1, basis="dubiner", precision="fp32")
+        got[n] = ctx.info.stage_kernel
+        r_ref = ora.residual(u0)
+        e, e_ln = _rel(ctx.residual(u0), r_ref), _rel(ln.residual(u0), r_ref)
+        assert e < 1e-3 and e < 2 * e_ln + 1e-6, (n, e, e_ln)
+        ctx.close()
+        ln.close()
+    assert got == {221: 1, 222: 0}
+
+
 # ---------------- stage adjoint ------------------------------------------------------------
 
 def _adj_setup(p, case, n=5):
