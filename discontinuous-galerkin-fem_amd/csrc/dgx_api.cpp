@@ -26,6 +26,20 @@ thread_local std::string g_error;
 
 namespace {
 
+// The kernels' block renumbering must visit every block once.
+constexpr bool xcd_tile_is_bijection(int n) {
+    uint64_t seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // n <= 512
+    for (int b = 0; b < n; ++b) {
+        const int t = dgx::xcd_tile(b, n);
+        if (t < 0 || t >= n || ((seen[t / 64] >> (t % 64)) & 1u)) return false;
+        seen[t / 64] |= uint64_t(1) << (t % 64);
+    }
+    return true;
+}
+static_assert(xcd_tile_is_bijection(1) && xcd_tile_is_bijection(5) && xcd_tile_is_bijection(8) &&
+                  xcd_tile_is_bijection(13) && xcd_tile_is_bijection(256) && xcd_tile_is_bijection(509),
+              "xcd_tile is not a permutation");
+
 // Inverse of a small dense SPD/general matrix by Gauss-Jordan with partial pivoting.
 std::vector<double> invert(std::vector<double> M, int n) {
     std::vector<double> I(n * n, 0.0);

@@ -12,6 +12,16 @@ namespace dgx {
 
 constexpr int kBlock = 256;  // threads per block of the element kernels (4 wave64)
 
+// Blocks are dealt round-robin over the GPU's 8 XCDs (block b to XCD b mod 8), each with its own
+// L2.  Renumbering block b to xcd_tile(b, n) gives every XCD one contiguous range of the n
+// blocks (the first n mod 8 XCDs one more), so that the mesh neighbours of a block's elements
+// -- the neighbouring tiles, the mesh rows above and below -- are mostly read through the same
+// L2.  A bijection of [0, n) for every n; it changes locality only.
+__host__ __device__ constexpr int xcd_tile(int b, int n) {
+    const int q = n / 8, r = n % 8, x = b % 8, k = b / 8;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // Volume points of the Dunavant rule exact to degree 2p (+1 for the 2P1 variant).
 constexpr int nq_of(int p, int rule) {
     return rule == 0 ? (p == 1 ? 3 : p == 2 ? 6 : p == 3 ? 12 : 16)

@@ -3,11 +3,14 @@
 // basis, tensor Gauss-Legendre volume rule, bilinear mapping (S/core/space.cpp:47-55,
 // S/basis/legendre.cpp, S/reference/elements.cpp:61-108).
 //
-// The mapping is bilinear, so the Jacobian varies inside an element: the geometry holds, per
-// element, dxi/dx and w_q |J_q| at every volume point, dxi/dx at every face point (for the
-// Navier-Stokes face gradients), and the faces' normals and lengths; the mass matrix
-// M_e = sum_q w_q |J_q| phi phi^T is inverted per element (FP64, [Np*Np][Kp]).  The stage
-// forms the residual R = M dU/dt with unfolded tables and applies M_e^-1 in FP64.
+// The mapping is bilinear, so the Jacobian varies inside an element.  The stage rebuilds it
+// from the element's four vertices: the map's six derivative coefficients in FP64, dxi/dx and
+// w_q |J_q| at the volume points and dxi/dx at the face points (the Navier-Stokes face
+// gradients, the neighbour's from the neighbour's vertices) in the state precision; the faces'
+// normals and lengths are read.  The stage forms the residual R = M dU/dt with unfolded tables
+// and applies M_e^-1 in FP64: on a parallelogram M_e = |J| diag(m_a m_b) (the 1-D Legendre
+// masses), otherwise the per-element inverse ([Np*Np][Kp], inverted on the host).  The host's
+// per-point geometry rows serve the projection and the reductions.
 //
 // Reference behaviour: volume, face, boundary and viscous terms as for triangles
 // (S/weak_forms/euler_weak_formulation.cpp:72-250, navier_stokes_weak_formulation.cpp:100-278),
@@ -20,14 +23,49 @@
 namespace dgx {
 namespace {
 
-// Physical gradient of U at face point q of local face f of element el:
+// The bilinear map of element e from its vertices (geometry rows gX..gX+7, FP64): dx/dxi =
+// ex0 + ex1 eta, dx/deta = fx0 + ex1 xi and y alike -- the host's quad_map regrouped -- with the
+// coefficients rounded to the state precision.
+template <typename Real>
+struct QuadJac {
+    Real ex0, ex1, fx0, ey0, ey1, fy0;
+    __device__ __forceinline__ void load(const double* __restrict__ geo, int gX, int Kp, int e, double* det_c = nullptr) {
+        double V[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) V[k] = geo[(size_t)(gX + k) * Kp + e];
+        const double dex0 = 0.25 * ((V[2] - V[0]) + (V[4] - V[6])), dex1 = 0.25 * ((V[0] - V[2]) + (V[4] - V[6]));
+        const double dfx0 = 0.25 * ((V[6] - V[0]) + (V[4] - V[2]));
+        const double dey0 = 0.25 * ((V[3] - V[1]) + (V[5] - V[7])), dey1 = 0.25 * ((V[1] - V[3]) + (V[5] - V[7]));
+        const double dfy0 = 0.25 * ((V[7] - V[1]) + (V[5] - V[3]));
+        if (det_c) *det_c = dex0 * dfy0 - dey0 * dfx0;
+        ex0 = Real(dex0), ex1 = Real(dex1), fx0 = Real(dfx0), ey0 = Real(dey0), ey1 = Real(dey1), fy0 = Real(dfy0);
+    }
+    // A = (dxi/dx, deta/dx, dxi/dy, deta/dy) and det J at (xi, eta)
+    __device__ __forceinline__ void at(Real xi, Real eta, Real (&A)[4], Real& det) const {
+        const Real j00 = ex0 + ex1 * eta, j01 = ey0 + ey1 * eta;  // d(x, y)/dxi
+        const Real j10 = fx0 + ex1 * xi, j11 = fy0 + ey1 * xi;    // d(x, y)/deta
+        det = j00 * j11 - j01 * j10;
+        const Real id = rcp(det);
+        A[0] = j11 * id, A[1] = -j01 * id, A[2] = -j10 * id, A[3] = j00 * id;
+    }
+};
+
+// Reference coordinates of face point s (a 1-D Gauss-Legendre node) of local face f: the edge
+// (v_f, v_f+1) traversed counter-clockwise (face_point_quad on the host).
+template <typename Real>
+__device__ __forceinline__ void quad_face_xy(int f, Real s, Real& xi, Real& eta) {
+    xi = f == 0 ? s : f == 1 ? Real(1) : f == 2 ? -s : Real(-1);
+    eta = f == 0 ? Real(-1) : f == 1 ? s : f == 2 ? Real(1) : -s;
+}
+
+// Physical gradient of U at face point q of local face f of an element with map J:
 // g[v*2+c] = sum_i u[i][v] G(q,i,c), G = the reference's read of the stored physical basis
 // gradients (entry (q',i',d) of the interleaved storage, dxi/dx at point q').
 // The constant mode's entry is nonzero under the reference's read: it takes the full
 // coefficient, the stored value plus the storage offset `off`.
 template <int P, bool QUIRK, typename Real>
 __device__ __forceinline__ void quad_face_grad(const Real (&u)[QuadDims(P).np][4], const Real* __restrict__ tab,
-                                               const double* __restrict__ geo, int Kp, int el, int f, int q,
+                                               const Real (&x1)[P + 1], const QuadJac<Real>& J, int f, int q,
                                                const Real (&off)[4], Real* g) {
     constexpr QuadDims D(P);
     constexpr int NP = D.np, NFQ = D.nfq;
@@ -43,8 +81,10 @@ __device__ __forceinline__ void quad_face_grad(const Real (&u)[QuadDims(P).np][4
                 qs = (k / 2) / NP;
                 is = (k / 2) % NP;
             }
-            const double* A = geo + (size_t)(D.gFA + (f * NFQ + qs) * 4 + d * 2) * Kp + el;
-            const Real a0 = Real(A[0]), a1 = Real(A[Kp]);
+            Real xi, eta, A[4], det;
+            quad_face_xy(f, x1[qs], xi, eta);
+            J.at(xi, eta, A, det);
+            const Real a0 = A[d * 2], a1 = A[d * 2 + 1];
             const Real* gr = tab + D.GF + ((f * NFQ + qs) * NP + is) * 2;
             const Real G = a0 * gr[0] + a1 * gr[1];
 #pragma unroll
@@ -59,7 +99,7 @@ template <int P, typename Real, bool NS, bool QUIRK>
 __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
     constexpr QuadDims D(P);
     constexpr int NP = D.np, NQ = D.nq, NFQ = D.nfq;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (e >= sp.K) return;
     const Real* __restrict__ tab = static_cast<const Real*>(sp.tab);
     const Real* __restrict__ uk = static_cast<const Real*>(sp.uk);
@@ -72,6 +112,14 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
     const Real kappa = Real(sp.kappa);
     const Real ur[4] = {Real(sp.uref[0]), Real(sp.uref[1]), Real(sp.uref[2]), Real(sp.uref[3])};
     const Real uo[4] = {Real(sp.uoff[0]), Real(sp.uoff[1]), Real(sp.uoff[2]), Real(sp.uoff[3])};
+    const double* __restrict__ t64 = sp.tab64;
+    constexpr int R = P + 1;
+    Real x1[R];  // the 1-D Gauss-Legendre nodes
+#pragma unroll
+    for (int a = 0; a < R; ++a) x1[a] = Real(t64[D.X1 + a]);
+    QuadJac<Real> J;
+    double det_c;  // det J at the centre (parallelograms: the constant Jacobian)
+    J.load(geo, D.gX, Kp, e, &det_c);
 
     Real u[NP][4];
 #pragma unroll
@@ -94,9 +142,11 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) U[v] = fma(ph, u[i][v], U[v]);
         }
-        const double* gq = geo + (size_t)(D.gV + 5 * q) * Kp + e;
-        const Real a00 = Real(gq[0]), a01 = Real(gq[Kp]), a10 = Real(gq[2 * Kp]), a11 = Real(gq[3 * Kp]);
-        const Real wd = Real(gq[4 * Kp]);
+        // volume point q = (s_(q mod R), s_(q / R)), weight w_(q mod R) w_(q / R)
+        Real A[4], det;
+        J.at(x1[q % R], x1[q / R], A, det);
+        const Real a00 = A[0], a01 = A[1], a10 = A[2], a11 = A[3];
+        const Real wd = Real(t64[D.W1 + q % R] * t64[D.W1 + q / R]) * fabs(det);
         const Real ri = rcp(U[0]);
         const Real uu = U[1] * ri, vv = U[2] * ri;
         const Real p = (gam - Real(1)) * (U[3] - Real(0.5) * U[0] * (uu * uu + vv * vv));
@@ -229,6 +279,8 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
             for (int v = 0; v < 4; ++v) Hs[q][v] = sgn * sfe * H[v];
         }
         if constexpr (NS) {
+            QuadJac<Real> Jn;  // the neighbour's map (its vertices are in cache: it was just read)
+            if (!boundary) Jn.load(geo, D.gX, Kp, nb);
             const Real sigma = Real(sp.sigma0 * sp.mu * (P + 1) * (P + 1) / fmax(len, 1e-12));
             // (1/2, -1/2): the reference's half difference; (-1/2, -1/2): the consistent -{Fv}.n
             const Real wL = sp.ns_avg_ref ? Real(0.5) : Real(-0.5), wR = Real(-0.5);
@@ -236,12 +288,12 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
 #pragma unroll
             for (int q = 0; q < NFQ; ++q) {
                 Real go[8], gn[8], Fvo[4], Gvo[4], Fn[4];
-                quad_face_grad<P, QUIRK>(u, tab, geo, Kp, e, f, q, uo, go);
+                quad_face_grad<P, QUIRK>(u, tab, x1, J, f, q, uo, go);
                 viscous_flux(Uo[q], go, gam, mu, kappa, Fvo, Gvo);
                 if (!boundary) {
                     const int qq = flip ? (NFQ - 1 - q) : q;
                     Real Fvn[4], Gvn[4];
-                    quad_face_grad<P, QUIRK>(un_c, tab, geo, Kp, nb, fn, qq, uo, gn);
+                    quad_face_grad<P, QUIRK>(un_c, tab, x1, Jn, fn, qq, uo, gn);
                     viscous_flux(Un[q], gn, gam, mu, kappa, Fvn, Gvn);
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
@@ -289,14 +341,23 @@ __global__ __launch_bounds__(kBlock) void quad_stage_kernel(StageParams sp) {
     }
     const double* __restrict__ mi = sp.minve;
     const Real dt = Real(sp.dt), ca = Real(sp.ca), cb = Real(sp.cb);
+    const bool affine = (code & kQuadAffineBit) != 0;
+    const double inv_det_c = 1.0 / fabs(det_c);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
         double r[4] = {0, 0, 0, 0};
+        if (affine) {
+            // mode i = (i mod R, i / R): M^-1 = diag(1 / (m_a m_b)) / |J|
+            const Real s = Real(t64[D.IM1 + i % R] * inv_det_c * t64[D.IM1 + i / R]);
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const double m = mi[(size_t)(i * NP + j) * Kp + e];
+            for (int v = 0; v < 4; ++v) r[v] = double(s * acc[i][v]);
+        } else {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) r[v] = fma(m, double(acc[j][v]), r[v]);
+            for (int j = 0; j < NP; ++j) {
+                const double m = mi[(size_t)(i * NP + j) * Kp + e];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) r[v] = fma(m, double(acc[j][v]), r[v]);
+            }
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {

@@ -72,7 +72,7 @@ void quad_tensor_stage_kernel(StageParams sp) {
     constexpr int T1 = 0, D1 = D.D1 - D.T1, FX = D.FX - D.T1, FY = D.FY - D.T1, LX = D.LX - D.T1;
     constexpr int X1 = 0, W1 = R, IM1 = 2 * R;
     const int l = threadIdx.x & 3;
-    const int e = blockIdx.x * (kTBlock / 4) + (threadIdx.x >> 2);
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * (kTBlock / 4) + (threadIdx.x >> 2);
     if (e >= sp.K) return;  // the four lanes of an element leave together
     const Real* __restrict__ uk = static_cast<const Real*>(sp.uk);
     const Real* __restrict__ un = static_cast<const Real*>(sp.un);

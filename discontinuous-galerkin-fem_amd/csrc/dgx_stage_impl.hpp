@@ -42,12 +42,6 @@ __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_amdgcn
 template <typename Real>
 __device__ __forceinline__ Real ref_max(Real a, Real b) { return (a < b) ? b : a; }
 
-// Block -> element-tile remap so that consecutive tiles (mesh neighbours) land on the same
-// XCD and share its L2 (blocks are dealt round-robin over the 8 XCDs).
-__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
-    if ((nblk & 7) != 0) return bid;
-    return (bid & 7) * (nblk >> 3) + (bid >> 3);
-}
 
 // Rusanov flux H(A, B, n) (rusanov_flux, euler_weak_formulation.cpp:51-70), with the normal
 // fluxes written as F.n = (rho un, rho u un + p nx, rho v un + p ny, (E+p) un).

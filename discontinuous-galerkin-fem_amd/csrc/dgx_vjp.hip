@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(VjpParams sp) {
     }
     __syncthreads();
 
-    const int e = xcd_tile_(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (e >= sp.K) return;
     const Real g = Real(sp.gamma);
     const bool residual_mode = sp.mode == MODE_RESIDUAL;

@@ -21,11 +21,6 @@ __device__ __forceinline__ Real sqrt_(Real x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float sqrt_<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-__device__ __forceinline__ int xcd_tile_(int bid, int nblk) {
-    if ((nblk & 7) != 0) return bid;
-    return (bid & 7) * (nblk >> 3) + (bid >> 3);
-}
-
 // Ubar += (dFn/dU)^T psi + sbar * d(|un| + c)/dU, Fn = (r un, mx un + p nx, my un + p ny,
 // (E+p) un), un = (u nx + v ny) for the (not necessarily unit) direction n.
 template <typename Real>
