@@ -69,8 +69,8 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     const int ec = valid ? e : sp.K - 1;  // lanes past the end compute on a valid element
     const unsigned uKp = unsigned(sp.Kp);
     const bool residual_mode = sp.mode == MODE_RESIDUAL;
-    const float gam = float(sp.gamma);
-    const float ur[4] = {float(sp.uref[0]), float(sp.uref[1]), float(sp.uref[2]), float(sp.uref[3])};
+    const float gam = sp.f.gamma;
+    const float ur[4] = {sp.f.uref[0], sp.f.uref[1], sp.f.uref[2], sp.f.uref[3]};
     const float* __restrict__ tm = static_cast<const float*>(sp.tabm);
     // wave-uniform values that enter scalar buffer offsets go through readfirstlane, so the
     // loads take them from a scalar register instead of a per-lane loop
@@ -106,7 +106,7 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
 #pragma unroll
         for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
     }
-    const bool rk_blend = sp.mode == MODE_RK && sp.ca != 0.0;
+    const bool rk_blend = sp.mode == MODE_RK && sp.f.ca != 0.0f;
     // geometry rows of the tile: wave w loads rows w + 4k (one 256-byte row per instruction;
     // rows past the last read at an offset past the resource: zeros, no traffic)
     constexpr int kGeoPerThread = (kGeo32Rows + 3) / 4;
@@ -138,10 +138,12 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     {
         const int2 x = reinterpret_cast<const int2*>(sp.halo)[tile * 64 + lane];
         hcode = uint32_t(x.y);
+        // no neighbour (x = -1): an offset past the resource, a zero read without traffic; the
+        // mode row is a scalar offset
+        const unsigned vo = x.x >= 0 ? unsigned(x.x) * 16u : state_bytes;
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
-            // no neighbour (x = -1): an offset past the resource, a zero read without traffic
-            const v4u y = load_x4(r_uk, x.x >= 0 ? (unsigned(i) * uKp + unsigned(x.x)) * 16u : state_bytes);
+            const v4u y = __builtin_amdgcn_raw_buffer_load_b128(r_uk, vo, unsigned(i) * 16u * uKp, 0);
 #pragma unroll
             for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
         }
@@ -400,10 +402,24 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     // ---------------- output ---------------------------------------------------------------
     if (!valid) return;  // (no barrier follows inside the tile)
     const auto r_out = make_rsrc(sp.out, state_bytes);
-    const float dt = float(sp.dt), ca = float(sp.ca), cb = float(sp.cb);
-    // u^n for the second and third RK stages
-    float unv[4][R];
-    if (rk_blend) {
+    const float dt = sp.f.dt, ca = sp.f.ca, cb = sp.f.cb;
+    auto store = [&](auto&& value) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v4u x;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) x[v] = __float_as_uint(value(v, r));
+            // padding slots address past the end and are dropped
+            __builtin_amdgcn_raw_buffer_store_b128(x, r_out, slot_off(r, unsigned(e)), 0, 0);
+        }
+    };
+    if (sp.mode != MODE_RK) {
+        store([&](int v, int r) { return acc[v][r]; });
+    } else if (!rk_blend) {
+        store([&](int v, int r) { return b[v][r] + dt * acc[v][r]; });
+    } else {
+        // u^n for the second and third RK stages
+        float unv[4][R];
         const auto r_un = make_rsrc(sp.un, state_bytes);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -411,20 +427,7 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
 #pragma unroll
             for (int v = 0; v < 4; ++v) unv[v][r] = __uint_as_float(x[v]);
         }
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        v4u x;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            float o;
-            if (sp.mode != MODE_RK) o = acc[v][r];
-            else if (!rk_blend) o = b[v][r] + dt * acc[v][r];
-            else o = ca * unv[v][r] + cb * (b[v][r] + dt * acc[v][r]);
-            x[v] = __float_as_uint(o);
-        }
-        // padding slots address past the end and are dropped
-        __builtin_amdgcn_raw_buffer_store_b128(x, r_out, slot_off(r, unsigned(e)), 0, 0);
+        store([&](int v, int r) { return ca * unv[v][r] + cb * (b[v][r] + dt * acc[v][r]); });
     }
 }
 
