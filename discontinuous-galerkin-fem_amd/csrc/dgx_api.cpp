@@ -1555,40 +1555,58 @@ void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every
     for (hipEvent_t& e : b.ev)
         if (!e) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     hipStream_t A = c->stream, B = c->side_stream();
-    // forward with checkpoints; the last two segments keep their stage states in the two stage
-    // sets instead (the reverse sweep starts on them without recomputation)
+    // Forward with checkpoints; the last two segments keep their stage states in the two stage
+    // sets instead (the reverse sweep starts on them without recomputation).  A segment's
+    // starting state lives where the sweep reads it -- its checkpoint, or entry 0 of its stage
+    // set -- and the previous segment's last stage writes it there, so only the initial state is
+    // copied.
     const int kept = std::min(nck, 2);
+    auto start_of = [&](int j) -> void* { return j < nck - kept ? b.ckpt[j].p : b.stages[j & 1][0].p; };
+    void* x = start_of(0);
+    hip_check(hipMemcpyAsync(x, c->u.p, sb, hipMemcpyDeviceToDevice, A), "initial state");
     for (int j = 0; j < nck; ++j) {
         const int s0 = j * every, len = std::min(every, nsteps - s0);
-        if (j < nck - kept) {
-            hip_check(hipMemcpyAsync(b.ckpt[j].p, c->u.p, sb, hipMemcpyDeviceToDevice, A), "ckpt");
-            for (int k = 0; k < len; ++k) c->rk3_step(dt);
-            continue;
-        }
+        const bool keep = j >= nck - kept;
         std::vector<DevBuf>& st = b.stages[j & 1];
-        hip_check(hipMemcpyAsync(st[0].p, c->u.p, sb, hipMemcpyDeviceToDevice, A), "segment start");
         for (int k = 0; k < len; ++k) {
-            void* v0 = st[3 * k].p;
-            void* v1 = st[3 * k + 1].p;
-            void* v2 = st[3 * k + 2].p;
-            c->stage_launch(v0, v0, v1, MODE_RK, dt, 0.0, 1.0);
-            c->stage_launch(v1, v0, v2, MODE_RK, dt, 0.75, 0.25);
-            c->stage_launch(v2, v0, k + 1 < len ? st[3 * k + 3].p : c->u.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
+            void *t1, *t2, *y;
+            if (keep) {
+                t1 = st[3 * k + 1].p;
+                t2 = st[3 * k + 2].p;
+                y = st[3 * k + 3].p;
+            } else {
+                // two of the context's state buffers other than x; the step's output reuses the
+                // first stage's (it is read only by the second stage)
+                void* pool[3] = {c->u.p, c->u1.p, c->u2.p};
+                int n = 0;
+                void* pick[2] = {nullptr, nullptr};
+                for (void* q : pool)
+                    if (q != x && n < 2) pick[n++] = q;
+                t1 = pick[0];
+                t2 = pick[1];
+                y = t1;
+            }
+            if (k + 1 == len) y = j + 1 < nck ? start_of(j + 1) : c->u.p;
+            c->stage_launch(x, x, t1, MODE_RK, dt, 0.0, 1.0);
+            c->stage_launch(t1, x, t2, MODE_RK, dt, 0.75, 0.25);
+            c->stage_launch(t2, x, y, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
+            x = y;
         }
     }
     if (mid) hip_check(hipEventRecord(mid, A), "record");
     hip_check(hipEventRecord(b.ev[0], A), "record");
     for (int set = 0; set < 2; ++set) hip_check(hipEventRecord(b.ev[1 + set], A), "record");  // the kept sets
     hip_check(hipStreamWaitEvent(B, b.ev[0], 0), "wait");
-    // stage states of segment j into its set, on the side stream
+    // the state at step k of segment j: its start for k = 0, else entry 3k of its stage set
+    auto state_at = [&](int j, int k) -> void* { return k == 0 ? start_of(j) : b.stages[j & 1][3 * k].p; };
+    // stage states of segment j into its set, on the side stream, from its checkpoint
     bool used[2] = {false, false};  // a reverse step has read the set
     auto recompute = [&](int j) {
         const int set = j & 1, s0 = j * every, len = std::min(every, nsteps - s0);
         std::vector<DevBuf>& st = b.stages[set];
         if (used[set]) hip_check(hipStreamWaitEvent(B, b.ev[3 + set], 0), "wait");
-        hip_check(hipMemcpyAsync(st[0].p, b.ckpt[j].p, sb, hipMemcpyDeviceToDevice, B), "restore");
         for (int k = 0; k < len; ++k) {
-            void* v0 = st[3 * k].p;
+            void* v0 = state_at(j, k);
             void* v1 = st[3 * k + 1].p;
             void* v2 = st[3 * k + 2].p;
             c->stage_launch(v0, v0, v1, MODE_RK, dt, 0.0, 1.0, B);
@@ -1603,7 +1621,7 @@ void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every
         std::vector<DevBuf>& st = b.stages[set];
         hip_check(hipStreamWaitEvent(A, b.ev[1 + set], 0), "wait");
         for (int k = len - 1; k >= 0; --k) {
-            void* v0 = st[3 * k].p;
+            void* v0 = state_at(j, k);
             void* v1 = st[3 * k + 1].p;
             void* v2 = st[3 * k + 2].p;
             vjp(c, v2, b.l0.p, nullptr, nullptr, b.acc.p, b.l1.p, MODE_RK, dt, 1.0 / 3.0, 2.0 / 3.0);
