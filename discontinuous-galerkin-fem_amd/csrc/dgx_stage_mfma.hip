@@ -68,7 +68,6 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     const bool valid = e < sp.K;
     const int ec = valid ? e : sp.K - 1;  // lanes past the end compute on a valid element
     const unsigned uKp = unsigned(sp.Kp);
-    const bool residual_mode = sp.mode == MODE_RESIDUAL;
     const float gam = sp.f.gamma;
     const float ur[4] = {sp.f.uref[0], sp.f.uref[1], sp.f.uref[2], sp.f.uref[3]};
     const float* __restrict__ tm = static_cast<const float*>(sp.tabm);
@@ -228,7 +227,6 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
         fny[f] = L.geo[G32_N0Y + 2 * f][eloc];
         fsf[f] = L.geo[G32_S0 + f][eloc];
     }
-    (void)residual_mode;
 
     // ---------------- evaluations at the volume and face slots (matrix cores) ---------------
     v4f U[4], Uf[4];
