@@ -24,7 +24,8 @@ Output: one JSON line (rank 0).  Besides the contract keys it carries
   bf16_volume_variant : the same run with bfloat16 operands in the volume products (FP32
                   accumulation and state): DOF-stage/s and per-field L2 vs FP64 at 200 steps
   adjoint       : the metric's second half at the adjoint workload (K = 32768, 200 steps,
-                  checkpoints every 20): reverse sweep / forward and whole gradient / forward
+                  checkpoints every 20): reverse sweep / forward and whole gradient / forward,
+                  medians of three profiled runs
   cpu_baseline  : the reference algorithm (oracle/refcpu.cpp) built with the reference's
                   Release flags on this host, FP64, median of 3 runs on OMP_NUM_THREADS
                   threads (value) and on one thread, with and without -ffast-math; lscpu model
@@ -318,12 +319,16 @@ def adjoint_ratios(n, p, steps, every):
     run of the same steps, gradient of the final kinetic energy w.r.t. the initial state."""
     ctx, _, _, _ = make_ctx(n, p, "fp32", SEED)
     dt = dgx.stable_dt(ctx, 0.3)
-    ctx.rk3_vjp_profile(dt, 3 * every, every)  # warm-up: checkpoint buffers, side stream, code paths
-    fwd, ckpt, rev = ctx.rk3_vjp_profile(dt, steps, every)
+    ctx.rk3_vjp_profile(dt, 3 * every, every)  # warm-up: side stream, code paths
+    runs = [ctx.rk3_vjp_profile(dt, steps, every) for _ in range(3)]
+    rev_ratio = [r / f for f, _, r in runs]
+    grad_ratio = [(a + r) / f for f, a, r in runs]
+    fwd, ckpt, rev = (float(np.median([x[k] for x in runs])) for k in range(3))
     out = {"workload": f"euler_vortex_adjoint_n{n}_p{p}", "elements": ctx.K, "steps": steps,
-           "checkpoint_every": every, "forward_ms": fwd, "checkpointed_forward_ms": ckpt,
-           "reverse_sweep_ms": rev, "reverse_over_forward": rev / fwd,
-           "gradient_over_forward": (ckpt + rev) / fwd}
+           "checkpoint_every": every, "runs": 3, "forward_ms": fwd, "checkpointed_forward_ms": ckpt,
+           "reverse_sweep_ms": rev, "reverse_over_forward": float(np.median(rev_ratio)),
+           "gradient_over_forward": float(np.median(grad_ratio)),
+           "gradient_over_forward_runs": grad_ratio}
     ctx.close()
     return out
 

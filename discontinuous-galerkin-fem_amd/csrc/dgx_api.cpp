@@ -1543,7 +1543,8 @@ struct AdjointBuffers {
 // set: the recomputation of one segment overlaps the stage adjoints of the next, which are
 // latency-bound launches that leave most of the GPU idle.  The forward pass keeps the last two
 // segments' stage states in the sets as it goes.
-void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every, hipEvent_t mid = nullptr) {
+// The checkpoint and stage-state buffers and the events of a sweep of nsteps steps.
+void adjoint_buffers(dgx_ctx* c, AdjointBuffers& b, int nsteps, int every) {
     const size_t sb = c->state_bytes();
     const int nck = (nsteps + every - 1) / every;
     if (b.ckpt.size() < (size_t)nck) b.ckpt.resize(nck);
@@ -1554,6 +1555,12 @@ void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every
     }
     for (hipEvent_t& e : b.ev)
         if (!e) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+}
+
+void rk3_adjoint(dgx_ctx* c, AdjointBuffers& b, double dt, int nsteps, int every, hipEvent_t mid = nullptr) {
+    const size_t sb = c->state_bytes();
+    const int nck = (nsteps + every - 1) / every;
+    adjoint_buffers(c, b, nsteps, every);
     hipStream_t A = c->stream, B = c->side_stream();
     // Forward with checkpoints; the last two segments keep their stage states in the two stage
     // sets instead (the reverse sweep starts on them without recomputation).  A segment's
@@ -1719,8 +1726,9 @@ dgx_status dgx_rk3_vjp_profile(dgx_ctx* c, double dt, int32_t nsteps, int32_t ev
         DevBuf u0;
         u0.alloc(c->state_bytes());
         hip_check(hipMemcpyAsync(u0.p, c->u.p, c->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
-        // warm-up allocations of the checkpoint and stage stores, and the side stream, with a
-        // sweep of three segments (one recomputed)
+        // every buffer of the timed sweep allocated before it, and a warm-up sweep of three
+        // segments (one recomputed) for the side stream and the code paths
+        adjoint_buffers(c, b, nsteps, every);
         hip_check(launch_ke_grad(c->precision, c->p, c->rule, c->u.p, c->tab64.as<double>(), c->qw.as<double>(),
                                  c->geo.as<double>(), b.l0.p, c->K, c->Kp, c->uref, c->stream), "ke grad");
         rk3_adjoint(c, b, dt, std::min(nsteps, 3 * every), every);
