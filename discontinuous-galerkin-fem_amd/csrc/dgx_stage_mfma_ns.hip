@@ -41,7 +41,29 @@
 namespace dgx {
 namespace {
 
-constexpr int kTileEl = kBlock / 4;  // elements per block tile: 4 waves x 16
+constexpr int kTileEl = kBlock / 4;
+
+// Face-point trace record from the state's deviation Ud = U - ur from the point-value storage
+// offset ur: the normal flux with the pressure taken relative to pr = (gamma - 1) ur3 and the
+// wave speed |u.n| + c from the full pressure.  A constant pressure in the momentum fluxes adds
+// p (int grad phi_i - sum_faces int phi_i n) to the residual, zero under the volume and face
+// rules used, so the shift changes nothing but the rounding: at low Mach number the pressure
+// is many times the dynamic pressure (71 times at M = 0.1), and the momentum fluxes then carry
+// the rounding of the variation instead of that of the full pressure.  The energy flux
+// (E + p) u.n is (Ud3 + pd + hr) u.n with hr = ur3 + pr = gamma ur3.
+__device__ __forceinline__ void trace_record_dev(const float* Ud, const float* ur, float pr, float hr, float nx,
+                                                 float ny, float g, float* Fn, float& s) {
+    const float r = Ud[0] + ur[0], mx = Ud[1] + ur[1], my = Ud[2] + ur[2];
+    const float ri = rcp(r);
+    const float u = mx * ri, v = my * ri;
+    const float pd = (g - 1.0f) * (Ud[3] - 0.5f * r * (u * u + v * v));
+    const float un = u * nx + v * ny;
+    s = fabsf(un) + dsqrt(g * (pd + pr) * ri);
+    Fn[0] = r * un;
+    Fn[1] = mx * un + pd * nx;
+    Fn[2] = my * un + pd * ny;
+    Fn[3] = (Ud[3] + pd + hr) * un;
+}  // elements per block tile: 4 waves x 16
 
 template <int P>
 struct NsTile {
@@ -81,6 +103,9 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
     // wave-uniform scalars rounded on the host (lam = viscous_lambda(mu))
     const float gam = sp.f.gamma, mu = sp.f.mu, kappa = sp.f.kappa, lam = sp.f.lam;
     const float* ur = sp.f.uref;
+    // the pressure offset of the momentum fluxes and the energy flux's constant (see
+    // trace_record_dev)
+    const float pr = (gam - 1.0f) * ur[3], hr = ur[3] + pr;
     // storage offset of the constant mode: under the reference's face-gradient read that mode
     // has a populated gradient row, so its operand is the full coefficient
     const float* uo = sp.f.uoff;
@@ -173,7 +198,7 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
             const bool read_is_l = (hcode >> 11) & 1u;
             const int qq = flip ? NFQ - 1 - hq : hq;  // the point in the neighbour's order
             const float4* ht = reinterpret_cast<const float4*>(&tab[S::TOTAL_T]) + (fn * NFQ + qq) * HP;
-            float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
+            float Un[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // the deviation from the storage offset
 #pragma unroll
             for (int i4 = 0; i4 < (NP + 3) / 4; ++i4) {
                 const float4 ph = ht[NP + i4];
@@ -211,8 +236,9 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
                 }
             }
             float Fv[4], Gv[4], Fn[4], sw;
-            viscous_flux_l(Un, gq, gam, mu, lam, kappa, Fv, Gv);
-            trace_record(Un, hnx, hny, gam, Fn, sw);
+            const float Uf_[4] = {Un[0] + ur[0], Un[1] + ur[1], Un[2] + ur[2], Un[3] + ur[3]};
+            viscous_flux_l(Uf_, gq, gam, mu, lam, kappa, Fv, Gv);
+            trace_record_dev(Un, ur, pr, hr, hnx, hny, gam, Fn, sw);
             const float cn_w = read_is_l ? cR : cL;  // the neighbour holds the other side
             const int k = NREC + hk * NFQ + hq;
             H.U[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
@@ -357,7 +383,8 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
             const int fp = mf_fpoint(NFQ, g, f);
             if (f == 3 && NFQ < 5) continue;
             const int face = fp < 0 ? 0 : fp / NFQ, q = fp < 0 ? 0 : fp % NFQ;
-            const float U[4] = {Uf[0][f] + ur[0], Uf[1][f] + ur[1], Uf[2][f] + ur[2], Uf[3][f] + ur[3]};
+            const float Ud[4] = {Uf[0][f], Uf[1][f], Uf[2][f], Uf[3][f]};  // deviation
+            const float U[4] = {Ud[0] + ur[0], Ud[1] + ur[1], Ud[2] + ur[2], Ud[3] + ur[3]};
             float gq[8];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -370,10 +397,10 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
             const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
             float Fv[4], Gv[4], Fn[4], sw;
             viscous_flux_l(U, gq, gam, mu, lam, kappa, Fv, Gv);
-            trace_record(U, nx, ny, gam, Fn, sw);
+            trace_record_dev(Ud, ur, pr, hr, nx, ny, gam, Fn, sw);
             if (fp >= 0) {
                 const int k = (face * NFQ + q) * EB + eloc;
-                H.U[k] = make_float4(U[0], U[1], U[2], U[3]);
+                H.U[k] = make_float4(Ud[0], Ud[1], Ud[2], Ud[3]);
                 H.G[k] = make_float4(Fn[0], Fn[1] + wv * (Fv[1] * nx + Gv[1] * ny), Fn[2] + wv * (Fv[2] * nx + Gv[2] * ny),
                                      Fn[3] + wv * (Fv[3] * nx + Gv[3] * ny));
                 H.S[k] = sw;
@@ -426,7 +453,10 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
             const float Uq[4] = {U[0][r] + ur[0], U[1][r] + ur[1], U[2][r] + ur[2], U[3][r] + ur[3]};
             const float ri = rcp(Uq[0]);
             const float uu = Uq[1] * ri, vv = Uq[2] * ri;
-            const float p = (gam - 1.0f) * (Uq[3] - 0.5f * Uq[0] * (uu * uu + vv * vv));
+            // the pressure relative to pr and the total energy flux's (E + p) from the deviation
+            // (see trace_record_dev)
+            const float p = (gam - 1.0f) * (U[3][r] - 0.5f * Uq[0] * (uu * uu + vv * vv));
+            const float ep = U[3][r] + p + hr;
             const float c1 = uu * a00 + vv * a10, c2 = uu * a01 + vv * a11;
             float gq[8];
 #pragma unroll
@@ -443,8 +473,8 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
             gh[1] = Uq[1] * c2 + p * a01 - (a01 * Fv[1] + a11 * Gv[1]);
             fh[2] = Uq[2] * c1 + p * a10 - (a00 * Fv[2] + a10 * Gv[2]);
             gh[2] = Uq[2] * c2 + p * a11 - (a01 * Fv[2] + a11 * Gv[2]);
-            fh[3] = (Uq[3] + p) * c1 - (a00 * Fv[3] + a10 * Gv[3]);
-            gh[3] = (Uq[3] + p) * c2 - (a01 * Fv[3] + a11 * Gv[3]);
+            fh[3] = ep * c1 - (a00 * Fv[3] + a10 * Gv[3]);
+            gh[3] = ep * c2 - (a01 * Fv[3] + a11 * Gv[3]);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 acc[v] = mfma16x16x4(lt[(S::M_PX + r) * 64], fh[v], acc[v]);
@@ -521,7 +551,8 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
             const float sg2 = face == 0 ? fsg2[0] : face == 1 ? fsg2[1] : fsg2[2];
             const int ko = (face * NFQ + q) * EB + eloc;
             const float4 uo = H.U[ko], go = H.G[ko];
-            const float Uo[4] = {uo.x, uo.y, uo.z, uo.w};
+            const float Uod[4] = {uo.x, uo.y, uo.z, uo.w};  // deviation
+            const float Uo[4] = {uo.x + ur[0], uo.y + ur[1], uo.z + ur[2], uo.w + ur[3]};
             const float Go[4] = {go.x, go.y, go.z, go.w};
             // ghost state and the condition's state U_bc at the point (:249-264), taken as
             // conserved except for NO_SLIP
@@ -534,10 +565,14 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Ub[v] = float(bcr[5 + v]);
             }
-            trace_record(Ug, nx, ny, gam, Fg, sg);
+            float Ugd[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Ugd[v] = Ug[v] - ur[v];
+            trace_record_dev(Ugd, ur, pr, hr, nx, ny, gam, Fg, sg);
             const float s = ref_max(H.S[ko], sg);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Hs[v][f] = hs * ((Go[v] + Fg[v]) - s * (Ug[v] - Uo[v]) - sg2 * (Ub[v] - Uo[v]));
+            for (int v = 0; v < 4; ++v)
+                Hs[v][f] = hs * ((Go[v] + Fg[v]) - s * (Ugd[v] - Uod[v]) - sg2 * ((Ub[v] - ur[v]) - Uod[v]));
         }
     }
     // lift: acc += LF^T H
