@@ -540,14 +540,19 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
             const int q = f < 3 ? g : 4;
             const int bf = reinterpret_cast<const int2*>(sp.ext)[xs + int(c & 2047u)].x;
             const double* bcr = bc_row(sp.bc, bf, NFQ, q);
-            if (bcr[0] < 0.0) {  // no condition: the face contributes nothing
-#pragma unroll
-                for (int v = 0; v < 4; ++v) Hs[v][f] = 0.0f;
-                continue;
-            }
             const float nx = face == 0 ? fnx[0] : face == 1 ? fnx[1] : fnx[2];
             const float ny = face == 0 ? fny[0] : face == 1 ? fny[1] : fny[2];
             const float hs = 0.5f * (face == 0 ? fsf[0] : face == 1 ? fsf[1] : fsf[2]);
+            if (bcr[0] < 0.0) {
+                // no condition: the face contributes nothing, which with the momentum fluxes'
+                // pressure taken relative to pr is the flux of -pr alone (twice the face flux
+                // of every other face carries -2 pr n)
+                Hs[0][f] = 0.0f;
+                Hs[1][f] = hs * (-2.0f * pr * nx);
+                Hs[2][f] = hs * (-2.0f * pr * ny);
+                Hs[3][f] = 0.0f;
+                continue;
+            }
             const float sg2 = face == 0 ? fsg2[0] : face == 1 ? fsg2[1] : fsg2[2];
             const int ko = (face * NFQ + q) * EB + eloc;
             const float4 uo = H.U[ko], go = H.G[ko];

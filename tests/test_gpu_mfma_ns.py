@@ -88,6 +88,21 @@ def test_ns_mfma_residual_matches_lane_kernel_and_oracle(p, face_grad, case):
 
 
 @pytest.mark.parametrize("p", [3, 4])
+@pytest.mark.parametrize("case", ["periodic", "mixed"])
+def test_ns_mfma_residual_with_storage_offset(p, case):
+    # Once a state is set the FP32 state carries a storage offset, and the matrix-core kernel
+    # forms its fluxes from the deviation, with the momentum pressure relative to a constant;
+    # every face must carry that constant's flux, the mixed box's face without a condition too.
+    ora, mf, ln, u0 = _setup(p, case)
+    for c in (mf, ln):
+        c.set_state(u0)
+    r_mf, r_ln, r_ref = mf.residual(u0), ln.residual(u0), ora.residual(u0)
+    e_mf, e_ln = _rel(r_mf, r_ref), _rel(r_ln, r_ref)
+    assert e_mf < 5e-5 and e_ln < 5e-5, (e_mf, e_ln)
+    assert e_mf < 2 * e_ln + 1e-6, (e_mf, e_ln)
+
+
+@pytest.mark.parametrize("p", [3, 4])
 def test_ns_mfma_viscous_part_matches_oracle(p):
     # the viscous contribution alone (NS residual minus the Euler residual of the same state)
     # is held to the same bar, so a wrong viscous term cannot hide under the inviscid one
