@@ -52,8 +52,9 @@ struct TraceRecs {
 };
 
 // One 64-element tile (tile index `tile`, elements 64 tile ..).  The caller separates
-// consecutive tiles of one block with a barrier.
-template <int P, bool BF16>
+// consecutive tiles of one block with a barrier.  LATE_EXT places the external-face traces
+// after the volume work instead of before it.
+template <int P, bool BF16, bool LATE_EXT>
 __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile, MfmaLds<P, BF16>& L,
                                                 float4* __restrict__ dyn) {
     using S = MfmaShape<P>;
@@ -82,9 +83,12 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
 
     // All global traffic goes through buffer resources (the per-lane part of an address is one
     // 32-bit offset, the variable or geometry row a scalar offset) and every load is issued up
-    // front in dependency order, so a wave waits for memory once.  A padding mode slot gets a
-    // per-lane offset past the end of its resource, which the range check turns into a zero
-    // read or a dropped write.
+    // front.  A padding mode slot gets a per-lane offset past the end of its resource, which
+    // the range check turns into a zero read or a dropped write.  The external-face list entry
+    // goes first: the neighbour-coefficient loads depend on it, and the wait for it (the memory
+    // counter drains in issue order) then leaves every load issued after it in flight.
+    const int2 hx = reinterpret_cast<const int2*>(sp.halo)[tile * 64 + lane];
+    __builtin_amdgcn_sched_barrier(0);
     const unsigned state_bytes = unsigned(NP) * 16u * uKp;
     const auto r_uk = make_rsrc(sp.uk, state_bytes);
     const auto r_geo = make_rsrc(sp.geo32, unsigned(kGeo32Rows) * 4u * uKp);
@@ -118,14 +122,16 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
             0, 0));
     }
     // the tables, staged in LDS by cooperative loads (so that no lane holds its table entries in
-    // registers while the loads are in flight)
     constexpr int NTAB = MfmaLds<P, BF16>::NTAB;
+    // registers while the loads are in flight); entries past the table read at an offset past
+    // the resource (zeros), so no branch stands between these loads and that wait
     constexpr int kTabPerThread = (NTAB + kBlock - 1) / kBlock;
+    const auto r_tab = make_rsrc(tm, unsigned(NTAB) * 4u);
     float tabv[kTabPerThread];
 #pragma unroll
     for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        tabv[k] = t < NTAB ? tm[t] : 0.0f;
+        tabv[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_tab, unsigned(t) * 4u, 0, 0));
     }
     // External faces with a neighbour: lane k of every wave takes face k of the tile (64 at a
     // time), each wave its own points of the face (point w, w + 4), so the four waves share the
@@ -134,8 +140,9 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     int hk = lane;
     uint32_t hcode = 0;
     float cn[NP][4];
+    __builtin_amdgcn_sched_barrier(0);
     {
-        const int2 x = reinterpret_cast<const int2*>(sp.halo)[tile * 64 + lane];
+        const int2 x = hx;
         hcode = uint32_t(x.y);
         // no neighbour (x = -1): an offset past the resource, a zero read without traffic; the
         // mode row is a scalar offset
@@ -160,47 +167,53 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     __builtin_amdgcn_sched_barrier(0);
 
     // ---------------- traces of the external faces ----------------------------------------
-    for (;;) {
-        if (hk < nint) {
-            const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
-            const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
-            // records along the neighbour's outward normal, the negated normal of the reading
-            // element's face
-            const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
-            const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
+    // Run either first, or last before the barrier that publishes the records: last lets the
+    // neighbour-coefficient loads arrive under the volume work, at the cost of their registers
+    // staying live through it (at p = 4 that spills; p = 2 is a little slower too).
+    auto external_traces = [&]() {
+        for (;;) {
+            if (hk < nint) {
+                const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
+                const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
+                // records along the neighbour's outward normal, the negated normal of the reading
+                // element's face
+                const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
+                const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
 #pragma unroll
-            for (int q0 = 0; q0 < NFQ; q0 += 4) {
-                const int q = q0 + wu;
-                if (q >= NFQ) continue;
-                float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
+                for (int q0 = 0; q0 < NFQ; q0 += 4) {
+                    const int q = q0 + wu;
+                    if (q >= NFQ) continue;
+                    float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        const float t = T[i * NFQ + q];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) Un[v] = fma(t, cn[i][v], Un[v]);
+                    }
+                    float Fn[4], sw;
+                    trace_record(Un, hnx, hny, gam, Fn, sw);
+                    const int k = NREC + hk * NFQ + q;
+                    H.U[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
+                    H.F[k] = make_float4(Fn[0], Fn[1], Fn[2], Fn[3]);
+                    H.S[k] = sw;
+                }
+            }
+            // more than 64 such faces (tiles that are not compact patches): the next 64
+            hk += 64;
+            if (hk - lane >= nint) break;
+            if (hk < nint) {
+                const int2 x = reinterpret_cast<const int2*>(sp.ext)[xs + hk];
+                hcode = uint32_t(x.y);
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
-                    const float t = T[i * NFQ + q];
+                    const v4u y = load_x4(r_uk, (unsigned(i) * uKp + unsigned(x.x)) * 16u);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) Un[v] = fma(t, cn[i][v], Un[v]);
+                    for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
                 }
-                float Fn[4], sw;
-                trace_record(Un, hnx, hny, gam, Fn, sw);
-                const int k = NREC + hk * NFQ + q;
-                H.U[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
-                H.F[k] = make_float4(Fn[0], Fn[1], Fn[2], Fn[3]);
-                H.S[k] = sw;
             }
         }
-        // more than 64 such faces (tiles that are not compact patches): the next 64
-        hk += 64;
-        if (hk - lane >= nint) break;
-        if (hk < nint) {
-            const int2 x = reinterpret_cast<const int2*>(sp.ext)[xs + hk];
-            hcode = uint32_t(x.y);
-#pragma unroll
-            for (int i = 0; i < NP; ++i) {
-                const v4u y = load_x4(r_uk, (unsigned(i) * uKp + unsigned(x.x)) * 16u);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
-            }
-        }
-    }
+    };
+    if constexpr (!LATE_EXT) external_traces();
 
     // this lane's entries of the product tables (A operands)
     const float* lt = &L.tab[lane];
@@ -322,6 +335,7 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
             acc[v] = mfma16x16x16bf16(yb[0], yb[64], pack_bf16x2(Gh[v][0], Gh[v][1]), pack_bf16x2(Gh[v][2], Gh[v][3]), acc[v]);
         }
     }
+    if constexpr (LATE_EXT) external_traces();
     __syncthreads();  // own traces and external-face traces complete
 
     // ---------------- faces: slot (g, f) = face f, point g (f < 3); face g, point 4 (f = 3) ---
@@ -435,10 +449,11 @@ __device__ __forceinline__ float4* halo_lds() {
     return s_halo;
 }
 
+// single contexts: the external-face traces last at p = 3 (measured 2% faster there)
 template <int P, bool BF16>
 __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     __shared__ MfmaLds<P, BF16> L;
-    stage_mfma_tile<P, BF16>(sp, xcd_tile(blockIdx.x, gridDim.x), L, halo_lds());
+    stage_mfma_tile<P, BF16, P == 3>(sp, xcd_tile(blockIdx.x, gridDim.x), L, halo_lds());
 }
 
 // Batched instances of one order: each block walks its share of the work items (instance,
@@ -452,7 +467,7 @@ __device__ __forceinline__ void batch_stage_mfma_body(const StageParams* __restr
         const BatchItem item = items[it];
         const StageParams& sp = params[item.instance];
         for (int e0 = item.e0; e0 < item.e0 + item.count; e0 += kTileEl) {
-            stage_mfma_tile<P, false>(sp, e0 / kTileEl, L, halo_lds());
+            stage_mfma_tile<P, false, false>(sp, e0 / kTileEl, L, halo_lds());
             __syncthreads();  // the next tile overwrites the staged geometry and traces
         }
     }
