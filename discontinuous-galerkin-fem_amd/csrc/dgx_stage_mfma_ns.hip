@@ -120,7 +120,14 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
     // record weights of the viscous normal flux: L side, R side, boundary face
     const float cL = sp.ns_avg_ref ? 1.0f : -1.0f, cR = -1.0f, cB = 2.0f * cL;
 
-    // ---------------- loads, issued up front in dependency order ----------------------------
+    // ---------------- loads, issued up front ---------------------------------------------------
+    // The external-face list entry of this thread's first pair goes first: the neighbour loads
+    // depend on it, and the wait for it (the memory counter drains in issue order) then leaves
+    // every load issued after it in flight.
+    const int npair = nint * NFQ;
+    int hp = (int(threadIdx.x) + 64 * (tile & 3)) & (kBlock - 1);
+    const int2 hx = reinterpret_cast<const int2*>(sp.halo)[tile * 64 + hp / NFQ];
+    __builtin_amdgcn_sched_barrier(0);
     const unsigned state_bytes = unsigned(NP) * 16u * uKp;
     const auto r_uk = make_rsrc(sp.uk, state_bytes);
     const auto r_geo = make_rsrc(sp.geo32, unsigned(kGeo32NsRows) * 4u * uKp);
@@ -144,21 +151,21 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
 #pragma unroll
         for (int v = 0; v < 4; ++v) b[v][s] = __uint_as_float(x[v]);
     }
-    // face-trace and external-face tables, staged in LDS by cooperative loads
+    // face-trace and external-face tables, staged in LDS by cooperative loads (entries past the
+    // table read at an offset past the resource: zeros, and no branch before that wait)
     constexpr int kT1 = (T::NT1 + kBlock - 1) / kBlock;
+    const auto r_t1 = make_rsrc(tt, unsigned(T::NT1) * 4u);
     float t1v[kT1];
 #pragma unroll
     for (int k = 0; k < kT1; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        t1v[k] = t < T::NT1 ? tt[t] : 0.0f;
+        t1v[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_t1, unsigned(t) * 4u, 0, 0));
     }
     // External-face points: pair t = (face hk, point hq) of the tile's external faces with a
     // neighbour, one per thread, dealt from wave (tile mod 4) on so that the tiles sharing a CU
     // load its four SIMDs evenly.  The first 256 pairs (256 / Nfq < 64 faces) come from the
     // fixed-stride copy of the tile's first 64 external faces; neighbour coefficients, dxi_dx and
     // the reading element's normal of the thread's first pair are loaded here.
-    const int npair = nint * NFQ;
-    int hp = (int(threadIdx.x) + 64 * (tile & 3)) & (kBlock - 1);
     float cn[NP][4];
     float hA[4], hnx, hny;
     uint32_t hcode;
@@ -180,7 +187,8 @@ __device__ __forceinline__ void ns_tile(const StageParams& sp, int tile, float* 
         hnx = -geo_at_lane(G32_N0X + 2 * rf, unsigned(eb0 + rel));
         hny = -geo_at_lane(G32_N0Y + 2 * rf, unsigned(eb0 + rel));
     };
-    halo_load(reinterpret_cast<const int2*>(sp.halo)[tile * 64 + hp / NFQ]);
+    __builtin_amdgcn_sched_barrier(0);
+    halo_load(hx);
 #pragma unroll
     for (int k = 0; k < kT1; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;

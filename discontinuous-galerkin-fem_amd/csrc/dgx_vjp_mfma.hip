@@ -29,7 +29,11 @@ namespace {
 
 constexpr int kTileEl = kBlock / 4;  // elements per block tile: 4 waves x 16
 
-template <int P>
+// LAT: the latency-bound form for grids of at most two tiles per CU, where registers are not
+// what limits residency: the neighbours' state and adjoint coefficients loaded together up
+// front and the external faces' records formed after the volume work, so that no memory round
+// trip stands between the tile's start and its arithmetic.
+template <int P, bool LAT>
 __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* __restrict__ tab,
                                          float4* __restrict__ dyn) {
     using F = MfmaShape<P>;
@@ -55,7 +59,14 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
     float4* __restrict__ RP = dyn + nrs;   // lift-weighted adjoint traces
     float* __restrict__ RS = reinterpret_cast<float*>(dyn + 2 * nrs);  // wave speeds |u.n| + c
 
-    // ---------------- loads, issued up front in dependency order ----------------------------
+    // ---------------- loads, issued up front ---------------------------------------------------
+    // The external-face list entry of this thread's first pair goes first: the neighbour loads
+    // depend on it, and the wait for it (the memory counter drains in issue order) then leaves
+    // every load issued after it in flight.
+    const int npair = nint * NFQ;
+    int hp = (int(threadIdx.x) + 64 * (tile & 3)) & (kBlock - 1);
+    const int2 hx = reinterpret_cast<const int2*>(sp.halo)[tile * 64 + hp / NFQ];
+    __builtin_amdgcn_sched_barrier(0);
     const unsigned state_bytes = unsigned(NP) * 16u * uKp;
     const auto r_uk = make_rsrc(sp.uk, state_bytes);
     const auto r_lam = make_rsrc(sp.lam, state_bytes);
@@ -83,47 +94,22 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
             lb[v][s] = __uint_as_float(y[v]);
         }
     }
-    // RK stage adjoint: the accumulator acc_out = acc_in + ca lambda and the output's part that
-    // needs no face or volume term, X = cb lambda + add, formed as soon as lambda is in (so
-    // that no load waits at the end of the tile)
-    const bool rk = sp.mode == MODE_RK;
-    float xo[4][R];
-    if (rk) {
-        const auto r_add = make_rsrc(sp.add, sp.add ? state_bytes : 0u);
-        const auto r_ain = make_rsrc(sp.acc_in, sp.acc_in ? state_bytes : 0u);
-        const auto r_aout = make_rsrc(sp.acc_out, sp.acc_out ? state_bytes : 0u);
-        const float cb = sp.f.cb, ca = sp.f.ca;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const unsigned vo = slot_off(r, unsigned(ec));
-            v4u ad = {0u, 0u, 0u, 0u}, ai = {0u, 0u, 0u, 0u};  // absent buffers read zero
-            if (sp.add) ad = load_x4(r_add, vo);
-            if (sp.acc_in) ai = load_x4(r_ain, vo);
-            v4u a;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                xo[v][r] = cb * lb[v][r] + __uint_as_float(ad[v]);
-                a[v] = __float_as_uint(__uint_as_float(ai[v]) + ca * lb[v][r]);
-            }
-            if (sp.acc_out && valid) __builtin_amdgcn_raw_buffer_store_b128(a, r_aout, vo, 0, 0);
-        }
-    }
-    // the tables, staged in LDS by cooperative loads
+    // the tables, staged in LDS by cooperative loads (entries past the table read at an offset
+    // past the resource: zeros, and no branch before that wait)
     constexpr int kTabPerThread = (S::TOTAL + kBlock - 1) / kBlock;
+    const auto r_tab = make_rsrc(tm, unsigned(S::TOTAL) * 4u);
     float tabv[kTabPerThread];
 #pragma unroll
     for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
-        tabv[k] = t < S::TOTAL ? tm[t] : 0.0f;
+        tabv[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_tab, unsigned(t) * 4u, 0, 0));
     }
     // External-face points: pair t = (face hk, point hq) of the tile's external faces with a
     // neighbour, one per thread, dealt from wave (tile mod 4) on; the first 256 pairs (256 / Nfq
     // < 64 faces) come from the fixed-stride copy of the tile's first 64 external faces.  The
     // neighbour's state coefficients of the thread's first pair are loaded here, its adjoint
     // coefficients once its state trace is formed.
-    const int npair = nint * NFQ;
-    int hp = (int(threadIdx.x) + 64 * (tile & 3)) & (kBlock - 1);
-    float cn[NP][4];
+    float cn[NP][4], cl[NP][4];  // the neighbour's state and (LAT) adjoint coefficients
     float hsf;       // the neighbour's lift scale of the shared face
     float hnx, hny;  // the reading element's outward normal of the face
     uint32_t hcode;
@@ -138,13 +124,59 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
             const v4u y = load_x4(r_uk, has ? (unsigned(i) * uKp + hen) * 16u : state_bytes);
 #pragma unroll
             for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
+            if constexpr (LAT) {
+                const v4u z = load_x4(r_lam, has ? (unsigned(i) * uKp + hen) * 16u : state_bytes);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cl[i][v] = __uint_as_float(z[v]);
+            }
         }
         hsf = geo_at_lane(G32_S0 + int(hcode & 3u), hen);
         const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
         hnx = geo_at_lane(G32_N0X + 2 * rf, unsigned(eb0 + rel));
         hny = geo_at_lane(G32_N0Y + 2 * rf, unsigned(eb0 + rel));
     };
-    halo_load(reinterpret_cast<const int2*>(sp.halo)[tile * 64 + hp / NFQ]);
+    __builtin_amdgcn_sched_barrier(0);
+    halo_load(hx);
+    // RK stage adjoint: the accumulator acc_out = acc_in + ca lambda and the output's part that
+    // needs no face or volume term, X = cb lambda + add.  LAT: the addend and accumulator are
+    // loaded here and used at the output, so the tile's first arithmetic does not wait for them
+    // (an absent buffer is a resource of size zero over lambda's memory: its loads read zero
+    // without traffic).  Otherwise X and acc_out are formed here, as soon as lambda is in, to
+    // keep those registers free.
+    const bool rk = sp.mode == MODE_RK;
+    const bool has_add = rk && sp.add, has_ain = rk && sp.acc_in;
+    const auto r_add = make_rsrc(has_add ? sp.add : sp.lam, has_add ? state_bytes : 0u);
+    const auto r_ain = make_rsrc(has_ain ? sp.acc_in : sp.lam, has_ain ? state_bytes : 0u);
+    const auto r_aout = make_rsrc(sp.acc_out, sp.acc_out ? state_bytes : 0u);
+    float xo[4][R];
+    v4u adv[LAT ? R : 1], aiv[LAT ? R : 1];
+    // X and acc_out of mode slot r from the addend and accumulator words
+    auto rk_terms = [&](int r, const v4u& ad, const v4u& ai) {
+        const float cb = sp.f.cb, ca = sp.f.ca;
+        v4u a;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            xo[v][r] = cb * lb[v][r] + __uint_as_float(ad[v]);
+            a[v] = __float_as_uint(__uint_as_float(ai[v]) + ca * lb[v][r]);
+        }
+        if (sp.acc_out && valid) __builtin_amdgcn_raw_buffer_store_b128(a, r_aout, slot_off(r, unsigned(ec)), 0, 0);
+    };
+    if constexpr (LAT) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            adv[r] = load_x4(r_add, slot_off(r, unsigned(ec)));
+            aiv[r] = load_x4(r_ain, slot_off(r, unsigned(ec)));
+        }
+    } else if (rk) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const unsigned vo = slot_off(r, unsigned(ec));
+            v4u ad = {0u, 0u, 0u, 0u}, ai = {0u, 0u, 0u, 0u};  // absent buffers read zero
+            if (sp.add) ad = load_x4(r_add, vo);
+            if (sp.acc_in) ai = load_x4(r_ain, vo);
+            rk_terms(r, ad, ai);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < kTabPerThread; ++k) {
         const int t = int(threadIdx.x) + k * kBlock;
@@ -155,52 +187,58 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
     __builtin_amdgcn_sched_barrier(0);
 
     // ---------------- records of the external faces' neighbour points ----------------------
-    for (;;) {
-        if (hp < npair) {
-            const int hk = hp / NFQ, hq = hp - hk * NFQ;
-            const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
-            const int qq = flip ? NFQ - 1 - hq : hq;  // the point in the neighbour's order
-            const float4* ht = reinterpret_cast<const float4*>(&tab[S::T_N * 64]) + (fn * NFQ + qq) * HV;
-            float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
+    auto external_records = [&]() {
+        for (;;) {
+            if (hp < npair) {
+                const int hk = hp / NFQ, hq = hp - hk * NFQ;
+                const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
+                const int qq = flip ? NFQ - 1 - hq : hq;  // the point in the neighbour's order
+                const float4* ht = reinterpret_cast<const float4*>(&tab[S::T_N * 64]) + (fn * NFQ + qq) * HV;
+                float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
 #pragma unroll
-            for (int m = 0; m < HV; ++m) {
-                const float4 t = ht[m];
+                for (int m = 0; m < HV; ++m) {
+                    const float4 t = ht[m];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) Un[v] = fma(t.x, cn[2 * m][v], Un[v]);
-                if (2 * m + 1 < NP) {
+                    for (int v = 0; v < 4; ++v) Un[v] = fma(t.x, cn[2 * m][v], Un[v]);
+                    if (2 * m + 1 < NP) {
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) Un[v] = fma(t.z, cn[2 * m + 1][v], Un[v]);
+                        for (int v = 0; v < 4; ++v) Un[v] = fma(t.z, cn[2 * m + 1][v], Un[v]);
+                    }
                 }
-            }
-            // the neighbour's adjoint coefficients, in the same registers
+                // the neighbour's adjoint coefficients (otherwise loaded here, into the same registers)
+                if constexpr (!LAT) {
 #pragma unroll
-            for (int i = 0; i < NP; ++i) {
-                const v4u y = load_x4(r_lam, (unsigned(i) * uKp + hen) * 16u);
+                    for (int i = 0; i < NP; ++i) {
+                        const v4u y = load_x4(r_lam, (unsigned(i) * uKp + hen) * 16u);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
-            }
-            float Pn[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int m = 0; m < HV; ++m) {
-                const float4 t = ht[m];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) Pn[v] = fma(t.y, cn[2 * m][v], Pn[v]);
-                if (2 * m + 1 < NP) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) Pn[v] = fma(t.w, cn[2 * m + 1][v], Pn[v]);
+                        for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
+                    }
                 }
+                const auto& ca_n = LAT ? cl : cn;
+                float Pn[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int m = 0; m < HV; ++m) {
+                    const float4 t = ht[m];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Pn[v] = fma(t.y, ca_n[2 * m][v], Pn[v]);
+                    if (2 * m + 1 < NP) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) Pn[v] = fma(t.w, ca_n[2 * m + 1][v], Pn[v]);
+                    }
+                }
+                const int k = NREC + hk * NFQ + hq;
+                RU[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
+                RP[k] = make_float4(hsf * Pn[0], hsf * Pn[1], hsf * Pn[2], hsf * Pn[3]);
+                RS[k] = wave_speed(Un, hnx, hny, gam);
             }
-            const int k = NREC + hk * NFQ + hq;
-            RU[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
-            RP[k] = make_float4(hsf * Pn[0], hsf * Pn[1], hsf * Pn[2], hsf * Pn[3]);
-            RS[k] = wave_speed(Un, hnx, hny, gam);
+            // more than 256 external-face points (tiles that are not compact patches): the thread's
+            // next pair, 256 on
+            hp += kBlock;
+            if (hp >= npair) break;
+            halo_load(reinterpret_cast<const int2*>(sp.ext)[xs + hp / NFQ]);
         }
-        // more than 256 external-face points (tiles that are not compact patches): the thread's
-        // next pair, 256 on
-        hp += kBlock;
-        if (hp >= npair) break;
-        halo_load(reinterpret_cast<const int2*>(sp.ext)[xs + hp / NFQ]);
-    }
+    };
+    if constexpr (!LAT) external_records();
 
     // ---------------- own face points -------------------------------------------------------
     // this lane's element geometry (its latency covered by the trace products below)
@@ -285,6 +323,7 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
             ub[v] = c;
         }
     }
+    if constexpr (LAT) external_records();
     __syncthreads();  // every record of the tile complete
 
     // ---------------- faces --------------------------------------------------------------
@@ -367,6 +406,12 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
         for (int t = 0; t < RF; ++t) ub[v] = mfma16x16x4(lt[(S::T_FT + t) * 64], Hb[v][t], ub[v]);
 
     // ---------------- output ---------------------------------------------------------------
+    if constexpr (LAT) {
+        if (rk) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) rk_terms(r, adv[r], aiv[r]);
+        }
+    }
     if (!valid) return;  // (no barrier follows inside the tile)
     const auto r_out = make_rsrc(sp.out, state_bytes);
     const float fs = rk ? sp.f.cb * sp.f.dt : 1.0f;  // weight of (dL/du)^T lambda in the output
@@ -385,17 +430,21 @@ __device__ __forceinline__ float4* vjp_records_lds() {
     return s_vjp_rec;
 }
 
-template <int P>
+template <int P, bool LAT>
 __device__ __forceinline__ void vjp_tile_body(const VjpParams& sp) {
     __shared__ float4 s_tab[(MfmaVjpShape<P>::TOTAL + 3) / 4];
-    vjp_tile<P>(sp, xcd_tile(blockIdx.x, gridDim.x), reinterpret_cast<float*>(s_tab), vjp_records_lds());
+    vjp_tile<P, LAT>(sp, xcd_tile(blockIdx.x, gridDim.x), reinterpret_cast<float*>(s_tab), vjp_records_lds());
 }
 
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void vjp_mfma_kernel_p3(VjpParams sp) {
-    vjp_tile_body<3>(sp);
+    vjp_tile_body<3, false>(sp);
 }
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void vjp_mfma_kernel_p4(VjpParams sp) {
-    vjp_tile_body<4>(sp);
+    vjp_tile_body<4, false>(sp);
+}
+// at most two tiles per CU: two waves per SIMD suffice (p = 3; at p = 4 this form spills)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8))) void vjp_mfma_lat_kernel_p3(VjpParams sp) {
+    vjp_tile_body<3, true>(sp);
 }
 
 // dynamic LDS: U, P (float4 each) and the wave speed of the tile's own face points and of its
@@ -413,8 +462,25 @@ hipError_t launch_vjp_mfma(int p, const VjpParams& vp, hipStream_t s) {
     if (!vp.geo32 || !vp.conn || !vp.ext || !vp.ext_off || !vp.ext_nint || !vp.halo) return hipErrorInvalidValue;
     dim3 grid((vp.K + 63) / 64), block(kBlock);
     const size_t lds = vjp_records_bytes(p, vp.halo_cap);
+    // the CU count of the current device, queried once per device
+    static int cus[64] = {};
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (dev >= 0 && dev < 64 && cus[dev] == 0) {
+        int n = 0;
+        err = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        if (err != hipSuccess) return err;
+        cus[dev] = n > 0 ? n : 1;
+    }
+    const int ncu = dev >= 0 && dev < 64 ? cus[dev] : 1;
+    // p = 3 grids of at most two tiles per CU: the latency-bound form (p = 4 would spill)
+    const bool lat = p == 3 && int(grid.x) <= 2 * ncu;
     switch (p) {
-    case 3: hipLaunchKernelGGL(vjp_mfma_kernel_p3, grid, block, lds, s, vp); break;
+    case 3:
+        if (lat) hipLaunchKernelGGL(vjp_mfma_lat_kernel_p3, grid, block, lds, s, vp);
+        else hipLaunchKernelGGL(vjp_mfma_kernel_p3, grid, block, lds, s, vp);
+        break;
     case 4: hipLaunchKernelGGL(vjp_mfma_kernel_p4, grid, block, lds, s, vp); break;
     default: return hipErrorInvalidValue;
     }
