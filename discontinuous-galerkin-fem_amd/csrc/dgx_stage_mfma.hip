@@ -474,7 +474,7 @@ __device__ __forceinline__ void batch_stage_mfma_body(const StageParams* __restr
 }
 
 // waves per SIMD as measured best on gfx950: p = 3 is asked for three (it gets four), p = 4
-// for four; p = 1, 2 for four
+// for four; p = 1, 2 for four; the batched p = 3 kernel for four (three at 129 VGPRs otherwise)
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void stage_mfma_kernel_p1(StageParams sp) {
     stage_mfma_body<1, false>(sp);
 }
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     const StageParams* params, const BatchItem* items, int n_items) {
     batch_stage_mfma_body<2>(params, items, n_items);
 }
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void batch_stage_mfma_kernel_p3(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void batch_stage_mfma_kernel_p3(
     const StageParams* params, const BatchItem* items, int n_items) {
     batch_stage_mfma_body<3>(params, items, n_items);
 }
