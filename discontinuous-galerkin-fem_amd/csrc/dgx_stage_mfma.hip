@@ -52,9 +52,10 @@ struct TraceRecs {
 };
 
 // One 64-element tile (tile index `tile`, elements 64 tile ..).  The caller separates
-// consecutive tiles of one block with a barrier.  LATE_EXT places the external-face traces
-// after the volume work instead of before it.
-template <int P, bool BF16, bool LATE_EXT>
+// consecutive tiles of one block with a barrier.  LATE places the external-face traces after
+// the volume work instead of before it, and issues the u^n loads of the RK blend before the face
+// pass instead of at the output.
+template <int P, bool BF16, bool LATE>
 __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile, MfmaLds<P, BF16>& L,
                                                 float4* __restrict__ dyn) {
     using S = MfmaShape<P>;
@@ -213,7 +214,7 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
             }
         }
     };
-    if constexpr (!LATE_EXT) external_traces();
+    if constexpr (!LATE) external_traces();
 
     // this lane's entries of the product tables (A operands)
     const float* lt = &L.tab[lane];
@@ -335,8 +336,17 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
             acc[v] = mfma16x16x16bf16(yb[0], yb[64], pack_bf16x2(Gh[v][0], Gh[v][1]), pack_bf16x2(Gh[v][2], Gh[v][3]), acc[v]);
         }
     }
-    if constexpr (LATE_EXT) external_traces();
+    if constexpr (LATE) external_traces();
     __syncthreads();  // own traces and external-face traces complete
+    // LATE: u^n for the second and third RK stages, issued here so that the face pass covers its
+    // latency (a resource of size zero when there is no blend: zero reads without traffic)
+    const auto r_un = make_rsrc(rk_blend ? sp.un : sp.uk, rk_blend ? state_bytes : 0u);
+    v4u unx[LATE ? R : 1];
+    if constexpr (LATE) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) unx[r] = load_x4(r_un, slot_off(r, unsigned(ec)));
+        __builtin_amdgcn_sched_barrier(0);
+    }
 
     // ---------------- faces: slot (g, f) = face f, point g (f < 3); face g, point 4 (f = 3) ---
     // From the two trace records of a face point, own (o) and neighbour (n), each with its normal
@@ -432,10 +442,9 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     } else {
         // u^n for the second and third RK stages
         float unv[4][R];
-        const auto r_un = make_rsrc(sp.un, state_bytes);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const v4u x = load_x4(r_un, slot_off(r, unsigned(e)));
+            const v4u x = LATE ? unx[LATE ? r : 0] : load_x4(r_un, slot_off(r, unsigned(e)));
 #pragma unroll
             for (int v = 0; v < 4; ++v) unv[v][r] = __uint_as_float(x[v]);
         }
@@ -449,7 +458,7 @@ __device__ __forceinline__ float4* halo_lds() {
     return s_halo;
 }
 
-// single contexts: the external-face traces last at p = 3 (measured 2% faster there)
+// single contexts: the latency order at p = 3 (measured 4% faster there, slower at p = 2, 4)
 template <int P, bool BF16>
 __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     __shared__ MfmaLds<P, BF16> L;
