@@ -169,6 +169,41 @@ def test_mfma_rk3_vjp_matches_lane(p):
     assert np.linalg.norm(g[0] - g[1]) / np.linalg.norm(g[1]) < 1e-5
 
 
+@pytest.mark.parametrize("n", [128, 132])
+def test_mfma_vjp_both_launch_forms(n):
+    # A p = 3 stage adjoint grid of at most two tiles per CU (n = 128: 512 tiles, two per CU of
+    # an MI355X) launches the kernel's latency form, a larger one (n = 132: 545 tiles) the other
+    # form.  Both against the FP64 adjoint, beside the per-lane FP32 adjoint: the stage adjoints
+    # (residual and right-hand side) to 1e-6 of their largest entry (measured 2-3e-7), and the
+    # trajectory adjoint over two RK3 steps with a checkpoint per step, whose FP32 error is that
+    # of the FP32 trajectory (6e-5 relative L2 for both kernels, most of it in rho v) and is held
+    # to 2e-4 and to twice the per-lane kernel's.
+    verts, tris = R.split_quad_mesh(n, 10.0)
+    mf = _context(False, verts, tris, 3, basis="dubiner", precision="fp32")
+    ln = _context(True, verts, tris, 3, basis="dubiner", precision="fp32")
+    c64 = _context(False, verts, tris, 3, basis="dubiner", precision="fp64")
+    assert mf.info.stage_kernel == 1 and ln.info.stage_kernel == 0
+    pert = np.repeat(0.01 * R.uniform_pm1(5, np.arange(mf.K))[:, None], mf.Nq, axis=1)
+    c64.project(R.vortex_state(c64.volume_points(), rho_pert=pert))
+    u0 = c64.get_state()
+    lam = np.random.default_rng(3).standard_normal(u0.shape)
+    for f in ("residual_vjp", "rhs_vjp"):
+        g64 = getattr(c64, f)(lam, u0)
+        e_mf = _rel(getattr(mf, f)(lam, u0), g64)
+        e_ln = _rel(getattr(ln, f)(lam, u0), g64)
+        assert e_mf < 1e-6 and e_mf < 2 * e_ln + 1e-7, (f, e_mf, e_ln)
+    dt = dgx.stable_dt(c64, 0.3)
+    lam_T = np.random.default_rng(9).standard_normal(u0.shape)
+    g = []
+    for c in (mf, ln, c64):
+        c.set_state(u0)
+        g.append(c.rk3_vjp(dt, 2, 1, lam_T))
+    e_mf, e_ln = (np.linalg.norm(x - g[2]) / np.linalg.norm(g[2]) for x in g[:2])
+    assert e_mf < 2e-4 and e_mf < 2 * e_ln, (e_mf, e_ln)
+    for c in (mf, ln, c64):
+        c.close()
+
+
 def test_vjp_no_slip_ghost_against_oracle_fd():
     # FP64 adjoint on a bounded mesh with far-field and no-slip faces, by central differences
     # of the oracle residual: the no-slip ghost state is built from primitive values
