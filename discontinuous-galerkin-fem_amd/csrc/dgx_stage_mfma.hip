@@ -20,8 +20,9 @@
 // products leave in each lane.  Every element leaves its own traces in LDS, so a face between
 // two elements of the tile reads the neighbour's trace there (both sides then see the same two
 // matrix-core traces, and both copies of the face flux agree bit for bit).  Only the faces that
-// leave the tile need the neighbour's trace evaluated from its coefficients: every wave does
-// that for the tile's list of external faces, one face per lane and its own points of the face.
+// leave the tile need the neighbour's trace evaluated from its coefficients, from the tile's
+// list of external faces: one face point per thread, or one face per lane with each wave taking
+// its own points of the face.
 #include "dgx_mfma_dev.hpp"
 
 namespace dgx {
@@ -54,8 +55,9 @@ struct TraceRecs {
 // One 64-element tile (tile index `tile`, elements 64 tile ..).  The caller separates
 // consecutive tiles of one block with a barrier.  LATE places the external-face traces after
 // the volume work instead of before it, and issues the u^n loads of the RK blend before the face
-// pass instead of at the output.
-template <int P, bool BF16, bool LATE>
+// pass instead of at the output.  PAIR deals the external faces' points one per thread (else
+// one face per lane, each wave its own points of it).
+template <int P, bool BF16, bool LATE, bool PAIR>
 __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile, MfmaLds<P, BF16>& L,
                                                 float4* __restrict__ dyn) {
     using S = MfmaShape<P>;
@@ -88,7 +90,20 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     // the range check turns into a zero read or a dropped write.  The external-face list entry
     // goes first: the neighbour-coefficient loads depend on it, and the wait for it (the memory
     // counter drains in issue order) then leaves every load issued after it in flight.
-    const int2 hx = reinterpret_cast<const int2*>(sp.halo)[tile * 64 + lane];
+    //
+    // External faces with a neighbour.  PAIR: pair t = (face hk, point hq) of the tile's
+    // external faces, one per thread, dealt from wave (tile mod 4) on so that the tiles sharing a
+    // CU load its four SIMDs evenly (with about half the 256 threads busy, half the issue of one
+    // face per lane).  Otherwise lane k of every wave takes face k (64 at a time), each wave its
+    // own points of the face.  The first 64 faces come from the fixed-stride copy of the list.
+    const int npair = nint * NFQ;
+    int hp = (int(threadIdx.x) + 64 * (tile & 3)) & (kBlock - 1);
+    int hk = lane;
+    auto face_entry = [&](int k) -> int2 {
+        if (kBlock / NFQ <= 64 || k < 64) return reinterpret_cast<const int2*>(sp.halo)[tile * 64 + k];
+        return k < nint ? reinterpret_cast<const int2*>(sp.ext)[xs + k] : int2{-1, 0};
+    };
+    const int2 hx = face_entry(PAIR ? hp / NFQ : lane);
     __builtin_amdgcn_sched_barrier(0);
     const unsigned state_bytes = unsigned(NP) * 16u * uKp;
     const auto r_uk = make_rsrc(sp.uk, state_bytes);
@@ -123,9 +138,9 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
             0, 0));
     }
     // the tables, staged in LDS by cooperative loads (so that no lane holds its table entries in
-    constexpr int NTAB = MfmaLds<P, BF16>::NTAB;
     // registers while the loads are in flight); entries past the table read at an offset past
     // the resource (zeros), so no branch stands between these loads and that wait
+    constexpr int NTAB = MfmaLds<P, BF16>::NTAB;
     constexpr int kTabPerThread = (NTAB + kBlock - 1) / kBlock;
     const auto r_tab = make_rsrc(tm, unsigned(NTAB) * 4u);
     float tabv[kTabPerThread];
@@ -134,11 +149,7 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
         const int t = int(threadIdx.x) + k * kBlock;
         tabv[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_tab, unsigned(t) * 4u, 0, 0));
     }
-    // External faces with a neighbour: lane k of every wave takes face k of the tile (64 at a
-    // time), each wave its own points of the face (point w, w + 4), so the four waves share the
-    // work evenly.  Neighbour coefficients of this lane's first face, all modes of one neighbour
-    // (the first 64 faces come from the fixed-stride copy, one memory step from here).
-    int hk = lane;
+    // neighbour coefficients of this thread's first external face, all modes of one neighbour
     uint32_t hcode = 0;
     float cn[NP][4];
     __builtin_amdgcn_sched_barrier(0);
@@ -172,18 +183,16 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
     // neighbour-coefficient loads arrive under the volume work, at the cost of their registers
     // staying live through it (at p = 4 that spills; p = 2 is a little slower too).
     auto external_traces = [&]() {
-        for (;;) {
-            if (hk < nint) {
-                const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
-                const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
-                // records along the neighbour's outward normal, the negated normal of the reading
-                // element's face
-                const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
-                const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
-#pragma unroll
-                for (int q0 = 0; q0 < NFQ; q0 += 4) {
-                    const int q = q0 + wu;
-                    if (q >= NFQ) continue;
+        if constexpr (PAIR) {
+            for (;;) {
+                if (hp < npair) {
+                    const int hk = hp / NFQ, q = hp - hk * NFQ;
+                    const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
+                    const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
+                    // records along the neighbour's outward normal, the negated normal of the reading
+                    // element's face
+                    const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
+                    const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
                     float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
 #pragma unroll
                     for (int i = 0; i < NP; ++i) {
@@ -198,18 +207,59 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
                     H.F[k] = make_float4(Fn[0], Fn[1], Fn[2], Fn[3]);
                     H.S[k] = sw;
                 }
-            }
-            // more than 64 such faces (tiles that are not compact patches): the next 64
-            hk += 64;
-            if (hk - lane >= nint) break;
-            if (hk < nint) {
-                const int2 x = reinterpret_cast<const int2*>(sp.ext)[xs + hk];
+                // more than 256 external-face points (tiles that are not compact patches): the
+                // thread's next pair, 256 on
+                hp += kBlock;
+                if (hp >= npair) break;
+                const int2 x = reinterpret_cast<const int2*>(sp.ext)[xs + hp / NFQ];
                 hcode = uint32_t(x.y);
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
                     const v4u y = load_x4(r_uk, (unsigned(i) * uKp + unsigned(x.x)) * 16u);
 #pragma unroll
                     for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
+                }
+            }
+        } else {
+            for (;;) {
+                if (hk < nint) {
+                    const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
+                    const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
+                    // records along the neighbour's outward normal, the negated normal of the reading
+                    // element's face
+                    const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
+                    const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
+#pragma unroll
+                    for (int q0 = 0; q0 < NFQ; q0 += 4) {
+                        const int q = q0 + wu;
+                        if (q >= NFQ) continue;
+                        float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
+#pragma unroll
+                        for (int i = 0; i < NP; ++i) {
+                            const float t = T[i * NFQ + q];
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) Un[v] = fma(t, cn[i][v], Un[v]);
+                        }
+                        float Fn[4], sw;
+                        trace_record(Un, hnx, hny, gam, Fn, sw);
+                        const int k = NREC + hk * NFQ + q;
+                        H.U[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
+                        H.F[k] = make_float4(Fn[0], Fn[1], Fn[2], Fn[3]);
+                        H.S[k] = sw;
+                    }
+                }
+                // more than 64 such faces (tiles that are not compact patches): the next 64
+                hk += 64;
+                if (hk - lane >= nint) break;
+                if (hk < nint) {
+                    const int2 x = reinterpret_cast<const int2*>(sp.ext)[xs + hk];
+                    hcode = uint32_t(x.y);
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        const v4u y = load_x4(r_uk, (unsigned(i) * uKp + unsigned(x.x)) * 16u);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) cn[i][v] = __uint_as_float(y[v]);
+                    }
                 }
             }
         }
@@ -458,11 +508,13 @@ __device__ __forceinline__ float4* halo_lds() {
     return s_halo;
 }
 
-// single contexts: the latency order at p = 3 (measured 4% faster there, slower at p = 2, 4)
+// single contexts: the latency order at p = 3 (measured 4% faster there, slower at p = 2, 4);
+// the external faces' points dealt one per thread except at p = 3 in FP32 (there its registers
+// or the uneven waves before the records barrier make it 4% slower; 1-3.5% faster elsewhere)
 template <int P, bool BF16>
 __device__ __forceinline__ void stage_mfma_body(const StageParams& sp) {
     __shared__ MfmaLds<P, BF16> L;
-    stage_mfma_tile<P, BF16, P == 3>(sp, xcd_tile(blockIdx.x, gridDim.x), L, halo_lds());
+    stage_mfma_tile<P, BF16, P == 3, BF16 || P != 3>(sp, xcd_tile(blockIdx.x, gridDim.x), L, halo_lds());
 }
 
 // Batched instances of one order: each block walks its share of the work items (instance,
@@ -476,7 +528,7 @@ __device__ __forceinline__ void batch_stage_mfma_body(const StageParams* __restr
         const BatchItem item = items[it];
         const StageParams& sp = params[item.instance];
         for (int e0 = item.e0; e0 < item.e0 + item.count; e0 += kTileEl) {
-            stage_mfma_tile<P, false, false>(sp, e0 / kTileEl, L, halo_lds());
+            stage_mfma_tile<P, false, false, true>(sp, e0 / kTileEl, L, halo_lds());
             __syncthreads();  // the next tile overwrites the staged geometry and traces
         }
     }
