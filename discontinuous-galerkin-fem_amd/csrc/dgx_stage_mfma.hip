@@ -186,11 +186,11 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
         if constexpr (PAIR) {
             for (;;) {
                 if (hp < npair) {
-                    const int hk = hp / NFQ, q = hp - hk * NFQ;
+                    const int fk = hp / NFQ, q = hp - fk * NFQ;  // face, point
                     const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
                     const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
-                    // records along the neighbour's outward normal, the negated normal of the reading
-                    // element's face
+                    // records along the neighbour's outward normal, the negated normal of the
+                    // reading element's face
                     const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
                     const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
                     float Un[4] = {ur[0], ur[1], ur[2], ur[3]};
@@ -202,7 +202,7 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
                     }
                     float Fn[4], sw;
                     trace_record(Un, hnx, hny, gam, Fn, sw);
-                    const int k = NREC + hk * NFQ + q;
+                    const int k = NREC + fk * NFQ + q;
                     H.U[k] = make_float4(Un[0], Un[1], Un[2], Un[3]);
                     H.F[k] = make_float4(Fn[0], Fn[1], Fn[2], Fn[3]);
                     H.S[k] = sw;
@@ -225,8 +225,8 @@ __device__ __forceinline__ void stage_mfma_tile(const StageParams& sp, int tile,
                 if (hk < nint) {
                     const int fn = int(hcode & 3u), flip = int((hcode >> 2) & 1u);
                     const float* T = &L.tab[S::T_N * 64 + (fn * 2 + flip) * NP * NFQ];
-                    // records along the neighbour's outward normal, the negated normal of the reading
-                    // element's face
+                    // records along the neighbour's outward normal, the negated normal of the
+                    // reading element's face
                     const int rel = int((hcode >> 5) & 63u), rf = int((hcode >> 3) & 3u);
                     const float hnx = -L.geo[G32_N0X + 2 * rf][rel], hny = -L.geo[G32_N0Y + 2 * rf][rel];
 #pragma unroll
