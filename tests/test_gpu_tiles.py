@@ -1,5 +1,5 @@
-"""The matrix-core tile kernels (FP32 Euler and Navier-Stokes, p = 3 and 4) on element orders
-whose 64-element tiles are not compact patches of the mesh.
+"""The matrix-core tile kernels (FP32 Euler, p = 1 to 4, and Navier-Stokes, p = 3 and 4) on element
+orders whose 64-element tiles are not compact patches of the mesh.
 
 A context numbers its triangles in Morton order of the centroids, so a tile is a compact patch
 with a few dozen faces that leave it.  With DGX_ELEMENT_ORDER=given the context keeps the
@@ -79,6 +79,28 @@ def test_scattered_tiles_residual_matches_oracle(physics, p, case):
     e_given, e_morton = _rel(r_given, r_ref), _rel(r_morton, r_ref)
     assert e_given < 5e-5 and e_morton < 5e-5, (e_given, e_morton)
     assert _rel(r_given, r_morton) < 1e-4
+
+
+@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("case", ["periodic", "mixed"])
+def test_scattered_tiles_low_order_euler(p, case):
+    # At p = 1, 2 the 256 threads of a block take 2 or 3 points per face, so the first pass over
+    # the external face points already reaches faces 64 and on, which are read from the tile's
+    # full list rather than its fixed-stride copy of the first 64.
+    ora, given, morton, u0 = _setup("euler", p, case)
+    assert given.info.stage_kernel == 1 and morton.info.stage_kernel == 1
+    r_ref = ora.residual(u0)
+    r_given, r_morton = given.residual(u0), morton.residual(u0)
+    e_given, e_morton = _rel(r_given, r_ref), _rel(r_morton, r_ref)
+    assert e_given < 5e-5 and e_morton < 5e-5, (e_given, e_morton)
+    assert _rel(r_given, r_morton) < 1e-4
+    morton.set_state(u0)
+    dt = 0.2 * dgx.stable_dt(morton, 0.3)
+    for c in (given, morton):
+        c.set_state(u0)
+        c.rk3_steps(dt, 3)
+    u_ref, _ = ora.rk3(u0, dt, 3)
+    assert _rel(given.get_state(), u_ref) < 5e-5
 
 
 @pytest.mark.parametrize("p", [3, 4])
