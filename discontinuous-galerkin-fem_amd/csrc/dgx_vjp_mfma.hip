@@ -138,18 +138,20 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
     __builtin_amdgcn_sched_barrier(0);
     halo_load(hx);
     // RK stage adjoint: the accumulator acc_out = acc_in + ca lambda and the output's part that
-    // needs no face or volume term, X = cb lambda + add.  LAT: the addend and accumulator are
+    // needs no face or volume term, X = cb lambda + add.  At p = 3 the addend and accumulator are
     // loaded here and used at the output, so the tile's first arithmetic does not wait for them
     // (an absent buffer is a resource of size zero over lambda's memory: its loads read zero
-    // without traffic).  Otherwise X and acc_out are formed here, as soon as lambda is in, to
+    // without traffic).  At p = 4 X and acc_out are formed here, as soon as lambda is in, to
     // keep those registers free.
     const bool rk = sp.mode == MODE_RK;
     const bool has_add = rk && sp.add, has_ain = rk && sp.acc_in;
     const auto r_add = make_rsrc(has_add ? sp.add : sp.lam, has_add ? state_bytes : 0u);
     const auto r_ain = make_rsrc(has_ain ? sp.acc_in : sp.lam, has_ain ? state_bytes : 0u);
     const auto r_aout = make_rsrc(sp.acc_out, sp.acc_out ? state_bytes : 0u);
+    // (at p = 4 the deferred form spills registers)
+    constexpr bool DEFER = LAT || P == 3;
     float xo[4][R];
-    v4u adv[LAT ? R : 1], aiv[LAT ? R : 1];
+    v4u adv[DEFER ? R : 1], aiv[DEFER ? R : 1];
     // X and acc_out of mode slot r from the addend and accumulator words
     auto rk_terms = [&](int r, const v4u& ad, const v4u& ai) {
         const float cb = sp.f.cb, ca = sp.f.ca;
@@ -161,7 +163,7 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
         }
         if (sp.acc_out && valid) __builtin_amdgcn_raw_buffer_store_b128(a, r_aout, slot_off(r, unsigned(ec)), 0, 0);
     };
-    if constexpr (LAT) {
+    if constexpr (DEFER) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             adv[r] = load_x4(r_add, slot_off(r, unsigned(ec)));
@@ -406,7 +408,7 @@ __device__ __forceinline__ void vjp_tile(const VjpParams& sp, int tile, float* _
         for (int t = 0; t < RF; ++t) ub[v] = mfma16x16x4(lt[(S::T_FT + t) * 64], Hb[v][t], ub[v]);
 
     // ---------------- output ---------------------------------------------------------------
-    if constexpr (LAT) {
+    if constexpr (DEFER) {
         if (rk) {
 #pragma unroll
             for (int r = 0; r < R; ++r) rk_terms(r, adv[r], aiv[r]);
