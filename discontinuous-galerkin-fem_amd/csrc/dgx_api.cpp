@@ -1422,7 +1422,7 @@ dgx_status dgx_rk3_steps(dgx_ctx* c, double dt, int32_t nsteps) { return dgx_rk3
 dgx_status dgx_rk3_profile(dgx_ctx* c, double dt, int32_t nsteps, double* stage_ms, double* total_ms) {
     return guarded([&] {
         if (!c || nsteps <= 0) throw std::invalid_argument("bad arguments");
-        // one warm-up step, then events around nsteps back-to-back steps on the context stream:
+        // five warm-up steps, then events around nsteps back-to-back steps on the context stream:
         // the mean stage time is the total over the 3 nsteps launches (a per-launch event pair
         // would add its own record overhead to every launch)
         auto step = [&] {
@@ -1432,7 +1432,7 @@ dgx_status dgx_rk3_profile(dgx_ctx* c, double dt, int32_t nsteps, double* stage_
             for (int k = 0; k < 3; ++k) c->stage_launch(in[k], c->u.p, outb[k], MODE_RK, dt, ca[k], cb[k]);
             std::swap(c->u, c->u1);
         };
-        step();
+        for (int s = 0; s < 5; ++s) step();
         hip_check(hipEventRecord(c->ev0, c->stream), "record");
         for (int s = 0; s < nsteps; ++s) step();
         hip_check(hipEventRecord(c->ev1, c->stream), "record");

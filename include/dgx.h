@@ -260,7 +260,7 @@ dgx_status dgx_rk3_steps(dgx_ctx* ctx, double dt, int32_t nsteps);
 dgx_status dgx_rk3_steps_async(dgx_ctx* ctx, double dt, int32_t nsteps);
 dgx_status dgx_synchronize(dgx_ctx* ctx);
 
-/* Runs one warm-up step and nsteps timed steps and reports the mean duration of one fused
+/* Runs five warm-up steps and nsteps timed steps and reports the mean duration of one fused
  * stage launch (HIP events around the 3 nsteps back-to-back launches on the context stream,
  * divided by their number; ms) and the event time of the whole timed loop (ms). */
 dgx_status dgx_rk3_profile(dgx_ctx* ctx, double dt, int32_t nsteps, double* stage_ms,
