@@ -191,6 +191,7 @@ def main():
                     choices=["euler", "ns", "quad", "adjoint", "batch"],
                     help="euler: the headline vortex case; ns: Taylor-Green Navier-Stokes; quad: the vortex "
                          "on quadrilaterals; adjoint: gradient of the final kinetic energy; batch: 64 mixed instances")
+    ap.add_argument("--no-autotune", action="store_true", help="batch: time the default tuning only")
     args = ap.parse_args()
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
@@ -512,8 +513,9 @@ def batch_rates(steps, autotune):
 
 
 def bench_batch(args):
-    """The 64-instance batch: 100 steps (or --steps), autotuned, vs one instance at a time."""
-    r = batch_rates(100 if args.steps == 200 else args.steps, autotune=True)
+    """The 64-instance batch: 100 steps (or --steps), autotuned (unless --no-autotune), vs one
+    instance at a time."""
+    r = batch_rates(100 if args.steps == 200 else args.steps, autotune=not args.no_autotune)
     _line("batched Euler DOF-stage updates/s (FP32, 64 instances)", r.pop("value"), "DOF-stage updates/s",
           r.pop("steps"), 2, r.pop("ms_per_step"), "f32",
           {"workload": r.pop("workload"), "instances": r.pop("instances"), "elements_total": r.pop("elements_total")},
